@@ -1,0 +1,210 @@
+"""ctypes front end of the CPU oracle (oracle/gspan_oracle.cc).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker / timed CPU baseline, never by the
+product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Callable
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_lib = None
+
+REPORT_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                             ctypes.c_int, ctypes.c_void_p)
+CHILD_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.c_longlong, ctypes.c_void_p)
+
+MNI_GPU = 0
+MNI_OMP_LITERAL = 1
+
+
+def build(force: bool = False) -> None:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "all"], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        c_int, c_ll = ctypes.c_int, ctypes.c_longlong
+        L.oracle_fsm.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, c_int, c_int, c_int,
+                                 c_int, REPORT_CB, ctypes.c_void_p, P(c_ll)]
+        L.oracle_fsm.restype = c_int
+        L.oracle_is_canonical.argtypes = [P(c_int), c_int]
+        L.oracle_is_canonical.restype = c_int
+        L.oracle_support.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, P(c_int), c_int,
+                                     P(P(c_int)), P(c_ll), c_int, c_int]
+        L.oracle_support.restype = c_int
+        L.oracle_extend.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, P(c_int), c_int,
+                                    P(P(c_int)), P(c_ll), c_int, CHILD_CB, ctypes.c_void_p]
+        L.oracle_extend.restype = c_ll
+        L.oracle_seed.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, CHILD_CB,
+                                  ctypes.c_void_p]
+        L.oracle_seed.restype = c_int
+        L.oracle_pattern_graph.argtypes = [P(c_int), c_int, P(c_int), P(c_int), P(c_int),
+                                           P(c_int)]
+        L.oracle_pattern_graph.restype = c_int
+        _lib = L
+    return _lib
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _graph_args(g):
+    return (_ip(g.offsets), _ip(g.vlabels), _ip(g.edges), g.nr_nodes, g.nr_edges)
+
+
+def _code_arr(code) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(code, dtype=np.int32).reshape(-1, 5))
+
+
+def fsm(g, k: int, min_sup: int, mni_mode: int = MNI_GPU, nthreads: int = 0,
+        on_report: Callable | None = None):
+    """Full mining run.  Returns (patterns, stats); patterns = [(code tuple, support)]
+    in the reference's report order."""
+    out = []
+
+    def cb(cp, m, sup, _u):
+        code = tuple(tuple(cp[5 * i + j] for j in range(5)) for i in range(m))
+        out.append((code, sup))
+        if on_report is not None:
+            on_report(code, sup)
+        return 0
+
+    st = (ctypes.c_longlong * 8)()
+    rc = lib().oracle_fsm(*_graph_args(g), k, min_sup, mni_mode, nthreads, REPORT_CB(cb), None,
+                          st)
+    if rc != 0:
+        raise RuntimeError("oracle_fsm failed")
+    names = ["extended", "candidates", "canonical", "reported", "overruns", "t_extend_us",
+             "t_support_us", "t_canon_us"]
+    return out, dict(zip(names, list(st)))
+
+
+def is_canonical(code) -> int:
+    c = _code_arr(code)
+    return lib().oracle_is_canonical(_ip(c), c.shape[0])
+
+
+def _layers(layers):
+    arrs = [np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1, 2)) for x in layers]
+    ptrs = (ctypes.POINTER(ctypes.c_int) * len(arrs))(*[_ip(a) for a in arrs])
+    counts = (ctypes.c_longlong * len(arrs))(*[a.shape[0] for a in arrs])
+    return arrs, ptrs, counts
+
+
+def support(g, code, layers, mni_mode: int = MNI_GPU, nthreads: int = 0) -> int:
+    c = _code_arr(code)
+    arrs, ptrs, counts = _layers(layers)
+    return lib().oracle_support(*_graph_args(g), _ip(c), c.shape[0], ptrs, counts, mni_mode,
+                                nthreads)
+
+
+def extend(g, code, layers, nthreads: int = 0):
+    """One extension step; returns [(child_edge5 tuple, rows int32[count, 2])] in the
+    reference's recursion order."""
+    c = _code_arr(code)
+    arrs, ptrs, counts = _layers(layers)
+    out = []
+
+    def cb(ep, rp, cnt, _u):
+        e5 = tuple(ep[j] for j in range(5))
+        rows = np.ctypeslib.as_array(rp, shape=(cnt * 2,)).reshape(cnt, 2).copy() if cnt else \
+            np.zeros((0, 2), np.int32)
+        out.append((e5, rows))
+
+    total = lib().oracle_extend(*_graph_args(g), _ip(c), c.shape[0], ptrs, counts, nthreads,
+                                CHILD_CB(cb), None)
+    if total < 0:
+        raise RuntimeError("oracle_extend failed")
+    return out
+
+
+def seed(g):
+    """Single-edge codes and their rows: [(edge5, rows int32[count, 2])]."""
+    out = []
+
+    def cb(ep, rp, cnt, _u):
+        e5 = tuple(ep[j] for j in range(5))
+        rows = np.ctypeslib.as_array(rp, shape=(cnt * 2,)).reshape(cnt, 2).copy()
+        out.append((e5, rows))
+
+    lib().oracle_seed(*_graph_args(g), CHILD_CB(cb), None)
+    return out
+
+
+def pattern_graph(code):
+    c = _code_arr(code)
+    dims = (ctypes.c_int * 2)()
+    lib().oracle_pattern_graph(_ip(c), c.shape[0], dims, None, None, None)
+    n, nnz = dims[0], dims[1]
+    off = np.zeros(n + 1, np.int32)
+    vl = np.zeros(n, np.int32)
+    ed = np.zeros((nnz, 3), np.int32)
+    lib().oracle_pattern_graph(_ip(c), c.shape[0], dims, _ip(off), _ip(vl), _ip(ed))
+    return off, vl, ed
+
+
+def format_report(patterns) -> str:
+    """The reference's stdout lines (gspan.cu:67-77)."""
+    lines = []
+    for code, sup in patterns:
+        lines.append("".join("(%d %d %d %d %d)" % e for e in code) + ": %d" % sup)
+    return "\n".join(lines) + ("\n" if lines else "")
+
+
+# ---------------------------------------------------------------- reference build (_ref)
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(REF_DIR, "convert")) and \
+        os.path.exists(os.path.join(REF_DIR, "libref_graph.so"))
+
+
+_ref_lib = None
+
+
+def ref_lib():
+    global _ref_lib
+    if _ref_lib is None:
+        L = ctypes.CDLL(os.path.join(REF_DIR, "libref_graph.so"))
+        P = ctypes.POINTER
+        L.ref_pattern_graph.argtypes = [P(ctypes.c_int), ctypes.c_int, P(ctypes.c_int),
+                                        P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]
+        L.ref_prune_file.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p]
+        _ref_lib = L
+    return _ref_lib
+
+
+def ref_pattern_graph(code):
+    c = _code_arr(code)
+    L = ref_lib()
+    dims = (ctypes.c_int * 2)()
+    L.ref_pattern_graph(_ip(c), c.shape[0], dims, None, None, None)
+    n, nnz = dims[0], dims[1]
+    off = np.zeros(n + 1, np.int32)
+    vl = np.zeros(n, np.int32)
+    ed = np.zeros((nnz, 3), np.int32)
+    L.ref_pattern_graph(_ip(c), c.shape[0], dims, _ip(off), _ip(vl), _ip(ed))
+    return off, vl, ed
+
+
+def ref_convert(txt_path: str, bin_path: str) -> str:
+    r = subprocess.run([os.path.join(REF_DIR, "convert"), txt_path, bin_path], check=True,
+                       capture_output=True, text=True)
+    return r.stdout
