@@ -1,0 +1,116 @@
+"""ctypes binding of libgspan_amd.so (C-ABI declared in include/gspan_amd.h).
+
+The library is built in-tree by ``gspan_amd.build`` (``__graft_entry__.build()``).  There
+is no fallback: if the shared library is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "lib", "libgspan_amd.so")
+HEADER = os.path.join(ROOT, "include", "gspan_amd.h")
+
+GS_OK = 0
+GS_ERR_INVALID = 1
+GS_ERR_HIP = 2
+GS_ERR_BUDGET = 3
+GS_ERR_LIMIT = 4
+GS_ERR_CALLBACK = 5
+GS_ERR_INTERNAL = 6
+
+GS_MODE_PLAIN = 0
+GS_MODE_BATCHED = 1
+GS_MNI_GPU = 0
+GS_MNI_OMP_LITERAL = 1
+GS_MAX_PATTERN_EDGES = 7
+
+
+class GsOpts(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int),
+        ("mni_mode", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("timing", ctypes.c_int),
+        ("budget_bytes", ctypes.c_ulonglong),
+        ("chunk_rows", ctypes.c_longlong),
+    ]
+
+
+class GsStats(ctypes.Structure):
+    _fields_ = [
+        ("extended", ctypes.c_longlong),
+        ("candidates", ctypes.c_longlong),
+        ("canonical", ctypes.c_longlong),
+        ("reported", ctypes.c_longlong),
+        ("batches", ctypes.c_longlong),
+        ("items", ctypes.c_longlong),
+        ("parent_rows", ctypes.c_longlong),
+        ("fill_launches", ctypes.c_longlong),
+        ("count_launches", ctypes.c_longlong),
+        ("alg_bytes_fill", ctypes.c_double),
+        ("t_fill_ms", ctypes.c_double),
+        ("t_count_ms", ctypes.c_double),
+        ("t_mine_ms", ctypes.c_double),
+        ("peak_device_bytes", ctypes.c_ulonglong),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+REPORT_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                             ctypes.c_int, ctypes.c_void_p)
+CHILD_CB = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                            ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+
+_lib = None
+
+
+class GsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[gspan_amd error {code}] {msg}")
+        self.code = code
+
+
+def declared_symbols() -> list[str]:
+    """Names of the functions declared in include/gspan_amd.h."""
+    with open(HEADER) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(gs_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    """Loads libgspan_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    c_int, c_ll, vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p
+    L.gs_version.restype = ctypes.c_char_p
+    L.gs_version.argtypes = []
+    L.gs_default_opts.argtypes = [P(GsOpts)]
+    L.gs_create.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, P(GsOpts), P(vp)]
+    L.gs_mine.argtypes = [vp, c_int, c_int, REPORT_CB, vp]
+    L.gs_support.argtypes = [vp, P(c_int), c_int, P(P(c_int)), P(c_ll), c_int, P(c_int)]
+    L.gs_extend.argtypes = [vp, P(c_int), c_int, P(P(c_int)), P(c_ll), c_int, CHILD_CB, vp,
+                            P(c_ll)]
+    L.gs_is_canonical_batch.argtypes = [vp, P(c_int), P(c_int), c_int, P(c_int)]
+    L.gs_get_stats.argtypes = [vp, P(GsStats)]
+    L.gs_reset_stats.argtypes = [vp]
+    L.gs_last_error.argtypes = [vp]
+    L.gs_last_error.restype = ctypes.c_char_p
+    L.gs_destroy.argtypes = [vp]
+    for name in ("gs_default_opts", "gs_create", "gs_mine", "gs_support", "gs_extend",
+                 "gs_is_canonical_batch", "gs_get_stats", "gs_reset_stats", "gs_destroy"):
+        getattr(L, name).restype = c_int
+    _lib = L
+    return L

@@ -1,0 +1,117 @@
+// gs_internal.h -- types shared by the gfx950 kernels (gs_kernels.hip) and the host
+// driver (gs_miner.cpp).  Not part of the public C-ABI (include/gspan_amd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+// Largest pattern the kernels handle: 7 edges, 8 vertices.  Projections in the
+// canonicality kernel pack 8 three-bit vertex ids and an 8-bit used-edge mask into 32 bits.
+#define GS_MAXE 7
+#define GS_MAXV 8
+
+// Threads per workgroup of the item kernels; items per workgroup tile.
+#define GS_BLOCK 256
+#define GS_TILE 1024
+
+// Canonicality kernel: one wave per candidate, projections double-buffered in LDS.
+#define GS_CANON_CAP 2048
+
+// Set kinds for MNI image sets.
+#define GS_SET_BITMAP 0
+#define GS_SET_HASH 1
+
+// One pattern of a batch, as the extension kernels see it.  Labels are dense ranks
+// (order-preserving), so every comparison of the reference's rules is unchanged.
+struct PatDev {
+    int m;          // edges
+    int nv;         // vertices (= v_n + 1)
+    int r;          // rightmost-path length
+    int v_n;        // rightmost vertex
+    int min_label;  // code[0].from_label rank (gspan.cu:618)
+    int key_base;   // first child key of this pattern in the batch key space
+    int cfrom[GS_MAXE], cto[GS_MAXE], cel[GS_MAXE];
+    int vlab[GS_MAXV];  // label rank of every pattern vertex
+    int rm[GS_MAXE];    // rightmost path (code edge indices, last forward edge first)
+    long long row_begin;   // first row in the parent level array
+    long long row_count;   // rows of this pattern
+    long long brow_begin;  // first row in the batch row space
+};
+
+// Per parent row context written by the prep kernel and read by the item kernels.
+// phi: image of every pattern vertex; fcum[j]: forward items through segment j
+// (segment 0 = neighbours of u_n, segment i+1 = neighbours of rmpath[i].from).
+struct RowCtx {
+    int phi[GS_MAXV];
+    int p;  // pattern index in the batch
+    int fcum[GS_MAXE];
+};
+
+struct LayerPtrs {
+    const int2* layer[GS_MAXE];  // layer[l] = rows of code edge l: {prev, eid}
+};
+
+// MNI set descriptor of one child key that survived canonicality.
+struct SetDesc {
+    long long off;  // first u32 word in the set pool
+    int kind;       // GS_SET_BITMAP / GS_SET_HASH
+    int size;       // words per vertex set (bitmap) or slots (hash, power of two)
+    int nv;         // vertices of the child pattern
+    int pad;
+};
+
+// Batch-wide key decode: key = key_base[p] + slot;
+//   slot <  (r-1)*EL                    : backward, rmpath index i = slot/EL + 1, el = slot%EL
+//   slot >= (r-1)*EL, s' = slot-(r-1)*EL: forward, j = s'/(EL*VL), el = (s'/VL)%EL, tl = s'%VL
+// j = 0: from v_n (first forward); j = i+1: from code[rm[i]].from.
+struct KeySpace {
+    int EL, VL;
+};
+
+struct DevGraph {
+    int n, nnz;
+    const int* offsets;  // n+1
+    const int* src;      // nnz: Edge.from
+    const uint2* adj;    // nnz: x = to, y = (el_rank << 16) | tl_rank
+    const int* vlab;     // n
+};
+
+// ------------------------------------------------------------ kernel launchers
+// All launches go to `stream`; every size argument is checked by the caller.
+namespace gsk {
+
+hipError_t seed_count(const DevGraph& g, int EL, int VL, int* key_count, long long* total,
+                      hipStream_t s);
+hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int* cursor,
+                     int2* rows_out, const SetDesc* sets, uint32_t* pool, hipStream_t s);
+
+hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
+                LayerPtrs L, RowCtx* ctx, int* work, hipStream_t s);
+hipError_t scan_exclusive(const int* in, long long* out, long long n, long long* tmp,
+                          long long tmp_elems, hipStream_t s);
+long long scan_tmp_elems(long long n);
+
+hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
+                       const long long* woff, long long b0, long long R, KeySpace ks,
+                       int* key_count, int* tile_counter, int grid, hipStream_t s);
+hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
+                      const long long* woff, long long b0, long long R, KeySpace ks,
+                      const int* key_slot, int* cursor, int2* rows_out, const SetDesc* sets,
+                      uint32_t* pool, int* tile_counter, int grid, hipStream_t s);
+int items_grid(int cus);
+
+hipError_t compact_active(const int* key_count, long long K, int* active_keys,
+                          int* active_counts, int* n_active, hipStream_t s);
+hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_keys,
+                 const int* n_active, int max_active, int* flags, int* err, hipStream_t s);
+hipError_t canon_codes(const int* codes, int ncodes, int* flags, int* err, hipStream_t s);
+
+hipError_t fill_i32(int* p, int v, long long n, hipStream_t s);
+hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
+hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s);
+hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool, int* counts,
+                      hipStream_t s);
+hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
+                        int mni_mode, const SetDesc* set, uint32_t* pool, hipStream_t s);
+
+}  // namespace gsk

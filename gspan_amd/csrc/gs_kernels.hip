@@ -1,0 +1,998 @@
+// gs_kernels.hip -- gfx950 kernels of the gSpan hot path.
+//
+//   seed_*        single-edge codes (gspan.cu:410-426, 1176-1180)
+//   prep          per parent row: rebuild the vertex images from the layered rows
+//                 (gspan.cu:385-394) and size its extension work
+//   items_count   rightmost-path extension, pass 1: every (row, candidate entry) pair is
+//                 one work item; emitted children are counted per child key
+//                 (gspan.cu:694-735, rules gspan.cu:428-483)
+//   items_fill    pass 2: children of canonical keys are written grouped by key and their
+//                 vertex images are inserted into the MNI sets (gspan.cu:165-190)
+//   canon         minimum-DFS-code check, one wave per candidate (gspan.cu:485-606)
+//   set_counts    distinct images per (key, pattern vertex)
+//
+// Work distribution: rows are flattened into items by an exclusive scan of per-row
+// work; persistent workgroups steal fixed tiles of items from an atomic counter and
+// locate their rows by a search over the scanned offsets staged in LDS.  Output slots
+// come from one atomic per (wave, distinct key) instead of one per child.
+#include "gs_internal.h"
+
+namespace {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned mbcnt64(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Adds 1 per active lane to counters[key] with one atomic per distinct key in the wave.
+// Returns, for active lanes, the counter value reserved for that lane.  Must be reached
+// by all lanes of the wave.
+__device__ __forceinline__ int wave_reserve(int* counters, int key, bool active) {
+    bool pending = active;
+    int result = 0;
+    for (;;) {
+        unsigned long long pm = __ballot(pending);
+        if (pm == 0) break;
+        int leader = __ffsll((long long)pm) - 1;
+        int k = __shfl(key, leader);
+        bool mine = pending && key == k;
+        unsigned long long gm = __ballot(mine);
+        int base = 0;
+        if (lane_id() == leader) base = atomicAdd(&counters[k], __popcll(gm));
+        base = __shfl(base, leader);
+        if (mine) {
+            result = base + (int)mbcnt64(gm);
+            pending = false;
+        }
+    }
+    return result;
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    // multiplicative (Fibonacci) hashing followed by a xor-shift to mix the high bits
+    x *= 0x9E3779B1u;
+    return x ^ (x >> 15);
+}
+
+// Inserts graph vertex x into set v of descriptor d.  Bits / slots only go from empty
+// to full, so a stale plain read can only send us to the atomic, never skip a new key.
+__device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int v, uint32_t x) {
+    if (d.kind == GS_SET_BITMAP) {
+        uint32_t* w = pool + d.off + (long long)v * d.size + (x >> 5);
+        uint32_t bit = 1u << (x & 31);
+        if (!(*w & bit)) atomicOr(w, bit);
+    } else {
+        uint32_t* base = pool + d.off + (long long)v * d.size;
+        uint32_t mask = (uint32_t)d.size - 1;
+        uint32_t key = x + 1;  // 0 marks an empty slot
+        uint32_t h = hash32(x) & mask;
+        for (;;) {
+            uint32_t cur = __hip_atomic_load(base + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key) return;
+            if (cur == 0) {
+                uint32_t prev = atomicCAS(base + h, 0u, key);
+                if (prev == 0 || prev == key) return;
+            }
+            h = (h + 1) & mask;
+        }
+    }
+}
+
+// has_edge (gspan.cu:396-401) on a row given by its vertex images: some pattern edge
+// with label el joins u and w.
+__device__ __forceinline__ bool row_has_edge(const PatDev& P, const int* phi, int u, int w,
+                                             int el) {
+    for (int j = 0; j < P.m; ++j) {
+        if (P.cel[j] != el) continue;
+        int a = phi[P.cfrom[j]], b = phi[P.cto[j]];
+        if ((a == u && b == w) || (a == w && b == u)) return true;
+    }
+    return false;
+}
+
+struct ItemOut {
+    bool valid;
+    int key;
+    int eid;
+    int newv;  // image of the new vertex (forward) or -1 (backward)
+};
+
+// One work item of a parent row: item q < r-1 is the backward probe of rmpath index
+// q+1 (gspan.cu:428-445); the rest walk the neighbours of u_n (first forward,
+// gspan.cu:447-462) and of every rightmost-path source (other forward, gspan.cu:464-483).
+__device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowCtx& c,
+                                             long long q, KeySpace ks) {
+    ItemOut o{false, 0, 0, -1};
+    const int r = P.r;
+    if (q < r - 1) {
+        const int i = (int)q + 1;
+        const int k = P.rm[i];
+        const int u_n = c.phi[P.v_n];
+        const int target = c.phi[P.cfrom[k]];
+        const int e_el = P.cel[k];
+        const bool le = P.vlab[P.cto[k]] <= P.vlab[P.v_n];
+        int lo = g.offsets[u_n];
+        const int end = g.offsets[u_n + 1];
+        int hi = end;
+        while (lo < hi) {  // first entry with to >= target; adjacency sorted by (to, label)
+            int mid = (lo + hi) >> 1;
+            if ((int)g.adj[mid].x < target) lo = mid + 1;
+            else hi = mid;
+        }
+        for (int e = lo; e < end; ++e) {
+            uint2 a = g.adj[e];
+            if ((int)a.x != target) break;
+            int el = (int)(a.y >> 16);
+            if (row_has_edge(P, c.phi, u_n, target, el)) continue;
+            if (e_el < el || (e_el == el && le)) {
+                o.valid = true;
+                o.eid = e;
+                o.key = P.key_base + (i - 1) * ks.EL + el;
+                break;
+            }
+        }
+        return o;
+    }
+    const long long f = q - (r - 1);
+    int j = 0;
+    while (j < r && f >= c.fcum[j]) ++j;
+    const int u = (j == 0) ? c.phi[P.v_n] : c.phi[P.cfrom[P.rm[j - 1]]];
+    const int e = g.offsets[u] + (int)(f - (j ? c.fcum[j - 1] : 0));
+    const uint2 a = g.adj[e];
+    const int to = (int)a.x, el = (int)(a.y >> 16), tl = (int)(a.y & 0xffffu);
+    if (tl < P.min_label) return o;
+    for (int v = 0; v < P.nv; ++v)
+        if (c.phi[v] == to) return o;
+    if (j > 0) {
+        const int k = P.rm[j - 1];
+        const int e_el = P.cel[k];
+        if (!(e_el < el || (e_el == el && P.vlab[P.cto[k]] <= tl))) return o;
+    }
+    o.valid = true;
+    o.eid = e;
+    o.newv = to;
+    o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
+    return o;
+}
+
+// ------------------------------------------------------------------ seeds
+
+__global__ __launch_bounds__(GS_BLOCK) void k_seed_count(DevGraph g, int EL, int VL,
+                                                         int* key_count, long long* total) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    long long local = 0;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < g.nnz; base += stride) {
+        long long i = base + threadIdx.x;
+        bool valid = false;
+        int key = 0;
+        if (i < g.nnz) {
+            int fl = g.vlab[g.src[i]];
+            uint2 a = g.adj[i];
+            int tl = (int)(a.y & 0xffffu), el = (int)(a.y >> 16);
+            valid = fl <= tl;  // gspan.cu:421
+            key = (fl * EL + el) * VL + tl;
+        }
+        local += valid;
+        wave_reserve(key_count, key, valid);
+    }
+    // wave reduction of the emitted count, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+    if (lane_id() == 0 && local) atomicAdd((unsigned long long*)total, (unsigned long long)local);
+}
+
+__global__ __launch_bounds__(GS_BLOCK) void k_seed_fill(DevGraph g, int EL, int VL,
+                                                        const int* key_slot, int* cursor,
+                                                        int2* rows_out, const SetDesc* sets,
+                                                        uint32_t* pool) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < g.nnz; base += stride) {
+        long long i = base + threadIdx.x;
+        bool valid = false;
+        int slot = -1, from = 0, to = 0;
+        if (i < g.nnz) {
+            from = g.src[i];
+            int fl = g.vlab[from];
+            uint2 a = g.adj[i];
+            to = (int)a.x;
+            int tl = (int)(a.y & 0xffffu), el = (int)(a.y >> 16);
+            if (fl <= tl) {
+                slot = key_slot[(fl * EL + el) * VL + tl];
+                valid = slot >= 0;
+            }
+        }
+        int pos = 0;
+        if (rows_out) pos = wave_reserve(cursor, slot, valid);
+        if (valid) {
+            if (rows_out) rows_out[pos] = make_int2(-1, (int)i);
+            const SetDesc d = sets[slot];
+            set_insert(d, pool, 0, (uint32_t)from);
+            set_insert(d, pool, 1, (uint32_t)to);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ prep
+
+__global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __restrict__ pats,
+                                                   int npats, long long b0, long long R, LayerPtrs L,
+                                                   RowCtx* __restrict__ ctx, int* __restrict__ work) {
+    const long long lb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lb >= R) return;
+    const long long b = b0 + lb;  // row in the batch row space
+    int lo = 0, hi = npats - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pats[mid].brow_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const PatDev& P = pats[lo];
+    RowCtx c;
+#pragma unroll
+    for (int v = 0; v < GS_MAXV; ++v) c.phi[v] = -1;
+    long long idx = P.row_begin + (b - P.brow_begin);
+    for (int l = P.m - 1; l >= 0; --l) {
+        int2 rr = L.layer[l][idx];
+        if (P.cfrom[l] < P.cto[l]) c.phi[P.cto[l]] = (int)g.adj[rr.y].x;
+        if (l == 0) c.phi[0] = g.src[rr.y];
+        idx = rr.x;
+    }
+    c.p = lo;
+    int fc = 0;
+#pragma unroll
+    for (int j = 0; j < GS_MAXE; ++j) {
+        if (j <= P.r) {
+            int u = (j == 0) ? c.phi[P.v_n] : c.phi[P.cfrom[P.rm[j - 1]]];
+            fc += g.offsets[u + 1] - g.offsets[u];
+        }
+        c.fcum[j] = fc;
+    }
+    ctx[lb] = c;
+    work[lb] = (P.r - 1) + fc;
+}
+
+// ------------------------------------------------------------------ scan (int32 -> int64)
+
+constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_TILE = GS_BLOCK * SCAN_PER_THREAD;
+
+__device__ long long block_exclusive_scan(long long v, long long* total) {
+    __shared__ long long s_wave[GS_BLOCK / 64];
+    const int lane = lane_id(), wid = threadIdx.x / 64;
+    long long x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        long long y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    long long wbase = 0, tot = 0;
+    for (int w = 0; w < GS_BLOCK / 64; ++w) {
+        if (w < wid) wbase += s_wave[w];
+        tot += s_wave[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return wbase + x - v;
+}
+
+__global__ __launch_bounds__(GS_BLOCK) void k_scan_reduce(const int* in, long long n,
+                                                          long long* partial) {
+    long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_PER_THREAD;
+    long long s = 0;
+    for (int i = 0; i < SCAN_PER_THREAD; ++i)
+        if (base + i < n) s += in[base + i];
+    long long tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(GS_BLOCK) void k_scan_top(long long* partial, long long nb) {
+    long long carry = 0;
+    for (long long base = 0; base < nb; base += GS_BLOCK) {
+        long long i = base + threadIdx.x;
+        long long v = i < nb ? partial[i] : 0;
+        long long tot;
+        long long ex = block_exclusive_scan(v, &tot);
+        if (i < nb) partial[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) partial[nb] = carry;
+}
+
+__global__ __launch_bounds__(GS_BLOCK) void k_scan_down(const int* in, long long n,
+                                                        const long long* partial, long long nb,
+                                                        long long* out) {
+    long long base = (long long)blockIdx.x * SCAN_TILE + (long long)threadIdx.x * SCAN_PER_THREAD;
+    int vals[SCAN_PER_THREAD];
+    long long s = 0;
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+        vals[i] = (base + i < n) ? in[base + i] : 0;
+        s += vals[i];
+    }
+    long long tot;
+    long long ex = block_exclusive_scan(s, &tot) + partial[blockIdx.x];
+    for (int i = 0; i < SCAN_PER_THREAD; ++i) {
+        if (base + i < n) out[base + i] = ex;
+        ex += vals[i];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = partial[nb];
+}
+
+// ------------------------------------------------------------------ extension items
+
+template <bool FILL>
+__global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
+                                                    const RowCtx* __restrict__ ctx,
+                                                    const long long* __restrict__ woff,
+                                                    long long b0, long long R, KeySpace ks,
+                                                    int* key_count,
+                                                    const int* __restrict__ key_slot, int* cursor,
+                                                    int2* rows_out, const SetDesc* __restrict__ sets,
+                                                    uint32_t* pool, int* tile_counter) {
+    __shared__ long long s_off[GS_TILE + 1];
+    __shared__ long long s_tile, s_rlo;
+    if (R <= 0) return;
+    const long long W = woff[R];
+    const long long ntiles = (W + GS_TILE - 1) / GS_TILE;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1);
+        __syncthreads();
+        const long long tile = s_tile;
+        if (tile >= ntiles) break;
+        const long long t0 = tile * GS_TILE;
+        const long long t1 = min(t0 + GS_TILE, W);
+        if (threadIdx.x == 0) {  // row holding item t0
+            long long lo = 0, hi = R - 1;
+            while (lo < hi) {
+                long long mid = (lo + hi + 1) >> 1;
+                if (woff[mid] <= t0) lo = mid;
+                else hi = mid - 1;
+            }
+            s_rlo = lo;
+        }
+        __syncthreads();
+        const long long rlo = s_rlo;
+        // every row has >= 1 item (u_n always has a neighbour), so a tile spans at most
+        // GS_TILE rows
+        const int nrows = (int)min((long long)GS_TILE, R - rlo);
+        for (int i = threadIdx.x; i <= nrows; i += blockDim.x) s_off[i] = woff[rlo + i];
+        __syncthreads();
+        for (long long base = t0; base < t1; base += blockDim.x) {
+            const long long t = base + threadIdx.x;
+            ItemOut o{false, 0, 0, -1};
+            int lo = 0;
+            if (t < t1) {
+                int hi = nrows - 1;
+                while (lo < hi) {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const RowCtx& c = ctx[rlo + lo];
+                o = eval_item(g, pats[c.p], c, t - s_off[lo], ks);
+            }
+            if (!FILL) {
+                wave_reserve(key_count, o.key, o.valid);
+            } else {
+                int slot = o.valid ? key_slot[o.key] : -1;
+                bool live = slot >= 0;
+                int pos = 0;
+                if (rows_out) pos = wave_reserve(cursor, slot, live);
+                if (live) {
+                    const RowCtx& c = ctx[rlo + lo];
+                    const PatDev& P = pats[c.p];
+                    if (rows_out) {
+                        long long prow = P.row_begin + (b0 + rlo + lo - P.brow_begin);
+                        rows_out[pos] = make_int2((int)prow, o.eid);
+                    }
+                    const SetDesc d = sets[slot];
+                    for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)c.phi[v]);
+                    if (o.newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o.newv);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ keys
+
+__global__ __launch_bounds__(GS_BLOCK) void k_compact_active(const int* key_count, long long K,
+                                                             int* active, int* active_counts,
+                                                             int* n_active) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x; base < K; base += stride) {
+        long long i = base + threadIdx.x;
+        bool a = i < K && key_count[i] > 0;
+        unsigned long long m = __ballot(a);
+        int wbase = 0;
+        int leader = 0;
+        if (m) {
+            leader = __ffsll((long long)m) - 1;
+            if (lane_id() == leader) wbase = atomicAdd(n_active, __popcll(m));
+            wbase = __shfl(wbase, leader);
+        }
+        if (a) {
+            const int at = wbase + (int)mbcnt64(m);
+            active[at] = (int)i;
+            active_counts[at] = key_count[i];
+        }
+    }
+}
+
+__global__ void k_scatter_slots(int* key_slot, const int* keys, int nkeys) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nkeys) key_slot[keys[i]] = i;
+}
+
+__global__ void k_fill_i32(int* p, int v, long long n) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    long long n4 = n / 4;
+    uint4* p4 = reinterpret_cast<uint4*>(p);
+    uint4 v4 = make_uint4(v, v, v, v);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) p4[i] = v4;
+    for (long long i = n4 * 4 + (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = v;
+}
+
+// distinct images per (set, vertex): popcount of a bitmap / non-empty hash slots
+__global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets,
+                                                         const uint32_t* pool, int* counts) {
+    const int s = blockIdx.x / GS_MAXV, v = blockIdx.x % GS_MAXV;
+    const SetDesc d = sets[s];
+    __shared__ int s_red[GS_BLOCK / 64];
+    int c = 0;
+    if (v < d.nv) {
+        const uint32_t* base = pool + d.off + (long long)v * d.size;
+        if (d.kind == GS_SET_BITMAP) {
+            for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += __popc(base[i]);
+        } else {
+            for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += base[i] != 0;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane_id() == 0) s_red[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < GS_BLOCK / 64; ++w) t += s_red[w];
+        counts[blockIdx.x] = v < d.nv ? t : 0;
+    }
+}
+
+// Support of one pattern straight from its layered rows (gspan.cu:165-190): the images of
+// every pattern vertex go into the pattern's sets.
+__global__ __launch_bounds__(GS_BLOCK) void k_support_rows(DevGraph g, const PatDev* pat,
+                                                           long long N, LayerPtrs L,
+                                                           const SetDesc* set, uint32_t* pool) {
+    long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    const PatDev& P = *pat;
+    int phi[GS_MAXV];
+#pragma unroll
+    for (int v = 0; v < GS_MAXV; ++v) phi[v] = -1;
+    long long idx = b;
+    for (int l = P.m - 1; l >= 0; --l) {
+        int2 rr = L.layer[l][idx];
+        if (P.cfrom[l] < P.cto[l]) phi[P.cto[l]] = (int)g.adj[rr.y].x;
+        if (l == 0) phi[0] = g.src[rr.y];
+        idx = rr.x;
+    }
+    const SetDesc d = *set;
+    for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)phi[v]);
+}
+
+// ------------------------------------------------------------------ canonicality
+// A candidate code lives in LDS as cf/ct/cel/ctl (ctl = -1 for backward edges) plus the
+// label of every pattern vertex.  The pattern graph is the candidate itself
+// (graph.cc:122-159): the adjacency of x lists the code edges touching x in reverse code
+// order.  A projection packs the image of code vertex c in bits [3c, 3c+3) and the set of
+// used pattern edges in bits [24, 32).
+
+struct CanonCode {
+    int m;
+    int cf[GS_MAXE], ct[GS_MAXE], cel[GS_MAXE], ctl[GS_MAXE];
+    int lab[GS_MAXV];
+};
+
+__device__ __forceinline__ int pj_phi(uint32_t p, int c) { return (int)((p >> (3 * c)) & 7u); }
+
+// has_edge (gspan.cu:396-401) on a projection: a used pattern edge with label l joins u, w
+__device__ __forceinline__ bool pj_has_edge(const CanonCode& C, uint32_t p, int u, int w, int l) {
+    uint32_t used = p >> 24;
+    while (used) {
+        int j = __builtin_ctz(used);
+        used &= used - 1;
+        if (C.cel[j] == l &&
+            ((C.cf[j] == u && C.ct[j] == w) || (C.cf[j] == w && C.ct[j] == u)))
+            return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool pj_has_vertex(uint32_t p, int nvm, int w) {
+    for (int c = 0; c < nvm; ++c)
+        if (pj_phi(p, c) == w) return true;
+    return false;
+}
+
+__device__ __forceinline__ long long wave_min(long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        long long o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+constexpr long long KEY_NONE = 0x7fffffffffffffffll;
+
+__device__ __forceinline__ int wave_excl_sum(int v, int* total) {
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off);
+        if (lane_id() >= off) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+// Backward probe of one projection at rightmost-path position k (gspan.cu:428-445 on the
+// pattern graph).  Returns the edge label or -1; *je = pattern edge used.
+__device__ __forceinline__ int canon_backward(const CanonCode& C, uint32_t p, int mf_k, int mt_k,
+                                              int mel_k, int v_n, int* je) {
+    const int u_n = pj_phi(p, v_n);
+    const int e_from = pj_phi(p, mf_k), e_to = pj_phi(p, mt_k);
+    for (int j = C.m - 1; j >= 0; --j) {
+        int w;
+        if (C.cf[j] == u_n) w = C.ct[j];
+        else if (C.ct[j] == u_n) w = C.cf[j];
+        else continue;
+        const int l = C.cel[j];
+        if (pj_has_edge(C, p, u_n, w, l)) continue;
+        if (w == e_from && (mel_k < l || (mel_k == l && C.lab[e_to] <= C.lab[u_n]))) {
+            *je = j;
+            return l;
+        }
+    }
+    return -1;
+}
+
+// Returns 1 canonical, 0 not; err bit 0: projection buffer overflow, bit 1: walk overran.
+__device__ int canon_wave(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, int* mf, int* mt,
+                          int* mel, int* err) {
+    const int lane = lane_id();
+    const int m = C.m;
+    if (m == 1) return 1;  // gspan.cu:592-593
+    // initial edge: min (from_label, edge_label, to_label) over entries with lab[from] <= lab[to]
+    long long key = KEY_NONE;
+    int x = 0, y = 0, jj = 0;
+    if (lane < 2 * m) {
+        jj = lane >> 1;
+        x = (lane & 1) ? C.ct[jj] : C.cf[jj];
+        y = (lane & 1) ? C.cf[jj] : C.ct[jj];
+        if (C.lab[x] <= C.lab[y])
+            key = ((long long)C.lab[x] << 32) | ((long long)C.cel[jj] << 16) | C.lab[y];
+    }
+    const long long kmin = wave_min(key);
+    bool take = lane < 2 * m && key == kmin;
+    unsigned long long tm = __ballot(take);
+    if (take) bufA[mbcnt64(tm)] = (uint32_t)x | ((uint32_t)y << 3) | (1u << (24 + jj));
+    int P = __popcll(tm);
+    uint32_t* cur = bufA;
+    uint32_t* nxt = bufB;
+    int len = 1;
+    if (lane == 0) {
+        mf[0] = 0;
+        mt[0] = 1;
+        mel[0] = (int)((kmin >> 16) & 0xffff);
+    }
+    const int min_label = (int)(kmin >> 32);
+    __syncthreads();
+    for (;;) {
+        // rightmost path of the minimum code so far (gspan.cu:53-65)
+        int rm[GS_MAXE];
+        int r = 0, last = -1;
+        for (int i = len - 1; i >= 0; --i)
+            if (mf[i] < mt[i] && (r == 0 || mt[i] == last)) {
+                rm[r++] = i;
+                last = mf[i];
+            }
+        const int v_n = mt[rm[0]];
+        const int nvm = v_n + 1;
+
+        // backward (gspan.cu:494-524)
+        int bi = -1, bl = 0x7fffffff;
+        for (int i = r - 1; i >= 1 && bi < 0; --i) {
+            const int k = rm[i];
+            long long best = KEY_NONE;
+            for (int base = 0; base < P; base += 64) {
+                int pi = base + lane;
+                if (pi < P) {
+                    int je;
+                    int l = canon_backward(C, cur[pi], mf[k], mt[k], mel[k], v_n, &je);
+                    if (l >= 0 && l < best) best = l;
+                }
+            }
+            best = wave_min(best);
+            if (best != KEY_NONE) {
+                bi = i;
+                bl = (int)best;
+            }
+        }
+        if (bi >= 0) {
+            const int k = rm[bi];
+            if (len >= m) {
+                *err |= 2;
+                return 1;
+            }
+            if (!(C.cf[len] == v_n && C.ct[len] == mf[k] && C.cel[len] == bl && C.ctl[len] < 0))
+                return 0;
+            int np = 0;
+            for (int base = 0; base < P; base += 64) {
+                int pi = base + lane;
+                bool keep = false;
+                uint32_t pr = 0;
+                if (pi < P) {
+                    int je;
+                    pr = cur[pi];
+                    int l = canon_backward(C, pr, mf[k], mt[k], mel[k], v_n, &je);
+                    if (l == bl) {
+                        keep = true;
+                        pr |= 1u << (24 + je);
+                    }
+                }
+                unsigned long long km = __ballot(keep);
+                if (keep) {
+                    int at = np + (int)mbcnt64(km);
+                    if (at < GS_CANON_CAP) nxt[at] = pr;
+                }
+                np += __popcll(km);
+            }
+            if (np > GS_CANON_CAP) {
+                *err |= 1;
+                return 1;
+            }
+            __syncthreads();
+            if (lane == 0) {
+                mf[len] = v_n;
+                mt[len] = mf[k];
+                mel[len] = bl;
+            }
+            ++len;
+            uint32_t* t = cur;
+            cur = nxt;
+            nxt = t;
+            P = np;
+            __syncthreads();
+            continue;
+        }
+
+        // forward (gspan.cu:526-581): first forward from v_n, else other forward from the
+        // rightmost path, deepest source first
+        int seg = -1;
+        long long fkey = KEY_NONE;
+        for (int s = 0; s <= r && seg < 0; ++s) {
+            long long best = KEY_NONE;
+            for (int base = 0; base < P; base += 64) {
+                int pi = base + lane;
+                if (pi >= P) continue;
+                const uint32_t pr = cur[pi];
+                int u, e_el = 0, bound = 0;
+                if (s == 0) {
+                    u = pj_phi(pr, v_n);
+                } else {
+                    const int k = rm[s - 1];
+                    u = pj_phi(pr, mf[k]);
+                    e_el = mel[k];
+                    bound = C.lab[pj_phi(pr, mt[k])];
+                }
+                for (int j = m - 1; j >= 0; --j) {
+                    int w;
+                    if (C.cf[j] == u) w = C.ct[j];
+                    else if (C.ct[j] == u) w = C.cf[j];
+                    else continue;
+                    const int l = C.cel[j], tl = C.lab[w];
+                    if (tl < min_label || pj_has_vertex(pr, nvm, w)) continue;
+                    if (s > 0 && !(e_el < l || (e_el == l && bound <= tl))) continue;
+                    const long long kk = ((long long)l << 32) | tl;
+                    if (kk < best) best = kk;
+                }
+            }
+            best = wave_min(best);
+            if (best != KEY_NONE) {
+                seg = s;
+                fkey = best;
+            }
+        }
+        if (seg < 0) return 1;  // gspan.cu:583
+        const int fel = (int)(fkey >> 32), ftl = (int)(fkey & 0xffffffff);
+        const int v_i = seg == 0 ? v_n : mf[rm[seg - 1]];
+        if (len >= m) {
+            *err |= 2;
+            return 1;
+        }
+        if (!(C.cf[len] == v_i && C.ct[len] == v_n + 1 && C.cel[len] == fel && C.ctl[len] == ftl))
+            return 0;
+        int np = 0;
+        for (int base = 0; base < P; base += 64) {
+            int pi = base + lane;
+            uint32_t pr = 0;
+            int cnt = 0;
+            uint32_t outs[GS_MAXE];
+            if (pi < P) {
+                pr = cur[pi];
+                const int u = pj_phi(pr, v_i);
+                for (int j = m - 1; j >= 0; --j) {
+                    int w;
+                    if (C.cf[j] == u) w = C.ct[j];
+                    else if (C.ct[j] == u) w = C.cf[j];
+                    else continue;
+                    const int l = C.cel[j], tl = C.lab[w];
+                    if (l != fel || tl != ftl) continue;
+                    if (pj_has_vertex(pr, nvm, w)) continue;
+                    // the min key already satisfies the segment's rule when it was chosen
+                    // from this segment; entries with the same (l, tl) satisfy it too
+                    outs[cnt++] = pr | ((uint32_t)w << (3 * (v_n + 1))) | (1u << (24 + j));
+                }
+            }
+            int tot;
+            int at = np + wave_excl_sum(cnt, &tot);
+            for (int q = 0; q < cnt; ++q)
+                if (at + q < GS_CANON_CAP) nxt[at + q] = outs[q];
+            np += tot;
+        }
+        if (np > GS_CANON_CAP) {
+            *err |= 1;
+            return 1;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            mf[len] = v_i;
+            mt[len] = v_n + 1;
+            mel[len] = fel;
+        }
+        ++len;
+        uint32_t* t = cur;
+        cur = nxt;
+        nxt = t;
+        P = np;
+        __syncthreads();
+    }
+}
+
+__device__ void decode_child(const PatDev& Pp, int slot, KeySpace ks, CanonCode& C) {
+    C.m = Pp.m + 1;
+    for (int j = 0; j < Pp.m; ++j) {
+        C.cf[j] = Pp.cfrom[j];
+        C.ct[j] = Pp.cto[j];
+        C.cel[j] = Pp.cel[j];
+        C.ctl[j] = Pp.cfrom[j] < Pp.cto[j] ? Pp.vlab[Pp.cto[j]] : -1;
+    }
+    for (int v = 0; v < GS_MAXV; ++v) C.lab[v] = v < Pp.nv ? Pp.vlab[v] : 0;
+    const int nb = (Pp.r - 1) * ks.EL;
+    const int m = Pp.m;
+    if (slot < nb) {
+        const int i = slot / ks.EL + 1, el = slot % ks.EL;
+        C.cf[m] = Pp.v_n;
+        C.ct[m] = Pp.cfrom[Pp.rm[i]];
+        C.cel[m] = el;
+        C.ctl[m] = -1;
+    } else {
+        const int s = slot - nb;
+        const int j = s / (ks.EL * ks.VL), el = (s / ks.VL) % ks.EL, tl = s % ks.VL;
+        C.cf[m] = j == 0 ? Pp.v_n : Pp.cfrom[Pp.rm[j - 1]];
+        C.ct[m] = Pp.v_n + 1;
+        C.cel[m] = el;
+        C.ctl[m] = tl;
+        C.lab[Pp.v_n + 1] = tl;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_canon(const PatDev* __restrict__ pats, int npats,
+                                              KeySpace ks, const int* active_keys,
+                                              const int* n_active, int* flags, int* err) {
+    __shared__ uint32_t bufA[GS_CANON_CAP], bufB[GS_CANON_CAP];
+    __shared__ CanonCode C;
+    __shared__ int mf[GS_MAXE], mt[GS_MAXE], mel[GS_MAXE];
+    const int na = *n_active;
+    for (int a = blockIdx.x; a < na; a += gridDim.x) {
+        if (threadIdx.x == 0) {
+            const int key = active_keys[a];
+            int lo = 0, hi = npats - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (pats[mid].key_base <= key) lo = mid;
+                else hi = mid - 1;
+            }
+            decode_child(pats[lo], key - pats[lo].key_base, ks, C);
+        }
+        __syncthreads();
+        int e = 0;
+        int f = canon_wave(C, bufA, bufB, mf, mt, mel, &e);
+        if (threadIdx.x == 0) {
+            flags[a] = f;
+            if (e) atomicOr(err, e);
+        }
+        __syncthreads();
+    }
+}
+
+// Codes given explicitly: codes[i*(1+5*GS_MAXE)] = m, then (from, to, from_label, edge_label,
+// to_label) per edge in label-rank space, -1 where the reference stores no label.
+__global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes, int* flags,
+                                                    int* err) {
+    __shared__ uint32_t bufA[GS_CANON_CAP], bufB[GS_CANON_CAP];
+    __shared__ CanonCode C;
+    __shared__ int mf[GS_MAXE], mt[GS_MAXE], mel[GS_MAXE];
+    for (int a = blockIdx.x; a < ncodes; a += gridDim.x) {
+        if (threadIdx.x == 0) {
+            const int* c = codes + (long long)a * (1 + 5 * GS_MAXE);
+            C.m = c[0];
+            for (int v = 0; v < GS_MAXV; ++v) C.lab[v] = 0;
+            for (int j = 0; j < C.m; ++j) {
+                const int* e = c + 1 + 5 * j;
+                C.cf[j] = e[0];
+                C.ct[j] = e[1];
+                C.cel[j] = e[3];
+                C.ctl[j] = e[0] < e[1] ? e[4] : -1;
+                if (e[2] >= 0) C.lab[e[0]] = e[2];
+                if (e[4] >= 0) C.lab[e[1]] = e[4];
+            }
+        }
+        __syncthreads();
+        int e = 0;
+        int f = canon_wave(C, bufA, bufB, mf, mt, mel, &e);
+        if (threadIdx.x == 0) {
+            flags[a] = f;
+            if (e) atomicOr(err, e);
+        }
+        __syncthreads();
+    }
+}
+
+inline int grid_for(long long n, int block, int cap) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+}  // namespace
+
+namespace gsk {
+
+hipError_t seed_count(const DevGraph& g, int EL, int VL, int* key_count, long long* total,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_seed_count, dim3(grid_for(g.nnz, GS_BLOCK, 8192)), dim3(GS_BLOCK), 0, s,
+                       g, EL, VL, key_count, total);
+    return hipGetLastError();
+}
+
+hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int* cursor,
+                     int2* rows_out, const SetDesc* sets, uint32_t* pool, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed_fill, dim3(grid_for(g.nnz, GS_BLOCK, 8192)), dim3(GS_BLOCK), 0, s,
+                       g, EL, VL, key_slot, cursor, rows_out, sets, pool);
+    return hipGetLastError();
+}
+
+hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
+                LayerPtrs L, RowCtx* ctx, int* work, hipStream_t s) {
+    if (R <= 0) return hipSuccess;
+    long long grid = (R + GS_BLOCK - 1) / GS_BLOCK;
+    hipLaunchKernelGGL(k_prep, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pats, npats, b0, R,
+                       L, ctx, work);
+    return hipGetLastError();
+}
+
+long long scan_tmp_elems(long long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+hipError_t scan_exclusive(const int* in, long long* out, long long n, long long* tmp,
+                          long long tmp_elems, hipStream_t s) {
+    long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(out), 0, 2);
+        return hipGetLastError();
+    }
+    if (tmp_elems < nb + 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(GS_BLOCK), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(GS_BLOCK), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(GS_BLOCK), 0, s, in, n, tmp, nb, out);
+    return hipGetLastError();
+}
+
+hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
+                       const long long* woff, long long b0, long long R, KeySpace ks,
+                       int* key_count, int* tile_counter, int grid, hipStream_t s) {
+    if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_items<false>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
+                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, tile_counter);
+    return hipGetLastError();
+}
+
+hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
+                      const long long* woff, long long b0, long long R, KeySpace ks,
+                      const int* key_slot, int* cursor, int2* rows_out, const SetDesc* sets,
+                      uint32_t* pool, int* tile_counter, int grid, hipStream_t s) {
+    if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_items<true>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
+                       ks, nullptr, key_slot, cursor, rows_out, sets, pool, tile_counter);
+    return hipGetLastError();
+}
+
+int items_grid(int cus) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_items<true>, GS_BLOCK, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 4;
+    return cus * per_cu;
+}
+
+hipError_t compact_active(const int* key_count, long long K, int* active_keys,
+                          int* active_counts, int* n_active, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_active, dim3(grid_for(K, GS_BLOCK, 4096)), dim3(GS_BLOCK), 0, s,
+                       key_count, K, active_keys, active_counts, n_active);
+    return hipGetLastError();
+}
+
+hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_keys,
+                 const int* n_active, int max_active, int* flags, int* err, hipStream_t s) {
+    if (max_active <= 0) return hipSuccess;
+    int grid = max_active < 16384 ? max_active : 16384;
+    hipLaunchKernelGGL(k_canon, dim3(grid), dim3(64), 0, s, pats, npats, ks, active_keys, n_active,
+                       flags, err);
+    return hipGetLastError();
+}
+
+hipError_t canon_codes(const int* codes, int ncodes, int* flags, int* err, hipStream_t s) {
+    if (ncodes <= 0) return hipSuccess;
+    int grid = ncodes < 16384 ? ncodes : 16384;
+    hipLaunchKernelGGL(k_canon_codes, dim3(grid), dim3(64), 0, s, codes, ncodes, flags, err);
+    return hipGetLastError();
+}
+
+hipError_t fill_i32(int* p, int v, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n / 4 + 1, 256, 8192)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s) {
+    if (nkeys <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_slots, dim3((nkeys + 255) / 256), dim3(256), 0, s, key_slot, keys,
+                       nkeys);
+    return hipGetLastError();
+}
+
+hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool, int* counts,
+                      hipStream_t s) {
+    if (nsets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_counts, dim3(nsets * GS_MAXV), dim3(GS_BLOCK), 0, s, sets, pool,
+                       counts);
+    return hipGetLastError();
+}
+
+hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
+                        int mni_mode, const SetDesc* set, uint32_t* pool, hipStream_t s) {
+    (void)mni_mode;
+    if (N <= 0) return hipSuccess;
+    long long grid = (N + GS_BLOCK - 1) / GS_BLOCK;
+    hipLaunchKernelGGL(k_support_rows, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pat, N, L,
+                       set, pool);
+    return hipGetLastError();
+}
+
+}  // namespace gsk

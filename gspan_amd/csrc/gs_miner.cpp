@@ -1,0 +1,1129 @@
+// gs_miner.cpp -- host side of the MI355X gSpan path: device residency of the graph,
+// the level driver, and the C-ABI of include/gspan_amd.h.
+//
+// Driver (replaces mine/fsm, gspan.cu:1118-1199).  A *batch* is a set of frequent,
+// canonical patterns of one size m whose rows live in one device level array.  One
+// batch step runs, on the device:
+//   prep -> scan -> items_count          child rows per child key (two-pass count,
+//                                        gspan.cu:737-755)
+//   compact_active -> canon              candidate codes and their canonicality
+//   items_fill -> set_counts             child rows grouped by key + MNI image sets
+// with two host synchronisations (active keys; set sizes).  GS_MODE_BATCHED feeds every
+// frequent child of a step into the next step (one step per DFS level unless the device
+// budget forces a split); GS_MODE_PLAIN runs one step per parent pattern in the
+// reference's DFS order.  Results are identical; reports are delivered in the reference's
+// order (gspan.cu:1142-1168) in both modes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gs_internal.h"
+#include "gspan_amd.h"
+
+namespace {
+
+struct GsError : std::runtime_error {
+    int code;
+    GsError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw GsError(GS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ device memory
+// Size-class caching pool: every device buffer of the miner comes from here, so the
+// per-batch allocations of the driver do not reach hipMalloc after warm-up.
+class DevPool {
+   public:
+    ~DevPool() { release_all(); }
+    void* get(size_t bytes) {
+        size_t cls = size_class(bytes);
+        auto it = free_.find(cls);
+        if (it != free_.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            cached_ -= cls;
+            in_use_ += cls;
+            peak_ = std::max(peak_, in_use_);
+            sizes_[p] = cls;
+            return p;
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, cls);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            e = hipMalloc(&p, cls);
+            if (e != hipSuccess)
+                throw GsError(GS_ERR_HIP, std::string("hipMalloc(") + std::to_string(cls) +
+                                              "): " + hipGetErrorString(e));
+        }
+        in_use_ += cls;
+        peak_ = std::max(peak_, in_use_);
+        sizes_[p] = cls;
+        return p;
+    }
+    void put(void* p) {
+        if (!p) return;
+        auto it = sizes_.find(p);
+        if (it == sizes_.end()) return;
+        size_t cls = it->second;
+        sizes_.erase(it);
+        in_use_ -= cls;
+        cached_ += cls;
+        free_[cls].push_back(p);
+    }
+    void trim() {
+        for (auto& kv : free_)
+            for (void* p : kv.second) (void)hipFree(p);
+        free_.clear();
+        cached_ = 0;
+    }
+    void release_all() {
+        trim();
+        for (auto& kv : sizes_) (void)hipFree(kv.first);
+        sizes_.clear();
+        in_use_ = 0;
+    }
+    size_t in_use() const { return in_use_; }
+    size_t peak() const { return peak_; }
+
+   private:
+    static size_t size_class(size_t b) {
+        if (b < 256) b = 256;
+        // powers of two up to 1 MiB, then multiples of 1/8 of the next power of two
+        size_t p = 256;
+        while (p < b && p < (1u << 20)) p <<= 1;
+        if (p >= b) return p;
+        size_t hb = 1;
+        while (hb < b) hb <<= 1;
+        size_t step = hb / 8;
+        return (b + step - 1) / step * step;
+    }
+    std::unordered_map<size_t, std::vector<void*>> free_;
+    std::unordered_map<void*, size_t> sizes_;
+    size_t in_use_ = 0, cached_ = 0, peak_ = 0;
+};
+
+class DBuf {
+   public:
+    DBuf() = default;
+    DBuf(DevPool* pool, size_t bytes) : pool_(pool), bytes_(bytes) {
+        p_ = bytes ? pool->get(bytes) : nullptr;
+    }
+    DBuf(DBuf&& o) noexcept : pool_(o.pool_), p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            pool_ = o.pool_;
+            p_ = o.p_;
+            bytes_ = o.bytes_;
+            o.p_ = nullptr;
+        }
+        return *this;
+    }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { reset(); }
+    void reset() {
+        if (p_ && pool_) pool_->put(p_);
+        p_ = nullptr;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p_);
+    }
+    size_t bytes() const { return bytes_; }
+
+   private:
+    DevPool* pool_ = nullptr;
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// ------------------------------------------------------------------ host patterns
+
+struct HPat {
+    int m = 0;
+    int f[GS_MAXE], t[GS_MAXE], el[GS_MAXE], tl[GS_MAXE];  // ranks; tl = -1 for backward
+    int fl0 = 0;                                           // from label of edge 0
+    long long row_begin = 0, row_count = 0;
+    int support = 0;
+
+    int nv() const {
+        int mx = 0;
+        for (int j = 0; j < m; ++j) mx = std::max(mx, std::max(f[j], t[j]));
+        return mx + 1;
+    }
+    // gspan.cu:53-65
+    int rmpath(int* rm) const {
+        int r = 0, last = -1;
+        for (int i = m - 1; i >= 0; --i)
+            if (f[i] < t[i] && (r == 0 || t[i] == last)) {
+                rm[r++] = i;
+                last = f[i];
+            }
+        return r;
+    }
+    void labels(int* lab) const {
+        for (int v = 0; v < GS_MAXV; ++v) lab[v] = 0;
+        lab[f[0]] = fl0;
+        for (int j = 0; j < m; ++j)
+            if (f[j] < t[j]) lab[t[j]] = tl[j];
+    }
+};
+
+struct Report {
+    std::vector<int> code5;  // original labels, reference placeholder form
+    std::vector<long long> order;
+    int support;
+};
+
+}  // namespace
+
+struct gs_ctx {
+    gs_opts opts{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    int n = 0, nnz = 0;
+    std::vector<int> vl_vals, el_vals;  // rank -> original label
+    int VL = 0, EL = 0;
+    DevPool pool;
+    DBuf d_offsets, d_src, d_adj, d_vlab;
+    DevGraph dg{};
+    DBuf d_misc;  // [0] tile counter, [1] n_active, [2] err, [4..5] int64 total
+    int grid_items = 0;
+    std::string err;
+    gs_stats st{};
+    size_t budget = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // mining state
+    int k = 0, min_sup = 0, mni_mode = 0;
+    std::vector<Report> reports;
+
+    int* tile_counter() const { return d_misc.as<int>(); }
+    int* n_active() const { return d_misc.as<int>() + 1; }
+    int* err_flag() const { return d_misc.as<int>() + 2; }
+    long long* total64() const { return reinterpret_cast<long long*>(d_misc.as<int>() + 4); }
+};
+
+namespace {
+
+void sync(gs_ctx* c) { HIPCHECK(hipStreamSynchronize(c->stream)); }
+
+template <class T>
+void h2d(gs_ctx* c, T* dst, const T* src, size_t count) {
+    if (count) HIPCHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+}
+template <class T>
+void d2h(gs_ctx* c, T* dst, const T* src, size_t count) {
+    if (count) HIPCHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+}
+
+int vl_rank(const gs_ctx* c, int label) {
+    auto it = std::lower_bound(c->vl_vals.begin(), c->vl_vals.end(), label);
+    return int(it - c->vl_vals.begin());
+}
+
+long long key_space(int r, int EL, int VL) {
+    return (long long)(r - 1) * EL + (long long)(r + 1) * EL * VL;
+}
+
+PatDev to_dev(const HPat& h) {
+    PatDev d{};
+    d.m = h.m;
+    d.nv = h.nv();
+    int rm[GS_MAXE];
+    d.r = h.rmpath(rm);
+    for (int i = 0; i < GS_MAXE; ++i) d.rm[i] = i < d.r ? rm[i] : 0;
+    d.v_n = h.t[rm[0]];
+    d.min_label = h.fl0;
+    for (int j = 0; j < GS_MAXE; ++j) {
+        d.cfrom[j] = j < h.m ? h.f[j] : 0;
+        d.cto[j] = j < h.m ? h.t[j] : 0;
+        d.cel[j] = j < h.m ? h.el[j] : 0;
+    }
+    h.labels(d.vlab);
+    d.row_begin = h.row_begin;
+    d.row_count = h.row_count;
+    return d;
+}
+
+// Child pattern of parent p for a slot of its key space (see KeySpace in gs_internal.h).
+HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
+    HPat c = p;
+    c.m = p.m + 1;
+    const int j = p.m;
+    const int nb = (d.r - 1) * EL;
+    if (slot < nb) {
+        const int i = slot / EL + 1;
+        c.f[j] = d.v_n;
+        c.t[j] = p.f[d.rm[i]];
+        c.el[j] = slot % EL;
+        c.tl[j] = -1;
+    } else {
+        const int s = slot - nb;
+        const int seg = s / (EL * VL);
+        c.f[j] = seg == 0 ? d.v_n : p.f[d.rm[seg - 1]];
+        c.t[j] = d.v_n + 1;
+        c.el[j] = (s / VL) % EL;
+        c.tl[j] = s % VL;
+    }
+    return c;
+}
+
+// Order of the reference's report lines: seeds by (fl, el, tl); within a node, backward
+// children (ascending v_i, el) before forward children (descending v_i, ascending el, tl);
+// a pattern precedes its descendants (gspan.cu:1142-1168, 1185).
+std::vector<long long> order_key(const HPat& h) {
+    std::vector<long long> k;
+    k.reserve(4 * h.m);
+    k.push_back(h.fl0);
+    k.push_back(h.el[0]);
+    k.push_back(h.tl[0]);
+    k.push_back(0);
+    for (int j = 1; j < h.m; ++j) {
+        if (h.f[j] > h.t[j]) {
+            k.push_back(0);
+            k.push_back(h.t[j]);
+            k.push_back(h.el[j]);
+            k.push_back(0);
+        } else {
+            k.push_back(1);
+            k.push_back(-h.f[j]);
+            k.push_back(h.el[j]);
+            k.push_back(h.tl[j]);
+        }
+    }
+    return k;
+}
+
+void add_report(gs_ctx* c, const HPat& h) {
+    Report r;
+    r.code5.resize(5 * h.m);
+    for (int j = 0; j < h.m; ++j) {
+        r.code5[5 * j + 0] = h.f[j];
+        r.code5[5 * j + 1] = h.t[j];
+        r.code5[5 * j + 2] = j == 0 ? c->vl_vals[h.fl0] : -1;
+        r.code5[5 * j + 3] = c->el_vals[h.el[j]];
+        r.code5[5 * j + 4] = h.tl[j] < 0 ? -1 : c->vl_vals[h.tl[j]];
+    }
+    r.order = order_key(h);
+    r.support = h.support;
+    c->reports.push_back(std::move(r));
+    c->st.reported++;
+}
+
+// Support from the per-vertex distinct-image counts (gspan.cu:282-288 / 367-373).
+int support_of(const HPat& h, const int* counts, int mni_mode) {
+    const int nv = h.nv();
+    bool v0 = true;
+    if (mni_mode == GS_MNI_OMP_LITERAL) {
+        v0 = false;
+        for (int j = 1; j < h.m; ++j)
+            if (h.f[j] == 0) v0 = true;
+    }
+    long long sup = INT_MAX;
+    for (int v = 0; v < nv; ++v) {
+        if (v == 0 && !v0) continue;
+        if (counts[v] > 0 && counts[v] < sup) sup = counts[v];
+    }
+    return (int)sup;
+}
+
+// Set memory per key: a bitmap over all graph vertices, or a power-of-two open-addressing
+// table of at least twice the key's rows when that is smaller.
+void plan_set(const gs_ctx* c, long long rows, int nv, SetDesc& d, long long& pool_words) {
+    const long long bitmap_words = (((long long)c->n + 31) / 32 + 3) / 4 * 4;
+    long long cap = 16;
+    const long long want = std::min<long long>(2 * rows, 2LL * c->n);
+    while (cap < want) cap <<= 1;
+    d.nv = nv;
+    if (cap < bitmap_words) {
+        d.kind = GS_SET_HASH;
+        d.size = (int)cap;
+    } else {
+        d.kind = GS_SET_BITMAP;
+        d.size = (int)bitmap_words;
+    }
+    d.off = pool_words;
+    pool_words += (long long)nv * d.size;
+    pool_words = (pool_words + 3) / 4 * 4;
+    d.pad = 0;
+}
+
+struct ChildInfo {
+    HPat pat;
+    long long count;
+    int canonical;
+    int support;
+    long long row_off;  // in the child row array (-1 when not written)
+};
+
+struct StepOut {
+    std::vector<ChildInfo> children;  // every candidate child (active key), key order
+    DBuf rows;                        // child rows of the written keys
+    long long emitted = 0;
+};
+
+constexpr int STEP_OK = 0;
+constexpr int STEP_SPLIT = 1;
+
+// One extension step of a batch of size-m patterns.  write_rows: keep child rows (needed
+// below the last level); all_live: write rows and sets of non-canonical children too.
+int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const int2*>& layers,
+             bool write_rows, bool all_live, StepOut& out) {
+    const int B = (int)pats.size();
+    if (B == 0) return STEP_OK;
+    const int m = pats[0].m;
+    if (m + 1 > GS_MAXE)
+        throw GsError(GS_ERR_LIMIT, "patterns larger than 7 edges are not supported");
+    std::vector<PatDev> pd(B);
+    long long R = 0, K = 0;
+    for (int i = 0; i < B; ++i) {
+        pd[i] = to_dev(pats[i]);
+        pd[i].key_base = (int)std::min<long long>(K, INT_MAX);
+        pd[i].brow_begin = R;
+        K += key_space(pd[i].r, c->EL, c->VL);
+        R += pats[i].row_count;
+    }
+    if (K > (1LL << 30)) {
+        if (B > 1) return STEP_SPLIT;
+        throw GsError(GS_ERR_LIMIT, "child key space of one pattern exceeds 2^30");
+    }
+    DevPool* P = &c->pool;
+    DBuf d_pats(P, sizeof(PatDev) * B);
+    h2d(c, d_pats.as<PatDev>(), pd.data(), B);
+    LayerPtrs L{};
+    for (int l = 0; l < m; ++l) L.layer[l] = layers[l];
+    const KeySpace ks{c->EL, c->VL};
+
+    const long long chunk = c->opts.chunk_rows > 0 ? c->opts.chunk_rows : (16LL << 20);
+    const long long crow = std::min(R, chunk);
+    const int nchunks = (int)((R + chunk - 1) / chunk);
+    DBuf d_ctx(P, sizeof(RowCtx) * std::max<long long>(crow, 1));
+    DBuf d_work(P, sizeof(int) * std::max<long long>(crow, 1));
+    DBuf d_woff(P, sizeof(long long) * (crow + 1));
+    const long long tmp_elems = gsk::scan_tmp_elems(crow);
+    DBuf d_tmp(P, sizeof(long long) * tmp_elems);
+    DBuf d_kc(P, sizeof(int) * K);
+    HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
+
+    long long W_total = 0;
+    auto prep_chunk = [&](int ci) -> long long {
+        const long long b0 = (long long)ci * chunk;
+        const long long Rc = std::min(chunk, R - b0);
+        HIPCHECK(gsk::prep(c->dg, d_pats.as<PatDev>(), B, b0, Rc, L, d_ctx.as<RowCtx>(),
+                           d_work.as<int>(), c->stream));
+        HIPCHECK(gsk::scan_exclusive(d_work.as<int>(), d_woff.as<long long>(), Rc,
+                                     d_tmp.as<long long>(), tmp_elems, c->stream));
+        return Rc;
+    };
+    auto time_begin = [&]() {
+        if (c->opts.timing) HIPCHECK(hipEventRecord(c->ev0, c->stream));
+    };
+    auto time_end = [&](double& acc) {
+        if (c->opts.timing) {
+            HIPCHECK(hipEventRecord(c->ev1, c->stream));
+            HIPCHECK(hipEventSynchronize(c->ev1));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            acc += ms;
+        }
+    };
+
+    // ---- pass 1: count children per key
+    for (int ci = 0; ci < nchunks; ++ci) {
+        const long long b0 = (long long)ci * chunk;
+        const long long Rc = prep_chunk(ci);
+        long long Wc = 0;
+        d2h(c, &Wc, d_woff.as<long long>() + Rc, 1);
+        HIPCHECK(gsk::fill_i32(c->tile_counter(), 0, 1, c->stream));
+        time_begin();
+        HIPCHECK(gsk::items_count(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                                  d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
+                                  c->tile_counter(), c->grid_items, c->stream));
+        time_end(c->st.t_count_ms);
+        c->st.count_launches++;
+        sync(c);
+        W_total += Wc;
+    }
+    c->st.items += W_total;
+    c->st.parent_rows += R;
+
+    // ---- candidates and their canonicality
+    DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
+    HIPCHECK(gsk::fill_i32(c->n_active(), 0, 2, c->stream));  // n_active and err
+    HIPCHECK(gsk::compact_active(d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>(),
+                                 c->n_active(), c->stream));
+    HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
+                        d_flags.as<int>(), c->err_flag(), c->stream));
+    int na_err[2] = {0, 0};
+    d2h(c, na_err, c->n_active(), 2);
+    sync(c);
+    if (na_err[1] & 1)
+        throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
+    if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
+    const int A = na_err[0];
+    std::vector<int> akeys(A), acnt(A), aflag(A);
+    d2h(c, akeys.data(), d_active.as<int>(), A);
+    d2h(c, acnt.data(), d_acnt.as<int>(), A);
+    d2h(c, aflag.data(), d_flags.as<int>(), A);
+    sync(c);
+    // key order = (parent, slot) order; compact_active leaves it unordered
+    std::vector<int> order(A);
+    for (int i = 0; i < A; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return akeys[a] < akeys[b]; });
+
+    // ---- plan child rows and sets of the live keys
+    std::vector<int> live_keys;
+    std::vector<int> cursor;
+    std::vector<SetDesc> sets;
+    out.children.reserve(A);
+    long long rows_total = 0, pool_words = 0, emitted = 0;
+    for (int oi = 0; oi < A; ++oi) {
+        const int a = order[oi];
+        const int key = akeys[a];
+        int p = int(std::upper_bound(pd.begin(), pd.end(), key,
+                                     [](int k, const PatDev& d) { return k < d.key_base; }) -
+                    pd.begin()) - 1;
+        ChildInfo ci;
+        ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
+        ci.count = acnt[a];
+        ci.canonical = aflag[a];
+        ci.support = 0;
+        ci.row_off = -1;
+        emitted += acnt[a];
+        if (ci.canonical || all_live) {
+            SetDesc d;
+            plan_set(c, ci.count, ci.pat.nv(), d, pool_words);
+            sets.push_back(d);
+            live_keys.push_back(key);
+            if (write_rows) {
+                ci.row_off = rows_total;
+                cursor.push_back((int)rows_total);
+                rows_total += ci.count;
+            } else {
+                cursor.push_back(0);
+            }
+        }
+        out.children.push_back(ci);
+    }
+    out.emitted = emitted;
+    const int NL = (int)live_keys.size();
+    if (rows_total > INT_MAX) {
+        if (B > 1) return STEP_SPLIT;
+        throw GsError(GS_ERR_BUDGET, "child rows of one pattern exceed 2^31");
+    }
+    const size_t need = (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
+                        (size_t)NL * (sizeof(SetDesc) + 40) + (size_t)K * 4;
+    if (c->pool.in_use() + need > c->budget) {
+        if (B > 1) return STEP_SPLIT;
+        throw GsError(GS_ERR_BUDGET, "children of one pattern exceed the device budget (" +
+                                         std::to_string(need) + " bytes)");
+    }
+    if (NL == 0) {
+        c->st.batches++;
+        return STEP_OK;
+    }
+
+    // ---- pass 2: write child rows and fill the MNI sets
+    DBuf d_slot(P, sizeof(int) * K);
+    DBuf d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(int) * NL);
+    DBuf d_sets(P, sizeof(SetDesc) * NL);
+    DBuf d_pool(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
+    DBuf d_counts(P, sizeof(int) * NL * GS_MAXV);
+    if (write_rows) out.rows = DBuf(P, sizeof(int2) * std::max<long long>(rows_total, 1));
+    HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
+    h2d(c, d_live.as<int>(), live_keys.data(), NL);
+    h2d(c, d_cursor.as<int>(), cursor.data(), NL);
+    h2d(c, d_sets.as<SetDesc>(), sets.data(), NL);
+    HIPCHECK(gsk::scatter_slots(d_slot.as<int>(), d_live.as<int>(), NL, c->stream));
+    HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
+    for (int ci = 0; ci < nchunks; ++ci) {
+        const long long b0 = (long long)ci * chunk;
+        const long long Rc = nchunks > 1 ? prep_chunk(ci) : std::min(chunk, R - b0);
+        HIPCHECK(gsk::fill_i32(c->tile_counter(), 0, 1, c->stream));
+        time_begin();
+        HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                                 d_woff.as<long long>(), b0, Rc, ks, d_slot.as<int>(),
+                                 d_cursor.as<int>(), write_rows ? out.rows.as<int2>() : nullptr,
+                                 d_sets.as<SetDesc>(), d_pool.as<uint32_t>(), c->tile_counter(),
+                                 c->grid_items, c->stream));
+        time_end(c->st.t_fill_ms);
+        c->st.fill_launches++;
+    }
+    // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
+    // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
+    {
+        double bytes = 16.0 * (double)W_total + 8.0 * (double)emitted;
+        for (int i = 0; i < B; ++i)
+            bytes += (double)pats[i].row_count *
+                     (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
+        c->st.alg_bytes_fill += bytes;
+    }
+    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(), d_counts.as<int>(),
+                             c->stream));
+    std::vector<int> counts((size_t)NL * GS_MAXV);
+    d2h(c, counts.data(), d_counts.as<int>(), counts.size());
+    sync(c);
+    int li = 0;
+    for (auto& ch : out.children) {
+        if (!(ch.canonical || all_live)) continue;
+        ch.support = support_of(ch.pat, &counts[(size_t)li * GS_MAXV], c->mni_mode);
+        ch.pat.support = ch.support;
+        ch.pat.row_begin = ch.row_off;
+        ch.pat.row_count = ch.count;
+        ++li;
+    }
+    c->st.batches++;
+    return STEP_OK;
+}
+
+void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers);
+
+// Extends a batch, reports its frequent canonical children and descends into them.
+void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers) {
+    const int m = pats[0].m;
+    const bool last = m + 1 >= c->k;
+    StepOut so;
+    if (run_step(c, pats, layers, !last, false, so) == STEP_SPLIT) {
+        // split by rows into two halves and run them one after the other
+        long long tot = 0, half = 0;
+        for (auto& p : pats) tot += p.row_count;
+        size_t cut = 0;
+        while (cut + 1 < pats.size() && half + pats[cut].row_count <= tot / 2)
+            half += pats[cut++].row_count;
+        if (cut == 0) cut = 1;
+        std::vector<HPat> a(pats.begin(), pats.begin() + cut), b(pats.begin() + cut, pats.end());
+        expand(c, a, layers);
+        expand(c, b, layers);
+        return;
+    }
+    c->st.extended += so.emitted;
+    std::vector<HPat> next;
+    for (auto& ch : so.children) {
+        c->st.candidates++;
+        if (!ch.canonical) continue;
+        c->st.canonical++;
+        if (ch.support < c->min_sup) continue;
+        add_report(c, ch.pat);
+        if (!last) next.push_back(ch.pat);
+    }
+    if (!last && !next.empty()) {
+        layers.push_back(so.rows.as<int2>());
+        mine_level(c, next, layers);
+        layers.pop_back();
+    }
+}
+
+void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers) {
+    if (pats.empty() || pats[0].m >= c->k) return;
+    if (c->opts.mode == GS_MODE_PLAIN) {
+        // one parent at a time, in the order the reference visits them
+        std::vector<size_t> idx(pats.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::vector<std::vector<long long>> keys(pats.size());
+        for (size_t i = 0; i < pats.size(); ++i) keys[i] = order_key(pats[i]);
+        std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return keys[a] < keys[b]; });
+        for (size_t i : idx) {
+            std::vector<HPat> one{pats[i]};
+            expand(c, one, layers);
+        }
+    } else {
+        expand(c, pats, layers);
+    }
+}
+
+// Single-edge codes (gspan.cu:1176-1190): counted, sized and filled in two passes over
+// the CSR entries; every code of size 1 is canonical (gspan.cu:592-593).
+void mine_seeds(gs_ctx* c) {
+    DevPool* P = &c->pool;
+    const long long K = (long long)c->VL * c->EL * c->VL;
+    if (K > (1LL << 30)) throw GsError(GS_ERR_LIMIT, "too many labels for the seed key space");
+    DBuf d_kc(P, sizeof(int) * K);
+    HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
+    HIPCHECK(gsk::fill_i32(c->n_active(), 0, 1, c->stream));
+    HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->total64()), 0, 2, c->stream));
+    HIPCHECK(gsk::seed_count(c->dg, c->EL, c->VL, d_kc.as<int>(), c->total64(), c->stream));
+    DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K);
+    HIPCHECK(gsk::compact_active(d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>(),
+                                 c->n_active(), c->stream));
+    int A = 0;
+    d2h(c, &A, c->n_active(), 1);
+    sync(c);
+    std::vector<int> akeys(A), acnt(A);
+    d2h(c, akeys.data(), d_active.as<int>(), A);
+    d2h(c, acnt.data(), d_acnt.as<int>(), A);
+    sync(c);
+    std::vector<int> order(A);
+    for (int i = 0; i < A; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return akeys[a] < akeys[b]; });
+    const bool rows = c->k > 1;
+    std::vector<HPat> seeds;
+    std::vector<int> live, cursor;
+    std::vector<SetDesc> sets;
+    long long rows_total = 0, pool_words = 0;
+    for (int oi = 0; oi < A; ++oi) {
+        const int a = order[oi];
+        const int key = akeys[a];
+        HPat h;
+        h.m = 1;
+        h.f[0] = 0;
+        h.t[0] = 1;
+        h.fl0 = key / (c->EL * c->VL);
+        h.el[0] = (key / c->VL) % c->EL;
+        h.tl[0] = key % c->VL;
+        h.row_begin = rows_total;
+        h.row_count = acnt[a];
+        SetDesc d;
+        plan_set(c, acnt[a], 2, d, pool_words);
+        sets.push_back(d);
+        live.push_back(key);
+        cursor.push_back((int)rows_total);
+        if (rows) rows_total += acnt[a];
+        seeds.push_back(h);
+    }
+    if (rows_total > INT_MAX) throw GsError(GS_ERR_BUDGET, "single-edge rows exceed 2^31");
+    const int NL = (int)live.size();
+    c->st.candidates += NL;
+    c->st.canonical += NL;
+    if (NL == 0) return;
+    DBuf d_slot(P, sizeof(int) * K), d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(int) * NL);
+    DBuf d_sets(P, sizeof(SetDesc) * NL);
+    DBuf d_pool(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
+    DBuf d_counts(P, sizeof(int) * NL * GS_MAXV);
+    DBuf d_rows;
+    if (rows) d_rows = DBuf(P, sizeof(int2) * std::max<long long>(rows_total, 1));
+    HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
+    h2d(c, d_live.as<int>(), live.data(), NL);
+    h2d(c, d_cursor.as<int>(), cursor.data(), NL);
+    h2d(c, d_sets.as<SetDesc>(), sets.data(), NL);
+    HIPCHECK(gsk::scatter_slots(d_slot.as<int>(), d_live.as<int>(), NL, c->stream));
+    HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
+    HIPCHECK(gsk::seed_fill(c->dg, c->EL, c->VL, d_slot.as<int>(), d_cursor.as<int>(),
+                            rows ? d_rows.as<int2>() : nullptr, d_sets.as<SetDesc>(),
+                            d_pool.as<uint32_t>(), c->stream));
+    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(), d_counts.as<int>(),
+                             c->stream));
+    std::vector<int> counts((size_t)NL * GS_MAXV);
+    d2h(c, counts.data(), d_counts.as<int>(), counts.size());
+    sync(c);
+    d_pool.reset();
+    std::vector<HPat> freq;
+    for (int i = 0; i < NL; ++i) {
+        seeds[i].support = support_of(seeds[i], &counts[(size_t)i * GS_MAXV], c->mni_mode);
+        if (seeds[i].support < c->min_sup) continue;
+        add_report(c, seeds[i]);
+        freq.push_back(seeds[i]);
+    }
+    if (rows && !freq.empty()) {
+        std::vector<const int2*> layers{d_rows.as<int2>()};
+        mine_level(c, freq, layers);
+    }
+}
+
+void set_error(gs_ctx* c, const std::string& msg) {
+    if (c) c->err = msg;
+}
+
+int guard(gs_ctx* c, const std::function<void()>& fn) {
+    try {
+        fn();
+        return GS_OK;
+    } catch (const GsError& e) {
+        set_error(c, e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error(c, "host allocation failed");
+        return GS_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        set_error(c, e.what());
+        return GS_ERR_INTERNAL;
+    }
+}
+
+HPat code_to_hpat(gs_ctx* c, const int* code5, int m) {
+    if (m < 1 || m > GS_MAXE) throw GsError(GS_ERR_LIMIT, "pattern size out of range");
+    HPat h;
+    h.m = m;
+    for (int j = 0; j < m; ++j) {
+        const int* e = code5 + 5 * j;
+        h.f[j] = e[0];
+        h.t[j] = e[1];
+        if (e[0] < 0 || e[1] < 0 || e[0] >= GS_MAXV || e[1] >= GS_MAXV || e[0] == e[1])
+            throw GsError(GS_ERR_INVALID, "bad pattern vertex ids");
+        auto ei = std::lower_bound(c->el_vals.begin(), c->el_vals.end(), e[3]);
+        if (ei == c->el_vals.end() || *ei != e[3])
+            throw GsError(GS_ERR_INVALID, "edge label not present in the graph");
+        h.el[j] = int(ei - c->el_vals.begin());
+        if (e[0] < e[1]) {
+            auto vi = std::lower_bound(c->vl_vals.begin(), c->vl_vals.end(), e[4]);
+            if (vi == c->vl_vals.end() || *vi != e[4])
+                throw GsError(GS_ERR_INVALID, "vertex label not present in the graph");
+            h.tl[j] = int(vi - c->vl_vals.begin());
+        } else {
+            h.tl[j] = -1;
+        }
+    }
+    auto fi = std::lower_bound(c->vl_vals.begin(), c->vl_vals.end(), code5[2]);
+    if (fi == c->vl_vals.end() || *fi != code5[2])
+        throw GsError(GS_ERR_INVALID, "vertex label not present in the graph");
+    h.fl0 = int(fi - c->vl_vals.begin());
+    if (h.f[0] != 0 || h.t[0] != 1) throw GsError(GS_ERR_INVALID, "code must start with (0, 1)");
+    // DFS-code shape: forward edges reach the next new vertex from a known one, backward
+    // edges join two known vertices (every vertex image is then defined by the rows)
+    int nv = 2;
+    for (int j = 1; j < m; ++j) {
+        if (h.f[j] < h.t[j]) {
+            if (h.t[j] != nv || h.f[j] >= nv) throw GsError(GS_ERR_INVALID, "bad forward edge");
+            ++nv;
+        } else if (h.f[j] >= nv) {
+            throw GsError(GS_ERR_INVALID, "bad backward edge");
+        }
+    }
+    return h;
+}
+
+std::vector<DBuf> upload_layers(gs_ctx* c, int m, const int* const* layers,
+                                const long long* counts, std::vector<const int2*>& ptrs) {
+    std::vector<DBuf> bufs;
+    ptrs.clear();
+    for (int l = 0; l < m; ++l) {
+        if (counts[l] < 0 || counts[l] > INT_MAX || !layers[l])
+            throw GsError(GS_ERR_INVALID, "layer size out of range");
+        // every row must point at an existing parent row and CSR entry
+        const int* rw = layers[l];
+        for (long long i = 0; i < counts[l]; ++i) {
+            const int prev = rw[2 * i], eid = rw[2 * i + 1];
+            if (eid < 0 || eid >= c->nnz) throw GsError(GS_ERR_INVALID, "edge id out of range");
+            if (l > 0 && (prev < 0 || prev >= counts[l - 1]))
+                throw GsError(GS_ERR_INVALID, "prev_idx out of range");
+        }
+        bufs.emplace_back(&c->pool, sizeof(int2) * std::max<long long>(counts[l], 1));
+        h2d(c, bufs.back().as<int2>(), reinterpret_cast<const int2*>(layers[l]), counts[l]);
+        ptrs.push_back(bufs.back().as<int2>());
+    }
+    return bufs;
+}
+
+}  // namespace
+
+// ================================================================== C-ABI
+
+extern "C" {
+
+const char* gs_version(void) { return "gspan_amd 0.1 gfx950"; }
+
+int gs_default_opts(gs_opts* o) {
+    if (!o) return GS_ERR_INVALID;
+    std::memset(o, 0, sizeof(*o));
+    o->mode = GS_MODE_BATCHED;
+    o->mni_mode = GS_MNI_GPU;
+    return GS_OK;
+}
+
+int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, int nnz,
+              const gs_opts* opts, gs_ctx** out) {
+    if (!out) return GS_ERR_INVALID;
+    *out = nullptr;
+    if (n < 0 || nnz < 0 || (n > 0 && (!offsets || !vlabels)) || (nnz > 0 && !edges3) ||
+        (n == 0 && nnz != 0))
+        return GS_ERR_INVALID;
+    std::unique_ptr<gs_ctx> c(new gs_ctx());
+    if (opts) c->opts = *opts;
+    else gs_default_opts(&c->opts);
+    int rc = guard(c.get(), [&]() {
+        gs_ctx* x = c.get();
+        // graph validation: the CSR invariants CSRGraph(Graph&) establishes (graph.cc:66-112)
+        if (n > 0 && offsets[0] != 0) throw GsError(GS_ERR_INVALID, "offsets[0] must be 0");
+        if (n > 0 && offsets[n] != nnz) throw GsError(GS_ERR_INVALID, "offsets[n] != nnz");
+        for (int v = 0; v < n; ++v) {
+            if (offsets[v + 1] < offsets[v]) throw GsError(GS_ERR_INVALID, "offsets not monotone");
+            for (int i = offsets[v]; i < offsets[v + 1]; ++i) {
+                const int* e = edges3 + 3 * (size_t)i;
+                if (e[0] != v) throw GsError(GS_ERR_INVALID, "edge.from does not match its row");
+                if (e[1] < 0 || e[1] >= n) throw GsError(GS_ERR_INVALID, "edge.to out of range");
+                if (i > offsets[v]) {
+                    const int* p = e - 3;
+                    if (p[1] > e[1] || (p[1] == e[1] && p[2] > e[2]))
+                        throw GsError(GS_ERR_INVALID, "adjacency not sorted by (to, label)");
+                }
+            }
+        }
+        x->n = n;
+        x->nnz = nnz;
+        x->vl_vals.assign(vlabels, vlabels + n);
+        std::sort(x->vl_vals.begin(), x->vl_vals.end());
+        x->vl_vals.erase(std::unique(x->vl_vals.begin(), x->vl_vals.end()), x->vl_vals.end());
+        x->el_vals.clear();
+        for (int i = 0; i < nnz; ++i) x->el_vals.push_back(edges3[3 * (size_t)i + 2]);
+        std::sort(x->el_vals.begin(), x->el_vals.end());
+        x->el_vals.erase(std::unique(x->el_vals.begin(), x->el_vals.end()), x->el_vals.end());
+        if (x->vl_vals.empty()) x->vl_vals.push_back(0);
+        if (x->el_vals.empty()) x->el_vals.push_back(0);
+        if (x->vl_vals.size() > 65535 || x->el_vals.size() > 65535)
+            throw GsError(GS_ERR_LIMIT, "more than 65535 distinct labels");
+        x->VL = (int)x->vl_vals.size();
+        x->EL = (int)x->el_vals.size();
+
+        HIPCHECK(hipSetDevice(x->opts.device));
+        x->device = x->opts.device;
+        HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&x->ev0));
+        HIPCHECK(hipEventCreate(&x->ev1));
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, x->device));
+        x->cus = prop.multiProcessorCount;
+        x->grid_items = gsk::items_grid(x->cus);
+        size_t fr = 0, tot = 0;
+        HIPCHECK(hipMemGetInfo(&fr, &tot));
+        x->budget = x->opts.budget_bytes ? (size_t)x->opts.budget_bytes : (size_t)(fr * 0.85);
+
+        std::vector<int> vr(n), src(nnz);
+        std::vector<uint2> adj(nnz);
+        std::unordered_map<int, int> vmap, emap;
+        for (int i = 0; i < x->VL; ++i) vmap[x->vl_vals[i]] = i;
+        for (int i = 0; i < x->EL; ++i) emap[x->el_vals[i]] = i;
+        for (int v = 0; v < n; ++v) vr[v] = vmap[vlabels[v]];
+        for (int i = 0; i < nnz; ++i) {
+            const int* e = edges3 + 3 * (size_t)i;
+            src[i] = e[0];
+            adj[i].x = (unsigned)e[1];
+            adj[i].y = ((unsigned)emap[e[2]] << 16) | (unsigned)vr[e[1]];
+        }
+        DevPool* P = &x->pool;
+        x->d_offsets = DBuf(P, sizeof(int) * (n + 1));
+        const int zero = 0;
+        if (n == 0) offsets = &zero;
+        x->d_src = DBuf(P, sizeof(int) * std::max(nnz, 1));
+        x->d_adj = DBuf(P, sizeof(uint2) * std::max(nnz, 1));
+        x->d_vlab = DBuf(P, sizeof(int) * std::max(n, 1));
+        x->d_misc = DBuf(P, 64);
+        h2d(x, x->d_offsets.as<int>(), offsets, n + 1);
+        h2d(x, x->d_src.as<int>(), src.data(), nnz);
+        h2d(x, x->d_adj.as<uint2>(), adj.data(), nnz);
+        h2d(x, x->d_vlab.as<int>(), vr.data(), n);
+        sync(x);
+        x->dg.n = n;
+        x->dg.nnz = nnz;
+        x->dg.offsets = x->d_offsets.as<int>();
+        x->dg.src = x->d_src.as<int>();
+        x->dg.adj = x->d_adj.as<uint2>();
+        x->dg.vlab = x->d_vlab.as<int>();
+    });
+    if (rc != GS_OK) {
+        std::fprintf(stderr, "gs_create: %s\n", c->err.c_str());
+        gs_destroy(c.release());
+        return rc;
+    }
+    *out = c.release();
+    return GS_OK;
+}
+
+int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
+    if (!c || k < 1) return GS_ERR_INVALID;
+    if (k > GS_MAXE) {
+        c->err = "k > 7 is not supported";
+        return GS_ERR_LIMIT;
+    }
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        auto t0 = std::chrono::steady_clock::now();
+        c->k = k;
+        c->min_sup = min_support;
+        c->mni_mode = c->opts.mni_mode;
+        c->reports.clear();
+        mine_seeds(c);
+        std::stable_sort(c->reports.begin(), c->reports.end(),
+                         [](const Report& a, const Report& b) { return a.order < b.order; });
+        auto t1 = std::chrono::steady_clock::now();
+        c->st.t_mine_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        c->st.peak_device_bytes = c->pool.peak();
+        if (cb) {
+            for (const Report& r : c->reports)
+                if (cb(r.code5.data(), (int)r.code5.size() / 5, r.support, user) != 0)
+                    throw GsError(GS_ERR_CALLBACK, "report callback stopped mining");
+        }
+    });
+}
+
+int gs_support(gs_ctx* c, const int* code5, int m, const int* const* layers,
+               const long long* counts, int mni_mode, int* support_out) {
+    if (!c || !code5 || !layers || !counts || !support_out) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        HPat h = code_to_hpat(c, code5, m);
+        std::vector<const int2*> ptrs;
+        std::vector<DBuf> bufs = upload_layers(c, m, layers, counts, ptrs);
+        PatDev d = to_dev(h);
+        LayerPtrs L{};
+        for (int l = 0; l < m; ++l) L.layer[l] = ptrs[l];
+        DBuf d_pat(&c->pool, sizeof(PatDev));
+        h2d(c, d_pat.as<PatDev>(), &d, 1);
+        long long words = 0;
+        SetDesc sd;
+        plan_set(c, counts[m - 1], h.nv(), sd, words);
+        DBuf d_set(&c->pool, sizeof(SetDesc)), d_pool(&c->pool, 4 * std::max(words, 4LL));
+        DBuf d_counts(&c->pool, sizeof(int) * GS_MAXV);
+        h2d(c, d_set.as<SetDesc>(), &sd, 1);
+        HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, words, c->stream));
+        HIPCHECK(gsk::support_rows(c->dg, d_pat.as<PatDev>(), counts[m - 1], L, mni_mode,
+                                   d_set.as<SetDesc>(), d_pool.as<uint32_t>(), c->stream));
+        HIPCHECK(gsk::set_counts(d_set.as<SetDesc>(), 1, d_pool.as<uint32_t>(),
+                                 d_counts.as<int>(), c->stream));
+        int cnt[GS_MAXV];
+        d2h(c, cnt, d_counts.as<int>(), GS_MAXV);
+        sync(c);
+        *support_out = support_of(h, cnt, mni_mode);
+    });
+}
+
+int gs_extend(gs_ctx* c, const int* code5, int m, const int* const* layers,
+              const long long* counts, int mni_mode, gs_child_cb cb, void* user,
+              long long* total) {
+    if (!c || !code5 || !layers || !counts) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        if (m + 1 > GS_MAXE) throw GsError(GS_ERR_LIMIT, "pattern too large to extend");
+        HPat h = code_to_hpat(c, code5, m);
+        h.row_begin = 0;
+        h.row_count = counts[m - 1];
+        std::vector<const int2*> ptrs;
+        std::vector<DBuf> bufs = upload_layers(c, m, layers, counts, ptrs);
+        c->mni_mode = mni_mode;
+        std::vector<HPat> one{h};
+        StepOut so;
+        if (run_step(c, one, ptrs, true, true, so) != STEP_OK)
+            throw GsError(GS_ERR_BUDGET, "extension does not fit the device budget");
+        if (total) *total = so.emitted;
+        // reference child order: backward ascending (v_i, el), forward descending v_i
+        std::vector<size_t> idx(so.children.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::vector<std::vector<long long>> keys(idx.size());
+        for (size_t i = 0; i < idx.size(); ++i) keys[i] = order_key(so.children[i].pat);
+        std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return keys[a] < keys[b]; });
+        long long rows_total = 0;
+        for (auto& ch : so.children) rows_total += ch.count;
+        std::vector<int2> rows((size_t)rows_total);
+        d2h(c, rows.data(), so.rows.as<int2>(), (size_t)rows_total);
+        sync(c);
+        for (size_t i : idx) {
+            const ChildInfo& ch = so.children[i];
+            const int j = m;
+            int e5[5] = {ch.pat.f[j], ch.pat.t[j], -1, c->el_vals[ch.pat.el[j]],
+                         ch.pat.tl[j] < 0 ? -1 : c->vl_vals[ch.pat.tl[j]]};
+            if (cb)
+                cb(e5, reinterpret_cast<const int*>(rows.data() + ch.row_off), ch.count,
+                   ch.canonical, ch.support, user);
+        }
+    });
+}
+
+int gs_is_canonical_batch(gs_ctx* c, const int* codes5, const int* lens, int ncodes,
+                          int* flags) {
+    if (!c || (ncodes > 0 && (!codes5 || !lens || !flags))) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        // canonicality depends only on the order of labels: rank them per call
+        std::vector<int> vls, els;
+        long long off = 0;
+        for (int i = 0; i < ncodes; ++i) {
+            if (lens[i] < 1 || lens[i] > GS_MAXE)
+                throw GsError(GS_ERR_LIMIT, "code length out of range");
+            for (int j = 0; j < lens[i]; ++j) {
+                const int* e = codes5 + 5 * (off + j);
+                if (e[2] >= 0) vls.push_back(e[2]);
+                if (e[4] >= 0) vls.push_back(e[4]);
+                els.push_back(e[3]);
+            }
+            off += lens[i];
+        }
+        std::sort(vls.begin(), vls.end());
+        vls.erase(std::unique(vls.begin(), vls.end()), vls.end());
+        std::sort(els.begin(), els.end());
+        els.erase(std::unique(els.begin(), els.end()), els.end());
+        auto rk = [](const std::vector<int>& v, int x) {
+            return int(std::lower_bound(v.begin(), v.end(), x) - v.begin());
+        };
+        const int stride = 1 + 5 * GS_MAXE;
+        std::vector<int> packed((size_t)ncodes * stride, 0);
+        off = 0;
+        for (int i = 0; i < ncodes; ++i) {
+            int* p = packed.data() + (size_t)i * stride;
+            p[0] = lens[i];
+            for (int j = 0; j < lens[i]; ++j) {
+                const int* e = codes5 + 5 * (off + j);
+                if (e[0] < 0 || e[1] < 0 || e[0] >= GS_MAXV || e[1] >= GS_MAXV)
+                    throw GsError(GS_ERR_INVALID, "bad pattern vertex ids");
+                int* q = p + 1 + 5 * j;
+                q[0] = e[0];
+                q[1] = e[1];
+                q[2] = e[2] >= 0 ? rk(vls, e[2]) : -1;
+                q[3] = rk(els, e[3]);
+                q[4] = e[4] >= 0 ? rk(vls, e[4]) : -1;
+            }
+            off += lens[i];
+        }
+        DBuf d_codes(&c->pool, sizeof(int) * std::max<size_t>(packed.size(), 1));
+        DBuf d_flags(&c->pool, sizeof(int) * std::max(ncodes, 1));
+        h2d(c, d_codes.as<int>(), packed.data(), packed.size());
+        HIPCHECK(gsk::fill_i32(c->err_flag(), 0, 1, c->stream));
+        HIPCHECK(gsk::canon_codes(d_codes.as<int>(), ncodes, d_flags.as<int>(), c->err_flag(),
+                                  c->stream));
+        int e = 0;
+        d2h(c, flags, d_flags.as<int>(), ncodes);
+        d2h(c, &e, c->err_flag(), 1);
+        sync(c);
+        if (e & 1) throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
+        if (e & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
+    });
+}
+
+int gs_get_stats(const gs_ctx* c, gs_stats* out) {
+    if (!c || !out) return GS_ERR_INVALID;
+    *out = c->st;
+    return GS_OK;
+}
+
+int gs_reset_stats(gs_ctx* c) {
+    if (!c) return GS_ERR_INVALID;
+    c->st = gs_stats{};
+    return GS_OK;
+}
+
+const char* gs_last_error(const gs_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int gs_destroy(gs_ctx* c) {
+    if (!c) return GS_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->d_offsets.reset();
+    c->d_src.reset();
+    c->d_adj.reset();
+    c->d_vlab.reset();
+    c->d_misc.reset();
+    c->pool.release_all();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""Python front end of the MI355X gSpan path.
+
+``fsm(g, k, min_support)`` mirrors the reference's ``void fsm(const CSRGraph&, int k,
+int min_support)`` (gspan.h:41, gspan.cu:1171-1199) but returns the reported patterns
+instead of printing them; ``format_report`` gives the exact lines the reference prints
+(gspan.cu:67-77).  Everything runs in libgspan_amd.so on the GPU; this module only
+marshals arguments.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .graph import CSRGraph
+
+__all__ = ["Miner", "fsm", "format_report"]
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _code_arr(code) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(code, dtype=np.int32).reshape(-1, 5))
+
+
+def _layers(layers):
+    arrs = [np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1, 2)) for x in layers]
+    ptrs = (ctypes.POINTER(ctypes.c_int) * max(len(arrs), 1))(*[_ip(a) for a in arrs])
+    counts = (ctypes.c_longlong * max(len(arrs), 1))(*[a.shape[0] for a in arrs])
+    return arrs, ptrs, counts
+
+
+class Miner:
+    """One device context holding one data graph (``gs_create`` .. ``gs_destroy``)."""
+
+    def __init__(self, g: CSRGraph, mode: str = "batched", mni: str = "gpu", device: int = 0,
+                 timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0):
+        self._lib = L.lib()
+        opts = L.GsOpts()
+        self._lib.gs_default_opts(ctypes.byref(opts))
+        opts.mode = {"plain": L.GS_MODE_PLAIN, "batched": L.GS_MODE_BATCHED}[mode]
+        opts.mni_mode = {"gpu": L.GS_MNI_GPU, "omp-literal": L.GS_MNI_OMP_LITERAL}[mni]
+        opts.device = device
+        opts.timing = 1 if timing else 0
+        opts.budget_bytes = budget_bytes
+        opts.chunk_rows = chunk_rows
+        self.mni_mode = opts.mni_mode
+        self.graph = g
+        h = ctypes.c_void_p()
+        rc = self._lib.gs_create(_ip(g.offsets), _ip(g.vlabels), _ip(g.edges), g.nr_nodes,
+                                 g.nr_edges, ctypes.byref(opts), ctypes.byref(h))
+        if rc != L.GS_OK:
+            raise L.GsError(rc, "gs_create failed (see stderr)")
+        self._h = h
+
+    def _check(self, rc: int):
+        if rc != L.GS_OK:
+            msg = self._lib.gs_last_error(self._h)
+            raise L.GsError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.gs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def mine(self, k: int, min_support: int):
+        """All frequent connected patterns with at most k edges, in the reference's
+        report order: [(code tuple of 5-tuples, support)]."""
+        out = []
+
+        def cb(cp, m, sup, _u):
+            out.append((tuple(tuple(cp[5 * i + j] for j in range(5)) for i in range(m)), sup))
+            return 0
+
+        self._check(self._lib.gs_mine(self._h, k, min_support, L.REPORT_CB(cb), None))
+        return out
+
+    def support(self, code, layers, mni: int | None = None) -> int:
+        c = _code_arr(code)
+        arrs, ptrs, counts = _layers(layers)
+        out = ctypes.c_int()
+        self._check(self._lib.gs_support(self._h, _ip(c), c.shape[0], ptrs, counts,
+                                         self.mni_mode if mni is None else mni,
+                                         ctypes.byref(out)))
+        return out.value
+
+    def extend(self, code, layers, mni: int | None = None):
+        """One extension step: [(edge5, rows int32[count, 2], canonical, support)] in the
+        reference's child order."""
+        c = _code_arr(code)
+        arrs, ptrs, counts = _layers(layers)
+        out = []
+
+        def cb(ep, rp, cnt, canon, sup, _u):
+            e5 = tuple(ep[j] for j in range(5))
+            rows = (np.ctypeslib.as_array(rp, shape=(cnt * 2,)).reshape(cnt, 2).copy()
+                    if cnt else np.zeros((0, 2), np.int32))
+            out.append((e5, rows, bool(canon), sup))
+
+        total = ctypes.c_longlong()
+        self._check(self._lib.gs_extend(self._h, _ip(c), c.shape[0], ptrs, counts,
+                                        self.mni_mode if mni is None else mni, L.CHILD_CB(cb),
+                                        None, ctypes.byref(total)))
+        return out
+
+    def is_canonical(self, codes) -> list[bool]:
+        codes = [_code_arr(c) for c in codes]
+        if not codes:
+            return []
+        flat = np.ascontiguousarray(np.concatenate(codes, axis=0))
+        lens = np.array([c.shape[0] for c in codes], dtype=np.int32)
+        flags = np.zeros(len(codes), dtype=np.int32)
+        self._check(self._lib.gs_is_canonical_batch(self._h, _ip(flat), _ip(lens), len(codes),
+                                                    _ip(flags)))
+        return [bool(x) for x in flags]
+
+    def stats(self) -> dict:
+        s = L.GsStats()
+        self._check(self._lib.gs_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self._lib.gs_reset_stats(self._h))
+
+
+def fsm(g: CSRGraph, k: int, min_support: int, mode: str = "batched", mni: str = "gpu"):
+    """fsm(g, k, min_support) (gspan.cu:1171): the reported patterns, in order."""
+    with Miner(g, mode=mode, mni=mni) as m:
+        return m.mine(k, min_support)
+
+
+def format_report(patterns) -> str:
+    """The reference's stdout lines (gspan.cu:67-77): ``(from to fl el tl)...: support``."""
+    lines = ["".join("(%d %d %d %d %d)" % e for e in code) + ": %d" % sup
+             for code, sup in patterns]
+    return "\n".join(lines) + ("\n" if lines else "")

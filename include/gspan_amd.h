@@ -1,0 +1,124 @@
+/* gspan_amd.h -- C-ABI of the MI355X (gfx950) gSpan single-large-graph mining path.
+ *
+ * The reference (haozixu/gSpan) has no FFI; its seams are C++ functions selected by
+ * editing `mine` (gspan.cu:1118-1169).  Every entry point below replaces one of them:
+ *
+ *   gs_create / gs_destroy   init_gpu_context + init_cpu_context (gspan.cu:97-136): the
+ *                            CSR graph (graph.h:51-65 layout) is copied to the device once
+ *   gs_mine                  fsm(const CSRGraph&, int k, int min_support) (gspan.h:41,
+ *                            gspan.cu:1171-1199) with mine() (gspan.cu:1118-1169); patterns
+ *                            are delivered through a callback in the reference's order
+ *                            (the lines gspan.cu:67-77 prints)
+ *   gs_support               support_gpu (gspan.cu:200-289) / support_omp (gspan.cu:321-374)
+ *   gs_extend                extend_omp (gspan.cu:674-780) for one pattern
+ *   gs_is_canonical_batch    is_canonical (gspan.cu:586-606) for many codes at once
+ *
+ * Conventions: every function returns an int status (GS_OK = 0) and nothing throws
+ * across the ABI; the message of the last failure is available from gs_last_error.
+ * Pointers are plain host pointers; the inputs are only borrowed for the call.  One
+ * context = one device + one HIP stream, used from one host thread at a time.
+ * Pattern codes are arrays of 5 ints per edge: (from, to, from_label, edge_label,
+ * to_label) exactly as PatternEdge (graph.h:28-47), with -1 where the reference stores no
+ * label (from_label of every edge after the first, to_label of backward edges).
+ * Embedding layers are arrays of 2 ints per row: (prev_idx, edge_id) as EmbeddingRel
+ * (gspan.h:18-24), prev_idx = -1 in layer 0.
+ */
+#ifndef GSPAN_AMD_H
+#define GSPAN_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_OK 0
+#define GS_ERR_INVALID 1  /* bad argument or malformed graph                         */
+#define GS_ERR_HIP 2      /* a HIP runtime call failed                               */
+#define GS_ERR_BUDGET 3   /* a single pattern's children exceed the device budget    */
+#define GS_ERR_LIMIT 4    /* pattern larger than the kernels support (k > 7, ...)    */
+#define GS_ERR_CALLBACK 5 /* the report callback asked to stop                       */
+#define GS_ERR_INTERNAL 6 /* kernel-side consistency check failed                    */
+
+#define GS_MODE_PLAIN 0   /* reference DFS order, one parent pattern per launch set   */
+#define GS_MODE_BATCHED 1 /* level-synchronous: every pattern of a DFS level at once   */
+
+#define GS_MNI_GPU 0         /* support_gpu semantics (gspan.cu:165-190), the shipped fsm */
+#define GS_MNI_OMP_LITERAL 1 /* support_omp semantics (gspan.cu:321-374)                  */
+
+#define GS_MAX_PATTERN_EDGES 7
+
+typedef struct gs_ctx gs_ctx;
+
+typedef struct gs_opts {
+    int mode;                        /* GS_MODE_*                                       */
+    int mni_mode;                    /* GS_MNI_*                                        */
+    int device;                      /* HIP device ordinal                              */
+    int timing;                      /* 1: time every extension kernel with HIP events */
+    unsigned long long budget_bytes; /* device bytes for mining buffers; 0 = 85% free  */
+    long long chunk_rows;            /* parent rows per extension pass; 0 = 16M        */
+} gs_opts;
+
+typedef struct gs_stats {
+    long long extended;     /* child rows emitted by extension (gspan.cu:708,717,730)    */
+    long long candidates;   /* candidate codes examined (mine() calls)                   */
+    long long canonical;    /* candidates passing the canonicality check                 */
+    long long reported;     /* frequent patterns reported                                */
+    long long batches;      /* extension batches run                                     */
+    long long items;        /* (row, candidate entry) work items scanned                 */
+    long long parent_rows;  /* parent rows extended                                      */
+    long long fill_launches;
+    long long count_launches;
+    double alg_bytes_fill;  /* algorithmic bytes of all fill launches                    */
+    double t_fill_ms;       /* summed fill-kernel time (timing = 1)                      */
+    double t_count_ms;      /* summed count-kernel time (timing = 1)                     */
+    double t_mine_ms;       /* wall time of the last gs_mine                             */
+    unsigned long long peak_device_bytes;
+} gs_stats;
+
+/* Receives one frequent pattern.  Return 0 to continue, non-zero to stop mining. */
+typedef int (*gs_report_cb)(const int* code5, int len, int support, void* user);
+
+/* Receives one child of gs_extend: its new edge (5 ints), its rows (2 ints each),
+ * whether the child code is canonical, and its MNI support. */
+typedef void (*gs_child_cb)(const int* edge5, const int* rows, long long count, int canonical,
+                            int support, void* user);
+
+/* Fills *opts with the defaults (batched, GPU MNI semantics, device 0). */
+int gs_default_opts(gs_opts* opts);
+
+/* Copies the CSR graph (graph.cc:178-214 layout: offsets[n+1], vlabels[n], edges as
+ * nnz x {from, to, label}) to the device.  Each adjacency must be sorted by (to, label),
+ * as CSRGraph(Graph&) (graph.cc:66-112) writes it. */
+int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, int nnz,
+              const gs_opts* opts, gs_ctx** out);
+
+/* Mines every frequent connected pattern with at most k edges (fsm, gspan.cu:1171). */
+int gs_mine(gs_ctx* ctx, int k, int min_support, gs_report_cb cb, void* user);
+
+/* MNI support of one pattern given its layered rows. */
+int gs_support(gs_ctx* ctx, const int* code5, int m, const int* const* layers,
+               const long long* counts, int mni_mode, int* support_out);
+
+/* One rightmost-path extension of one pattern.  Children arrive in the reference's
+ * recursion order (backward: ascending (v_i, el); forward: descending v_i, then el, tl).
+ * *total = child rows emitted. */
+int gs_extend(gs_ctx* ctx, const int* code5, int m, const int* const* layers,
+              const long long* counts, int mni_mode, gs_child_cb cb, void* user,
+              long long* total);
+
+/* Canonicality of ncodes codes; code i has lens[i] edges and starts at 5*sum(lens[:i]). */
+int gs_is_canonical_batch(gs_ctx* ctx, const int* codes5, const int* lens, int ncodes,
+                          int* flags);
+
+int gs_get_stats(const gs_ctx* ctx, gs_stats* out);
+int gs_reset_stats(gs_ctx* ctx);
+const char* gs_last_error(const gs_ctx* ctx);
+int gs_destroy(gs_ctx* ctx);
+
+/* Library build string (architecture and version). */
+const char* gs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSPAN_AMD_H */

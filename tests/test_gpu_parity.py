@@ -1,0 +1,139 @@
+"""GPU parity: every kernel path against the CPU oracle on the same seeded inputs.
+
+Integer graph work, so the bar is bit-exact: identical pattern sets, identical MNI
+supports, identical child row multisets, identical canonicality flags.
+"""
+import numpy as np
+import pytest
+
+import brute
+from gspan_amd.graph import synthetic, csr_from_undirected
+from gspan_amd.miner import Miner, format_report
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def small_graphs(count, seed0=0):
+    for s in range(count):
+        rng = np.random.default_rng(1000 + seed0 + s)
+        n = int(rng.integers(6, 14))
+        m = int(rng.integers(n - 1, min(24, n * (n - 1) // 2)))
+        yield s, rng, brute.random_small_graph(rng, n, m, int(rng.integers(1, 4)),
+                                               int(rng.integers(1, 3)))
+
+
+def test_canonical_batch_matches_oracle(oracle_mod):
+    _announce("canonical batch vs oracle")
+    g = synthetic("er", n=50, m=100, n_vlabels=2, seed=1)
+    rng = np.random.default_rng(5)
+    codes = []
+    while len(codes) < 2000:
+        c = brute.random_code(rng, 6, int(rng.integers(1, 4)), int(rng.integers(1, 3)))
+        if c[0][2] <= c[0][4]:
+            codes.append(c)
+    with Miner(g) as mn:
+        got = mn.is_canonical(codes)
+    want = [oracle_mod.is_canonical(c) == 1 for c in codes]
+    assert got == want
+    assert 50 < sum(want) < len(want)
+
+
+def test_canonical_star_projection_count():
+    # K_{1,6} with one label has 720 projections of its full code: below the 2048 cap
+    _announce("canonical 6-star")
+    g = synthetic("er", n=50, m=100, n_vlabels=1, seed=1)
+    star = ((0, 1, 0, 0, 0),) + tuple((0, j, -1, 0, 0) for j in range(2, 7))
+    with Miner(g) as mn:
+        assert mn.is_canonical([star]) == [True]
+
+
+@pytest.mark.parametrize("mni", ["gpu", "omp-literal"])
+def test_support_matches_oracle(oracle_mod, mni):
+    for s, rng, g in small_graphs(6):
+        _announce(f"support mni={mni} graph={s}")
+        with Miner(g, mni=mni) as mn:
+            for e5, rows in oracle_mod.seed(g):
+                code, layers = (e5,), [rows]
+                mode = 1 if mni == "omp-literal" else 0
+                assert mn.support(code, layers) == oracle_mod.support(g, code, layers, mode)
+                for ce5, crows in oracle_mod.extend(g, code, layers, nthreads=1)[:4]:
+                    c2, l2 = code + (ce5,), layers + [crows]
+                    assert mn.support(c2, l2) == oracle_mod.support(g, c2, l2, mode)
+
+
+def _canon_rows(rows):
+    return sorted(map(tuple, np.asarray(rows).tolist()))
+
+
+def test_extend_matches_oracle(oracle_mod):
+    for s, rng, g in small_graphs(6, seed0=50):
+        _announce(f"extend graph={s}")
+        with Miner(g) as mn:
+            for e5, rows in oracle_mod.seed(g)[:6]:
+                frontier = [((e5,), [rows])]
+                for depth in range(3):
+                    nxt = []
+                    for code, layers in frontier[:4]:
+                        want = oracle_mod.extend(g, code, layers, nthreads=2)
+                        got = mn.extend(code, layers)
+                        assert [w[0] for w in want] == [x[0] for x in got], code
+                        for (we5, wrows), (ge5, grows, gcanon, gsup) in zip(want, got):
+                            assert _canon_rows(wrows) == _canon_rows(grows), (code, we5)
+                            child = code + (we5,)
+                            assert gcanon == (oracle_mod.is_canonical(child) == 1)
+                            assert gsup == oracle_mod.support(g, child, layers + [wrows], 0)
+                            nxt.append((child, layers + [wrows]))
+                    frontier = nxt
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+@pytest.mark.parametrize("mni", ["gpu", "omp-literal"])
+def test_mining_small_graphs(oracle_mod, mode, mni):
+    for s, rng, g in small_graphs(8, seed0=200):
+        k = int(rng.integers(2, 5))
+        ms = int(rng.integers(1, 4))
+        _announce(f"mine small mode={mode} mni={mni} graph={s} k={k} ms={ms}")
+        want, st = oracle_mod.fsm(g, k, ms, mni_mode=1 if mni == "omp-literal" else 0)
+        with Miner(g, mode=mode, mni=mni) as mn:
+            got = mn.mine(k, ms)
+            gst = mn.stats()
+        assert got == want
+        assert gst["extended"] == st["extended"]
+        assert gst["candidates"] == st["candidates"]
+        assert gst["canonical"] == st["canonical"]
+
+
+def test_mining_er_config_k3(oracle_mod):
+    g = synthetic("er", n=10000, m=40000, n_vlabels=5, n_elabels=1, seed=1)
+    _announce("mine ER 10k/40k k=3 min_sup=200")
+    want, st = oracle_mod.fsm(g, 3, 200)
+    with Miner(g) as mn:
+        got = mn.mine(3, 200)
+        assert mn.stats()["extended"] == st["extended"]
+    assert format_report(got) == oracle_mod.format_report(want)
+
+
+def test_mining_rmat_small(oracle_mod):
+    g = synthetic("rmat", scale=11, edge_factor=8, n_vlabels=8, vlabel_zipf=1.1, n_elabels=2,
+                  seed=1)
+    _announce("mine rmat scale 11 k=3 min_sup=30")
+    want, st = oracle_mod.fsm(g, 3, 30)
+    for mode in ("batched", "plain"):
+        with Miner(g, mode=mode) as mn:
+            got = mn.mine(3, 30)
+            assert mn.stats()["extended"] == st["extended"]
+        assert got == want, mode
+
+
+def test_chunked_and_split_runs_agree(oracle_mod):
+    g = synthetic("er", n=3000, m=12000, n_vlabels=4, n_elabels=2, seed=4)
+    _announce("mine ER 3k chunked / budget-split")
+    want, _ = oracle_mod.fsm(g, 3, 20)
+    with Miner(g, chunk_rows=1000) as mn:
+        assert mn.mine(3, 20) == want
+    with Miner(g, budget_bytes=48 << 20) as mn:
+        assert mn.mine(3, 20) == want
