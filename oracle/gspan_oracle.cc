@@ -517,6 +517,7 @@ struct Miner {
     Stats st;
     Bitmaps bm;
     bool stop = false;
+    double deadline = 0;  // omp_get_wtime() bound for a bounded sample run (0 = none)
 
     Miner(const CSR& g_, int k_, int ms, int mode_, int nt, report_cb cb_, void* u)
         : g(g_), k(k_), min_sup(ms), mode(mode_), nthreads(nt), cb(cb_), user(u) {}
@@ -538,6 +539,10 @@ struct Miner {
     // gspan.cu:1118-1169
     void mine(DFSCode& code, Layers& L) {
         if (stop) return;
+        if (deadline > 0 && omp_get_wtime() > deadline) {
+            stop = true;
+            return;
+        }
         ++st.candidates;
         double t0 = omp_get_wtime();
         bool overrun = false;
@@ -625,15 +630,16 @@ using namespace oracle;
 
 extern "C" {
 
-// Full mining run (gspan.cu:1171-1199).  stats_out[0..7]: extended, candidates, canonical,
-// reported, overruns, t_extend_us, t_support_us, t_canon_us.
-int oracle_fsm(const int* offsets, const int* vlabels, const int* edges3, int n, int nnz, int k,
-               int min_sup, int mni_mode, int nthreads, report_cb cb, void* user,
-               long long* stats_out) {
+// Mining run stopped at the first mine() call after max_seconds (0 = unbounded); returns
+// 2 when the bound cut the run short.  Used for bounded CPU-baseline samples.
+int oracle_fsm_bounded(const int* offsets, const int* vlabels, const int* edges3, int n, int nnz,
+                       int k, int min_sup, int mni_mode, int nthreads, report_cb cb, void* user,
+                       long long* stats_out, double max_seconds) {
     try {
         if (nthreads <= 0) nthreads = omp_get_max_threads();
         CSR g = make_csr(offsets, vlabels, edges3, n, nnz);
         Miner mn(g, k, min_sup, mni_mode, nthreads, cb, user);
+        if (max_seconds > 0) mn.deadline = omp_get_wtime() + max_seconds;
         mn.run();
         if (stats_out) {
             stats_out[0] = mn.st.extended;
@@ -645,11 +651,20 @@ int oracle_fsm(const int* offsets, const int* vlabels, const int* edges3, int n,
             stats_out[6] = (long long)(mn.st.t_support * 1e6);
             stats_out[7] = (long long)(mn.st.t_canon * 1e6);
         }
-        return 0;
+        return mn.stop ? 2 : 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "oracle_fsm: %s\n", e.what());
         return 1;
     }
+}
+
+// Full mining run (gspan.cu:1171-1199).  stats_out[0..7]: extended, candidates, canonical,
+// reported, overruns, t_extend_us, t_support_us, t_canon_us.
+int oracle_fsm(const int* offsets, const int* vlabels, const int* edges3, int n, int nnz, int k,
+               int min_sup, int mni_mode, int nthreads, report_cb cb, void* user,
+               long long* stats_out) {
+    return oracle_fsm_bounded(offsets, vlabels, edges3, n, nnz, k, min_sup, mni_mode, nthreads, cb,
+                              user, stats_out, 0.0);
 }
 
 // is_canonical (gspan.cu:586-606).  Returns 1/0; 2 when the greedy walk ran past the code.

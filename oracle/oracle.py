@@ -44,6 +44,8 @@ def lib():
         L.oracle_fsm.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, c_int, c_int, c_int,
                                  c_int, REPORT_CB, ctypes.c_void_p, P(c_ll)]
         L.oracle_fsm.restype = c_int
+        L.oracle_fsm_bounded.argtypes = L.oracle_fsm.argtypes + [ctypes.c_double]
+        L.oracle_fsm_bounded.restype = c_int
         L.oracle_is_canonical.argtypes = [P(c_int), c_int]
         L.oracle_is_canonical.restype = c_int
         L.oracle_support.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, P(c_int), c_int,
@@ -75,9 +77,10 @@ def _code_arr(code) -> np.ndarray:
 
 
 def fsm(g, k: int, min_sup: int, mni_mode: int = MNI_GPU, nthreads: int = 0,
-        on_report: Callable | None = None):
+        on_report: Callable | None = None, max_seconds: float = 0.0):
     """Full mining run.  Returns (patterns, stats); patterns = [(code tuple, support)]
-    in the reference's report order."""
+    in the reference's report order.  With max_seconds > 0 the run stops at the first
+    pattern visited after that time (stats["truncated"] = 1)."""
     out = []
 
     def cb(cp, m, sup, _u):
@@ -88,13 +91,15 @@ def fsm(g, k: int, min_sup: int, mni_mode: int = MNI_GPU, nthreads: int = 0,
         return 0
 
     st = (ctypes.c_longlong * 8)()
-    rc = lib().oracle_fsm(*_graph_args(g), k, min_sup, mni_mode, nthreads, REPORT_CB(cb), None,
-                          st)
-    if rc != 0:
+    rc = lib().oracle_fsm_bounded(*_graph_args(g), k, min_sup, mni_mode, nthreads, REPORT_CB(cb),
+                                  None, st, float(max_seconds))
+    if rc not in (0, 2):
         raise RuntimeError("oracle_fsm failed")
     names = ["extended", "candidates", "canonical", "reported", "overruns", "t_extend_us",
              "t_support_us", "t_canon_us"]
-    return out, dict(zip(names, list(st)))
+    d = dict(zip(names, list(st)))
+    d["truncated"] = int(rc == 2)
+    return out, d
 
 
 def is_canonical(code) -> int:

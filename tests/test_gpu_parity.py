@@ -42,13 +42,17 @@ def test_canonical_batch_matches_oracle(oracle_mod):
     assert 50 < sum(want) < len(want)
 
 
-def test_canonical_star_projection_count():
-    # K_{1,6} with one label has 720 projections of its full code: below the 2048 cap
+def test_canonical_star_projection_count(oracle_mod):
+    # K_{1,6} with one label: its minimum code starts at a leaf and walks the other five
+    # leaves from the centre; the walk carries up to 6! = 720 projections (cap 2048).
+    # Rooting the code at the centre is not canonical.
     _announce("canonical 6-star")
     g = synthetic("er", n=50, m=100, n_vlabels=1, seed=1)
-    star = ((0, 1, 0, 0, 0),) + tuple((0, j, -1, 0, 0) for j in range(2, 7))
+    leaf = ((0, 1, 0, 0, 0),) + tuple((1, j, -1, 0, 0) for j in range(2, 7))
+    centre = ((0, 1, 0, 0, 0),) + tuple((0, j, -1, 0, 0) for j in range(2, 7))
     with Miner(g) as mn:
-        assert mn.is_canonical([star]) == [True]
+        assert mn.is_canonical([leaf, centre]) == [True, False]
+    assert oracle_mod.is_canonical(leaf) == 1 and oracle_mod.is_canonical(centre) == 0
 
 
 @pytest.mark.parametrize("mni", ["gpu", "omp-literal"])
