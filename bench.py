@@ -1,0 +1,208 @@
+"""Mining benchmark: extended embeddings/s and mining wall time, batched vs plain, 1 MI355X.
+
+A *step* is one full mining run (gs_mine: seeds, every extension level, MNI supports,
+canonicality checks, ordered report) over one synthetic graph already resident in HBM.
+Prints ONE JSON line.
+
+Workloads (seeded, synthetic, reference binary-CSR layout):
+  rmat20    R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), edge factor 8, 20 vertex labels
+            Zipf(1.1), 2 edge labels, MNI min-support 2000; default k = 2
+  rmat20b   the same graph shape with 100 labels Zipf(1.0), 4 edge labels, min-support 500
+  rmat22    R-MAT scale 22, edge factor 16, 50 labels Zipf(1.2), 4 edge labels,
+            min-support 5000
+  er10k     Erdos-Renyi 10,000 vertices / 40,000 edges, 5 uniform labels, 1 edge label,
+            min-support 200, k = 4
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    "rmat20": dict(kind="rmat", scale=20, edge_factor=8, n_vlabels=20, vlabel_zipf=1.1,
+                   n_elabels=2, min_sup=2000, k=2),
+    "rmat20b": dict(kind="rmat", scale=20, edge_factor=8, n_vlabels=100, vlabel_zipf=1.0,
+                    n_elabels=4, min_sup=500, k=2),
+    "rmat22": dict(kind="rmat", scale=22, edge_factor=16, n_vlabels=50, vlabel_zipf=1.2,
+                   n_elabels=4, min_sup=5000, k=2),
+    "er10k": dict(kind="er", n=10000, m=40000, n_vlabels=5, vlabel_zipf=None, n_elabels=1,
+                  min_sup=200, k=4),
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_graph(w: dict):
+    from gspan_amd.graph import synthetic
+    if w["kind"] == "rmat":
+        return synthetic("rmat", scale=w["scale"], edge_factor=w["edge_factor"],
+                         n_vlabels=w["n_vlabels"], vlabel_zipf=w["vlabel_zipf"],
+                         n_elabels=w["n_elabels"], seed=1)
+    return synthetic("er", n=w["n"], m=w["m"], n_vlabels=w["n_vlabels"],
+                     n_elabels=w["n_elabels"], seed=1)
+
+
+def cpu_baseline(g, w: dict, k: int, target_s: float) -> dict:
+    """The oracle (CPU/OpenMP restatement of the reference path) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    if w["kind"] == "er":
+        t0 = time.perf_counter()
+        _, st = oracle.fsm(g, k, w["min_sup"], nthreads=threads, max_seconds=target_s)
+        dt = time.perf_counter() - t0
+        sample = (f"full mining run of the workload (k={k})" if not st["truncated"] else
+                  f"mining in reference order cut after {target_s:.0f}s")
+        return {"value": st["extended"] / dt, "unit": "extended embeddings/s", "cores": threads,
+                "kind": "port", "sample": sample, "seconds": dt}
+    # R-MAT: one extension step of the first frequent single-edge code (reference order),
+    # over a prefix of its rows grown until the step takes ~target/4 seconds
+    seeds = oracle.seed(g)
+    freq = [(e5, rows) for e5, rows in seeds
+            if oracle.support(g, (e5,), [rows], 0, threads) >= w["min_sup"]]
+    e5, rows = freq[0]
+    n = 1 << 12
+    while True:
+        part = rows[:n]
+        t0 = time.perf_counter()
+        kids = oracle.extend(g, (e5,), [part], nthreads=threads)
+        dt = time.perf_counter() - t0
+        emitted = sum(r.shape[0] for _, r in kids)
+        del kids
+        if dt >= target_s / 4 or n >= rows.shape[0]:
+            break
+        n = min(rows.shape[0], n * 4)
+    return {"value": emitted / dt, "unit": "extended embeddings/s", "cores": threads,
+            "kind": "port",
+            "sample": f"extend_omp restatement on the first {part.shape[0]} rows of single-edge "
+                      f"code {tuple(e5)} ({emitted} child rows)", "seconds": dt}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat20", choices=sorted(WORKLOADS))
+    ap.add_argument("--k", type=int, default=0)
+    ap.add_argument("--min-sup", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has a single MI355X, so "
+              f"--gpus {args.gpus} is not run (the path would shard by DFS subtree).",
+              file=sys.stderr)
+        return 2
+
+    from gspan_amd.miner import Miner
+
+    w = dict(WORKLOADS[args.workload])
+    k = args.k or w["k"]
+    ms = args.min_sup or w["min_sup"]
+    t0 = time.perf_counter()
+    g = make_graph(w)
+    gstats = g.stats()
+    log(f"graph {args.workload}: {gstats} ({time.perf_counter() - t0:.1f}s)")
+
+    with Miner(g, mode="batched", timing=True) as mn:
+        for i in range(args.warmup):
+            t = time.perf_counter()
+            pats = mn.mine(k, ms)
+            log(f"warmup {i}: {len(pats)} patterns, {time.perf_counter() - t:.3f}s")
+        mn.reset_stats()
+        times = []
+        for i in range(args.steps):
+            t = time.perf_counter()
+            pats = mn.mine(k, ms)  # synchronous: returns after the last device copy
+            times.append(time.perf_counter() - t)
+            log(f"step {i}: {times[-1]:.3f}s")
+        st = mn.stats()
+    steps = args.steps
+    ext_per_step = st["extended"] / steps
+    total_s = sum(times)
+    value = st["extended"] / total_s
+
+    # dominant extension kernel: per-launch algorithmic bytes / per-launch time (HIP events)
+    fill_ms, count_ms = st["t_fill_ms"], st["t_count_ms"]
+    if fill_ms >= count_ms:
+        kname, kms, kbytes, klaunch = "k_items<true> (fill)", fill_ms, st["alg_bytes_fill"], \
+            st["fill_launches"]
+    else:
+        kname, kms, kbytes, klaunch = "k_items<false> (count)", count_ms, st["alg_bytes_count"], \
+            st["count_launches"]
+    achieved = (kbytes / klaunch) / (kms / klaunch * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": kname, "launches_per_step": klaunch / steps,
+                "avg_launch_ms": kms / max(klaunch, 1),
+                "alg_bytes_per_launch": kbytes / max(klaunch, 1)}
+
+    plain = None
+    if not args.no_plain:
+        with Miner(g, mode="plain") as mp:
+            t = time.perf_counter()
+            ppats = mp.mine(k, ms)
+            pdt = time.perf_counter() - t
+            pst = mp.stats()
+        if ppats != pats:
+            log("ERROR: plain and batched results differ")
+            return 1
+        plain = {"value": pst["extended"] / pdt, "ms_per_step": pdt * 1e3,
+                 "batches": pst["batches"]}
+        log(f"plain: {pdt:.3f}s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(g, w, k, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.3e} /s over {cpu['seconds']:.1f}s")
+
+    out = {
+        "metric": "extended embeddings/s + mining wall-time, batched vs plain, 1 MI355X",
+        "value": value,
+        "unit": "extended embeddings/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_s / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded generator, reference binary-CSR layout)",
+        "config": {
+            "workload": f"{args.workload}: "
+                        + (f"R-MAT scale {w['scale']} ef {w['edge_factor']}, "
+                           if w["kind"] == "rmat" else f"ER n={w['n']} m={w['m']}, ")
+                        + f"{w['n_vlabels']} vlabels"
+                        + (f" Zipf({w['vlabel_zipf']})" if w["vlabel_zipf"] else " uniform")
+                        + f", {w['n_elabels']} elabels, min_sup {ms}, k {k}",
+            "k": k, "min_support": ms, "mode": "batched", "mni": "gpu",
+            "graph": gstats,
+        },
+        "patterns": len(pats),
+        "extended_per_step": ext_per_step,
+        "mining_wall_s": total_s / steps,
+        "plain": plain,
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "peak_device_bytes": st["peak_device_bytes"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

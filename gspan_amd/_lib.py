@@ -52,6 +52,7 @@ class GsStats(ctypes.Structure):
         ("fill_launches", ctypes.c_longlong),
         ("count_launches", ctypes.c_longlong),
         ("alg_bytes_fill", ctypes.c_double),
+        ("alg_bytes_count", ctypes.c_double),
         ("t_fill_ms", ctypes.c_double),
         ("t_count_ms", ctypes.c_double),
         ("t_mine_ms", ctypes.c_double),

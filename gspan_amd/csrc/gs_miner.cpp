@@ -572,11 +572,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
     {
-        double bytes = 16.0 * (double)W_total + 8.0 * (double)emitted;
+        double bytes = 16.0 * (double)W_total;
         for (int i = 0; i < B; ++i)
             bytes += (double)pats[i].row_count *
                      (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
-        c->st.alg_bytes_fill += bytes;
+        c->st.alg_bytes_count += bytes;
+        c->st.alg_bytes_fill += bytes + 8.0 * (double)emitted;
     }
     HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(), d_counts.as<int>(),
                              c->stream));

@@ -68,6 +68,7 @@ typedef struct gs_stats {
     long long fill_launches;
     long long count_launches;
     double alg_bytes_fill;  /* algorithmic bytes of all fill launches                    */
+    double alg_bytes_count; /* algorithmic bytes of all count launches                   */
     double t_fill_ms;       /* summed fill-kernel time (timing = 1)                      */
     double t_count_ms;      /* summed count-kernel time (timing = 1)                     */
     double t_mine_ms;       /* wall time of the last gs_mine                             */
