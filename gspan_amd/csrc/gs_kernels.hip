@@ -14,7 +14,7 @@
 // Work distribution: rows are flattened into items by an exclusive scan of per-row
 // work; persistent workgroups steal fixed tiles of items from an atomic counter and
 // locate their rows by a search over the scanned offsets staged in LDS.  Output slots
-// come from one atomic per (wave, distinct key) instead of one per child.
+// come from one atomic per (tile, distinct key): keys are aggregated per tile in LDS.
 #include "gs_internal.h"
 
 namespace {
@@ -24,30 +24,6 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ unsigned mbcnt64(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                      __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
-// Adds 1 per active lane to counters[key] with one atomic per distinct key in the wave.
-// Returns, for active lanes, the counter value reserved for that lane.  Must be reached
-// by all lanes of the wave.
-__device__ __forceinline__ int wave_reserve(int* counters, int key, bool active) {
-    bool pending = active;
-    int result = 0;
-    for (;;) {
-        unsigned long long pm = __ballot(pending);
-        if (pm == 0) break;
-        int leader = __ffsll((long long)pm) - 1;
-        int k = __shfl(key, leader);
-        bool mine = pending && key == k;
-        unsigned long long gm = __ballot(mine);
-        int base = 0;
-        if (lane_id() == leader) base = atomicAdd(&counters[k], __popcll(gm));
-        base = __shfl(base, leader);
-        if (mine) {
-            result = base + (int)mbcnt64(gm);
-            pending = false;
-        }
-    }
-    return result;
 }
 
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {
@@ -158,58 +134,92 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
 }
 
 // ------------------------------------------------------------------ seeds
+// Every CSR entry with label(from) <= label(to) is a row of the single-edge code
+// (label(from), edge label, label(to)) (gspan.cu:410-426).  Tiles of GS_TILE entries,
+// keys aggregated per tile in LDS as in k_items.
 
-__global__ __launch_bounds__(GS_BLOCK) void k_seed_count(DevGraph g, int EL, int VL,
-                                                         int* key_count, long long* total) {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    long long local = 0;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < g.nnz; base += stride) {
-        long long i = base + threadIdx.x;
-        bool valid = false;
-        int key = 0;
-        if (i < g.nnz) {
-            int fl = g.vlab[g.src[i]];
-            uint2 a = g.adj[i];
-            int tl = (int)(a.y & 0xffffu), el = (int)(a.y >> 16);
-            valid = fl <= tl;  // gspan.cu:421
-            key = (fl * EL + el) * VL + tl;
-        }
-        local += valid;
-        wave_reserve(key_count, key, valid);
+constexpr int SEED_SLOTS = 2 * GS_TILE;
+
+__device__ __forceinline__ int seed_lds_slot(int* s_key, int key) {
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> 21;
+    for (;;) {
+        const int prev = atomicCAS(&s_key[h], -1, key);
+        if (prev == -1 || prev == key) return (int)h;
+        h = (h + 1) & (SEED_SLOTS - 1);
     }
-    // wave reduction of the emitted count, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
-    if (lane_id() == 0 && local) atomicAdd((unsigned long long*)total, (unsigned long long)local);
 }
 
-__global__ __launch_bounds__(GS_BLOCK) void k_seed_fill(DevGraph g, int EL, int VL,
-                                                        const int* key_slot, int* cursor,
-                                                        int2* rows_out, const SetDesc* sets,
-                                                        uint32_t* pool) {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < g.nnz; base += stride) {
-        long long i = base + threadIdx.x;
-        bool valid = false;
-        int slot = -1, from = 0, to = 0;
-        if (i < g.nnz) {
-            from = g.src[i];
-            int fl = g.vlab[from];
-            uint2 a = g.adj[i];
-            to = (int)a.x;
-            int tl = (int)(a.y & 0xffffu), el = (int)(a.y >> 16);
-            if (fl <= tl) {
-                slot = key_slot[(fl * EL + el) * VL + tl];
-                valid = slot >= 0;
+template <bool FILL>
+__global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, int* key_count,
+                                                   long long* total, const int* key_slot,
+                                                   int* cursor, int2* rows_out,
+                                                   const SetDesc* sets, uint32_t* pool) {
+    constexpr int IPT = GS_TILE / GS_BLOCK;
+    __shared__ int s_key[SEED_SLOTS];
+    __shared__ int s_cnt[SEED_SLOTS];
+    __shared__ int s_base[FILL ? SEED_SLOTS : 1];
+    for (int h = threadIdx.x; h < SEED_SLOTS; h += GS_BLOCK) {
+        s_key[h] = -1;
+        s_cnt[h] = 0;
+    }
+    __syncthreads();
+    long long local = 0;
+    for (long long t0 = (long long)blockIdx.x * GS_TILE; t0 < g.nnz;
+         t0 += (long long)gridDim.x * GS_TILE) {
+        int hs[IPT], rank[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) {
+            const long long i = t0 + (long long)q * GS_BLOCK + threadIdx.x;
+            hs[q] = -1;
+            rank[q] = 0;
+            if (i >= g.nnz) continue;
+            const int fl = g.vlab[g.src[i]];
+            const uint2 a = g.adj[i];
+            const int tl = (int)(a.y & 0xffffu), el = (int)(a.y >> 16);
+            if (fl > tl) continue;  // gspan.cu:421
+            const int key = (fl * EL + el) * VL + tl;
+            const int agg = FILL ? key_slot[key] : key;
+            if (agg < 0) continue;
+            ++local;
+            hs[q] = seed_lds_slot(s_key, agg);
+            rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
+        }
+        __syncthreads();
+        for (int h = threadIdx.x; h < SEED_SLOTS; h += GS_BLOCK) {
+            const int key = s_key[h];
+            if (key < 0) continue;
+            if (FILL) {
+                if (rows_out) s_base[h] = atomicAdd(&cursor[key], s_cnt[h]);
+            } else {
+                atomicAdd(&key_count[key], s_cnt[h]);
             }
         }
-        int pos = 0;
-        if (rows_out) pos = wave_reserve(cursor, slot, valid);
-        if (valid) {
-            if (rows_out) rows_out[pos] = make_int2(-1, (int)i);
-            const SetDesc d = sets[slot];
-            set_insert(d, pool, 0, (uint32_t)from);
-            set_insert(d, pool, 1, (uint32_t)to);
+        __syncthreads();
+        if (FILL) {
+#pragma unroll
+            for (int q = 0; q < IPT; ++q) {
+                if (hs[q] < 0) continue;
+                const long long i = t0 + (long long)q * GS_BLOCK + threadIdx.x;
+                const int slot = s_key[hs[q]];
+                if (rows_out) rows_out[s_base[hs[q]] + rank[q]] = make_int2(-1, (int)i);
+                const SetDesc d = sets[slot];
+                set_insert(d, pool, 0, (uint32_t)g.src[i]);
+                set_insert(d, pool, 1, g.adj[i].x);
+            }
         }
+        __syncthreads();
+        for (int h = threadIdx.x; h < SEED_SLOTS; h += GS_BLOCK) {
+            if (s_key[h] >= 0) {
+                s_key[h] = -1;
+                s_cnt[h] = 0;
+            }
+        }
+        __syncthreads();
+    }
+    if (!FILL) {
+        for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+        if (lane_id() == 0 && local)
+            atomicAdd((unsigned long long*)total, (unsigned long long)local);
     }
 }
 
@@ -322,6 +332,20 @@ __global__ __launch_bounds__(GS_BLOCK) void k_scan_down(const int* in, long long
 
 // ------------------------------------------------------------------ extension items
 
+// Per-tile key aggregation in LDS: an open-addressing table (key -> items in this tile)
+// so that a tile issues one global atomic per distinct child key, however hot the key.
+constexpr int ITEMS_PER_THREAD = GS_TILE / GS_BLOCK;
+constexpr int LDS_SLOTS = 2 * GS_TILE;  // > distinct keys of a tile, so probing ends
+
+__device__ __forceinline__ int lds_slot(int* s_key, int key) {
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> 21;  // 11 bits = LDS_SLOTS
+    for (;;) {
+        const int prev = atomicCAS(&s_key[h], -1, key);
+        if (prev == -1 || prev == key) return (int)h;
+        h = (h + 1) & (LDS_SLOTS - 1);
+    }
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
@@ -331,9 +355,17 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     const int* __restrict__ key_slot, int* cursor,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
                                                     uint32_t* pool, int* tile_counter) {
+    static_assert(LDS_SLOTS == 2048, "lds_slot hashes to 11 bits");
     __shared__ long long s_off[GS_TILE + 1];
+    __shared__ int s_key[LDS_SLOTS];
+    __shared__ int s_cnt[LDS_SLOTS];
+    __shared__ int s_base[FILL ? LDS_SLOTS : 1];
     __shared__ long long s_tile, s_rlo;
     if (R <= 0) return;
+    for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+        s_key[h] = -1;
+        s_cnt[h] = 0;
+    }
     const long long W = woff[R];
     const long long ntiles = (W + GS_TILE - 1) / GS_TILE;
     for (;;) {
@@ -359,41 +391,71 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         const int nrows = (int)min((long long)GS_TILE, R - rlo);
         for (int i = threadIdx.x; i <= nrows; i += blockDim.x) s_off[i] = woff[rlo + i];
         __syncthreads();
-        for (long long base = t0; base < t1; base += blockDim.x) {
-            const long long t = base + threadIdx.x;
-            ItemOut o{false, 0, 0, -1};
-            int lo = 0;
+
+        // phase 1: evaluate this thread's items, count them per key in LDS
+        ItemOut o[ITEMS_PER_THREAD];
+        int row[ITEMS_PER_THREAD], hs[ITEMS_PER_THREAD], rank[ITEMS_PER_THREAD];
+#pragma unroll
+        for (int q = 0; q < ITEMS_PER_THREAD; ++q) {
+            const long long t = t0 + (long long)q * GS_BLOCK + threadIdx.x;
+            o[q] = ItemOut{false, 0, 0, -1};
+            row[q] = 0;
+            hs[q] = -1;
+            rank[q] = 0;
             if (t < t1) {
-                int hi = nrows - 1;
+                int lo = 0, hi = nrows - 1;
                 while (lo < hi) {
                     int mid = (lo + hi + 1) >> 1;
                     if (s_off[mid] <= t) lo = mid;
                     else hi = mid - 1;
                 }
+                row[q] = lo;
                 const RowCtx& c = ctx[rlo + lo];
-                o = eval_item(g, pats[c.p], c, t - s_off[lo], ks);
+                o[q] = eval_item(g, pats[c.p], c, t - s_off[lo], ks);
             }
-            if (!FILL) {
-                wave_reserve(key_count, o.key, o.valid);
-            } else {
-                int slot = o.valid ? key_slot[o.key] : -1;
-                bool live = slot >= 0;
-                int pos = 0;
-                if (rows_out) pos = wave_reserve(cursor, slot, live);
-                if (live) {
-                    const RowCtx& c = ctx[rlo + lo];
-                    const PatDev& P = pats[c.p];
-                    if (rows_out) {
-                        long long prow = P.row_begin + (b0 + rlo + lo - P.brow_begin);
-                        rows_out[pos] = make_int2((int)prow, o.eid);
-                    }
-                    const SetDesc d = sets[slot];
-                    for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)c.phi[v]);
-                    if (o.newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o.newv);
-                }
+            int agg = -1;  // aggregation key: child key (count) / live slot (fill)
+            if (o[q].valid) agg = FILL ? key_slot[o[q].key] : o[q].key;
+            if (agg >= 0) {
+                hs[q] = lds_slot(s_key, agg);
+                rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
             }
         }
         __syncthreads();
+        // phase 2: one global atomic per distinct key of the tile
+        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+            const int key = s_key[h];
+            if (key < 0) continue;
+            if (FILL) {
+                if (rows_out) s_base[h] = atomicAdd(&cursor[key], s_cnt[h]);
+            } else {
+                atomicAdd(&key_count[key], s_cnt[h]);
+            }
+        }
+        __syncthreads();
+        // phase 3 (fill): write child rows at their reserved positions, insert images
+        if (FILL) {
+#pragma unroll
+            for (int q = 0; q < ITEMS_PER_THREAD; ++q) {
+                if (hs[q] < 0) continue;
+                const int slot = s_key[hs[q]];
+                const RowCtx& c = ctx[rlo + row[q]];
+                const PatDev& P = pats[c.p];
+                if (rows_out) {
+                    const long long prow = P.row_begin + (b0 + rlo + row[q] - P.brow_begin);
+                    rows_out[s_base[hs[q]] + rank[q]] = make_int2((int)prow, o[q].eid);
+                }
+                const SetDesc d = sets[slot];
+                for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)c.phi[v]);
+                if (o[q].newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o[q].newv);
+            }
+        }
+        __syncthreads();
+        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+            if (s_key[h] >= 0) {
+                s_key[h] = -1;
+                s_cnt[h] = 0;
+            }
+        }
     }
 }
 
@@ -870,15 +932,15 @@ namespace gsk {
 
 hipError_t seed_count(const DevGraph& g, int EL, int VL, int* key_count, long long* total,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_seed_count, dim3(grid_for(g.nnz, GS_BLOCK, 8192)), dim3(GS_BLOCK), 0, s,
-                       g, EL, VL, key_count, total);
+    hipLaunchKernelGGL(k_seed<false>, dim3(grid_for(g.nnz, GS_TILE, 4096)), dim3(GS_BLOCK), 0, s,
+                       g, EL, VL, key_count, total, nullptr, nullptr, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
 hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int* cursor,
                      int2* rows_out, const SetDesc* sets, uint32_t* pool, hipStream_t s) {
-    hipLaunchKernelGGL(k_seed_fill, dim3(grid_for(g.nnz, GS_BLOCK, 8192)), dim3(GS_BLOCK), 0, s,
-                       g, EL, VL, key_slot, cursor, rows_out, sets, pool);
+    hipLaunchKernelGGL(k_seed<true>, dim3(grid_for(g.nnz, GS_TILE, 4096)), dim3(GS_BLOCK), 0, s,
+                       g, EL, VL, nullptr, nullptr, key_slot, cursor, rows_out, sets, pool);
     return hipGetLastError();
 }
 
