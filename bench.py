@@ -92,7 +92,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rmat20", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="er10k", choices=sorted(WORKLOADS))
     ap.add_argument("--k", type=int, default=0)
     ap.add_argument("--min-sup", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
