@@ -118,16 +118,17 @@ def main() -> int:
     with Miner(g, mode="batched", timing=True) as mn:
         for i in range(args.warmup):
             t = time.perf_counter()
-            pats = mn.mine(k, ms)
-            log(f"warmup {i}: {len(pats)} patterns, {time.perf_counter() - t:.3f}s")
+            res = mn.mine_raw(k, ms)
+            log(f"warmup {i}: {res[1].shape[0]} patterns, {time.perf_counter() - t:.3f}s")
         mn.reset_stats()
         times = []
         for i in range(args.steps):
             t = time.perf_counter()
-            pats = mn.mine(k, ms)  # synchronous: returns after the last device copy
+            res = mn.mine_raw(k, ms)  # synchronous: returns after the last device copy
             times.append(time.perf_counter() - t)
             log(f"step {i}: {times[-1]:.3f}s")
         st = mn.stats()
+    pats = Miner.to_patterns(*res)
     steps = args.steps
     ext_per_step = st["extended"] / steps
     total_s = sum(times)
@@ -152,9 +153,10 @@ def main() -> int:
     if not args.no_plain:
         with Miner(g, mode="plain") as mp:
             t = time.perf_counter()
-            ppats = mp.mine(k, ms)
+            pres = mp.mine_raw(k, ms)
             pdt = time.perf_counter() - t
             pst = mp.stats()
+        ppats = Miner.to_patterns(*pres)
         if ppats != pats:
             log("ERROR: plain and batched results differ")
             return 1

@@ -105,13 +105,16 @@ def lib():
     L.gs_extend.argtypes = [vp, P(c_int), c_int, P(P(c_int)), P(c_ll), c_int, CHILD_CB, vp,
                             P(c_ll)]
     L.gs_is_canonical_batch.argtypes = [vp, P(c_int), P(c_int), c_int, P(c_int)]
+    L.gs_result_count.argtypes = [vp, P(c_ll), P(c_ll)]
+    L.gs_result_copy.argtypes = [vp, P(c_int), P(c_int), P(c_int)]
     L.gs_get_stats.argtypes = [vp, P(GsStats)]
     L.gs_reset_stats.argtypes = [vp]
     L.gs_last_error.argtypes = [vp]
     L.gs_last_error.restype = ctypes.c_char_p
     L.gs_destroy.argtypes = [vp]
     for name in ("gs_default_opts", "gs_create", "gs_mine", "gs_support", "gs_extend",
-                 "gs_is_canonical_batch", "gs_get_stats", "gs_reset_stats", "gs_destroy"):
+                 "gs_is_canonical_batch", "gs_result_count", "gs_result_copy", "gs_get_stats",
+                 "gs_reset_stats", "gs_destroy"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

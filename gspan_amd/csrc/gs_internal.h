@@ -97,7 +97,7 @@ hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
 hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                       const long long* woff, long long b0, long long R, KeySpace ks,
                       const int* key_slot, int* cursor, int2* rows_out, const SetDesc* sets,
-                      uint32_t* pool, int* tile_counter, int grid, hipStream_t s);
+                      uint32_t* pool, int* tile_counter, int grid, int nlive, hipStream_t s);
 int items_grid(int cus);
 
 hipError_t compact_active(const int* key_count, long long K, int* active_keys,

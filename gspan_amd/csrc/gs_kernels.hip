@@ -346,6 +346,17 @@ __device__ __forceinline__ int lds_slot(int* s_key, int key) {
     }
 }
 
+// Inserts key into an LDS set; true for the one caller that inserted it.
+__device__ __forceinline__ bool lds_first(int* s_set, int key) {
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> 21;
+    for (;;) {
+        const int prev = atomicCAS(&s_set[h], -1, key);
+        if (prev == -1) return true;
+        if (prev == key) return false;
+        h = (h + 1) & (LDS_SLOTS - 1);
+    }
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
@@ -354,17 +365,21 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     int* key_count,
                                                     const int* __restrict__ key_slot, int* cursor,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
-                                                    uint32_t* pool, int* tile_counter) {
+                                                    uint32_t* pool, int* tile_counter, int dedup) {
     static_assert(LDS_SLOTS == 2048, "lds_slot hashes to 11 bits");
     __shared__ long long s_off[GS_TILE + 1];
     __shared__ int s_key[LDS_SLOTS];
     __shared__ int s_cnt[LDS_SLOTS];
     __shared__ int s_base[FILL ? LDS_SLOTS : 1];
+    // (live slot, row in tile) pairs already seen: only their first item inserts the
+    // parent's vertex images, which are the same for every item of the pair
+    __shared__ int s_pair[FILL ? LDS_SLOTS : 1];
     __shared__ long long s_tile, s_rlo;
     if (R <= 0) return;
     for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
         s_key[h] = -1;
         s_cnt[h] = 0;
+        if (FILL) s_pair[h] = -1;
     }
     const long long W = woff[R];
     const long long ntiles = (W + GS_TILE - 1) / GS_TILE;
@@ -395,6 +410,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         // phase 1: evaluate this thread's items, count them per key in LDS
         ItemOut o[ITEMS_PER_THREAD];
         int row[ITEMS_PER_THREAD], hs[ITEMS_PER_THREAD], rank[ITEMS_PER_THREAD];
+        bool lead[ITEMS_PER_THREAD];
 #pragma unroll
         for (int q = 0; q < ITEMS_PER_THREAD; ++q) {
             const long long t = t0 + (long long)q * GS_BLOCK + threadIdx.x;
@@ -402,6 +418,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             row[q] = 0;
             hs[q] = -1;
             rank[q] = 0;
+            lead[q] = true;
             if (t < t1) {
                 int lo = 0, hi = nrows - 1;
                 while (lo < hi) {
@@ -418,6 +435,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             if (agg >= 0) {
                 hs[q] = lds_slot(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
+                if (FILL && dedup) lead[q] = lds_first(s_pair, agg * GS_TILE + row[q]);
             }
         }
         __syncthreads();
@@ -445,7 +463,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     rows_out[s_base[hs[q]] + rank[q]] = make_int2((int)prow, o[q].eid);
                 }
                 const SetDesc d = sets[slot];
-                for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)c.phi[v]);
+                if (lead[q])
+                    for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)c.phi[v]);
                 if (o[q].newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o[q].newv);
             }
         }
@@ -455,6 +474,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 s_key[h] = -1;
                 s_cnt[h] = 0;
             }
+            if (FILL) s_pair[h] = -1;
         }
     }
 }
@@ -974,17 +994,19 @@ hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                        int* key_count, int* tile_counter, int grid, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_items<false>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, tile_counter);
+                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, tile_counter, 0);
     return hipGetLastError();
 }
 
 hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                       const long long* woff, long long b0, long long R, KeySpace ks,
                       const int* key_slot, int* cursor, int2* rows_out, const SetDesc* sets,
-                      uint32_t* pool, int* tile_counter, int grid, hipStream_t s) {
+                      uint32_t* pool, int* tile_counter, int grid, int nlive, hipStream_t s) {
     if (R <= 0) return hipSuccess;
+    // pair keys slot * GS_TILE + row must stay below 2^31
+    const int dedup = nlive < (1 << 21) ? 1 : 0;
     hipLaunchKernelGGL(k_items<true>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, nullptr, key_slot, cursor, rows_out, sets, pool, tile_counter);
+                       ks, nullptr, key_slot, cursor, rows_out, sets, pool, tile_counter, dedup);
     return hipGetLastError();
 }
 

@@ -213,6 +213,11 @@ struct gs_ctx {
     gs_stats st{};
     size_t budget = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // timing=1: event pairs recorded around count/fill launches, resolved at the next
+    // host synchronisation so that timing adds no synchronisation of its own
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;  // (pool index, 0 = count / 1 = fill)
+    size_t ev_used = 0;
     // mining state
     int k = 0, min_sup = 0, mni_mode = 0;
     std::vector<Report> reports;
@@ -368,6 +373,17 @@ void plan_set(const gs_ctx* c, long long rows, int nv, SetDesc& d, long long& po
     d.pad = 0;
 }
 
+// Folds the event pairs recorded since the last host synchronisation into the stats.
+void resolve_timing(gs_ctx* c) {
+    for (auto& pk : c->ev_pending) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, c->ev_pool[pk.first].first, c->ev_pool[pk.first].second));
+        (pk.second == 0 ? c->st.t_count_ms : c->st.t_fill_ms) += ms;
+    }
+    c->ev_pending.clear();
+    c->ev_used = 0;
+}
+
 struct ChildInfo {
     HPat pat;
     long long count;
@@ -426,6 +442,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
 
     long long W_total = 0;
+    std::vector<long long> Wc_host(nchunks, 0);
     auto prep_chunk = [&](int ci) -> long long {
         const long long b0 = (long long)ci * chunk;
         const long long Rc = std::min(chunk, R - b0);
@@ -436,33 +453,34 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         return Rc;
     };
     auto time_begin = [&]() {
-        if (c->opts.timing) HIPCHECK(hipEventRecord(c->ev0, c->stream));
-    };
-    auto time_end = [&](double& acc) {
-        if (c->opts.timing) {
-            HIPCHECK(hipEventRecord(c->ev1, c->stream));
-            HIPCHECK(hipEventSynchronize(c->ev1));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            acc += ms;
+        if (!c->opts.timing) return;
+        if (c->ev_used == c->ev_pool.size()) {
+            hipEvent_t a, b;
+            HIPCHECK(hipEventCreate(&a));
+            HIPCHECK(hipEventCreate(&b));
+            c->ev_pool.emplace_back(a, b);
         }
+        HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].first, c->stream));
+    };
+    auto time_end = [&](int kind) {
+        if (!c->opts.timing) return;
+        HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].second, c->stream));
+        c->ev_pending.emplace_back((int)c->ev_used, kind);
+        ++c->ev_used;
     };
 
     // ---- pass 1: count children per key
     for (int ci = 0; ci < nchunks; ++ci) {
         const long long b0 = (long long)ci * chunk;
         const long long Rc = prep_chunk(ci);
-        long long Wc = 0;
-        d2h(c, &Wc, d_woff.as<long long>() + Rc, 1);
+        d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
         HIPCHECK(gsk::fill_i32(c->tile_counter(), 0, 1, c->stream));
         time_begin();
         HIPCHECK(gsk::items_count(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                   d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
                                   c->tile_counter(), c->grid_items, c->stream));
-        time_end(c->st.t_count_ms);
+        time_end(0);
         c->st.count_launches++;
-        sync(c);
-        W_total += Wc;
     }
     c->st.items += W_total;
     c->st.parent_rows += R;
@@ -477,6 +495,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     int na_err[2] = {0, 0};
     d2h(c, na_err, c->n_active(), 2);
     sync(c);
+    resolve_timing(c);
+    for (long long w : Wc_host) W_total += w;
     if (na_err[1] & 1)
         throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
     if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
@@ -565,8 +585,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                  d_woff.as<long long>(), b0, Rc, ks, d_slot.as<int>(),
                                  d_cursor.as<int>(), write_rows ? out.rows.as<int2>() : nullptr,
                                  d_sets.as<SetDesc>(), d_pool.as<uint32_t>(), c->tile_counter(),
-                                 c->grid_items, c->stream));
-        time_end(c->st.t_fill_ms);
+                                 c->grid_items, NL, c->stream));
+        time_end(1);
         c->st.fill_launches++;
     }
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
@@ -584,6 +604,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     std::vector<int> counts((size_t)NL * GS_MAXV);
     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
     sync(c);
+    resolve_timing(c);
     int li = 0;
     for (auto& ch : out.children) {
         if (!(ch.canonical || all_live)) continue;
@@ -1097,6 +1118,28 @@ int gs_is_canonical_batch(gs_ctx* c, const int* codes5, const int* lens, int nco
     });
 }
 
+int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
+    if (!c || !count || !total_edges) return GS_ERR_INVALID;
+    long long e = 0;
+    for (const Report& r : c->reports) e += (long long)r.code5.size() / 5;
+    *count = (long long)c->reports.size();
+    *total_edges = e;
+    return GS_OK;
+}
+
+int gs_result_copy(const gs_ctx* c, int* codes5, int* lens, int* supports) {
+    if (!c || (!c->reports.empty() && (!codes5 || !lens || !supports))) return GS_ERR_INVALID;
+    long long at = 0;
+    for (size_t i = 0; i < c->reports.size(); ++i) {
+        const Report& r = c->reports[i];
+        std::memcpy(codes5 + at, r.code5.data(), sizeof(int) * r.code5.size());
+        at += (long long)r.code5.size();
+        lens[i] = (int)r.code5.size() / 5;
+        supports[i] = r.support;
+    }
+    return GS_OK;
+}
+
 int gs_get_stats(const gs_ctx* c, gs_stats* out) {
     if (!c || !out) return GS_ERR_INVALID;
     *out = c->st;
@@ -1120,6 +1163,10 @@ int gs_destroy(gs_ctx* c) {
     c->d_vlab.reset();
     c->d_misc.reset();
     c->pool.release_all();
+    for (auto& e : c->ev_pool) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -78,17 +78,36 @@ class Miner:
     def __exit__(self, *exc):
         self.close()
 
+    def mine_raw(self, k: int, min_support: int):
+        """Runs gs_mine and returns its results as flat arrays in the reference's report
+        order: (codes int32[E, 5], lens int32[P], supports int32[P])."""
+        self._check(self._lib.gs_mine(self._h, k, min_support, L.REPORT_CB(0), None))
+        return self.results()
+
+    def results(self):
+        n, e = ctypes.c_longlong(), ctypes.c_longlong()
+        self._check(self._lib.gs_result_count(self._h, ctypes.byref(n), ctypes.byref(e)))
+        codes = np.zeros((e.value, 5), np.int32)
+        lens = np.zeros(n.value, np.int32)
+        sups = np.zeros(n.value, np.int32)
+        self._check(self._lib.gs_result_copy(self._h, _ip(codes), _ip(lens), _ip(sups)))
+        return codes, lens, sups
+
+    @staticmethod
+    def to_patterns(codes, lens, sups):
+        """Flat result arrays -> [(code tuple of 5-tuples, support)]."""
+        out = []
+        at = 0
+        rows = codes.tolist()
+        for m, sup in zip(lens.tolist(), sups.tolist()):
+            out.append((tuple(tuple(r) for r in rows[at:at + m]), sup))
+            at += m
+        return out
+
     def mine(self, k: int, min_support: int):
         """All frequent connected patterns with at most k edges, in the reference's
         report order: [(code tuple of 5-tuples, support)]."""
-        out = []
-
-        def cb(cp, m, sup, _u):
-            out.append((tuple(tuple(cp[5 * i + j] for j in range(5)) for i in range(m)), sup))
-            return 0
-
-        self._check(self._lib.gs_mine(self._h, k, min_support, L.REPORT_CB(cb), None))
-        return out
+        return self.to_patterns(*self.mine_raw(k, min_support))
 
     def support(self, code, layers, mni: int | None = None) -> int:
         c = _code_arr(code)

@@ -110,6 +110,13 @@ int gs_extend(gs_ctx* ctx, const int* code5, int m, const int* const* layers,
 int gs_is_canonical_batch(gs_ctx* ctx, const int* codes5, const int* lens, int ncodes,
                           int* flags);
 
+/* Results of the last gs_mine, in report order, as flat arrays (for callers that want
+ * them in bulk instead of through a callback; gs_mine may then be called with cb = NULL).
+ * gs_result_count: number of patterns and total code edges.  gs_result_copy fills
+ * codes5[5 * total_edges], lens[count] and supports[count]. */
+int gs_result_count(const gs_ctx* ctx, long long* count, long long* total_edges);
+int gs_result_copy(const gs_ctx* ctx, int* codes5, int* lens, int* supports);
+
 int gs_get_stats(const gs_ctx* ctx, gs_stats* out);
 int gs_reset_stats(gs_ctx* ctx);
 const char* gs_last_error(const gs_ctx* ctx);
