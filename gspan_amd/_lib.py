@@ -37,6 +37,8 @@ class GsOpts(ctypes.Structure):
         ("timing", ctypes.c_int),
         ("budget_bytes", ctypes.c_ulonglong),
         ("chunk_rows", ctypes.c_longlong),
+        ("two_pass", ctypes.c_int),
+        ("reserved", ctypes.c_int),
     ]
 
 

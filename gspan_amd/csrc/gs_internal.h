@@ -14,6 +14,10 @@
 #define GS_BLOCK 256
 #define GS_TILE 1024
 
+// XCD-aware tile queues of the item kernels: one counter per XCD, 16 ints apart.
+#define GS_XCDS 8
+#define GS_XCD_STRIDE 16
+
 // Canonicality kernel: one wave per candidate, projections double-buffered in LDS.
 #define GS_CANON_CAP 2048
 
@@ -91,13 +95,16 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
                           long long tmp_elems, hipStream_t s);
 long long scan_tmp_elems(long long n);
 
+// tile_counters: GS_XCDS * GS_XCD_STRIDE ints, reset by the launchers.
 hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                        const long long* woff, long long b0, long long R, KeySpace ks,
-                       int* key_count, int* tile_counter, int grid, hipStream_t s);
+                       int* key_count, int* tile_counters, int grid, hipStream_t s);
+// key_count (nullable): also count children per key; key_slot == nullptr: slot = key.
 hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                       const long long* woff, long long b0, long long R, KeySpace ks,
-                      const int* key_slot, int* cursor, int2* rows_out, const SetDesc* sets,
-                      uint32_t* pool, int* tile_counter, int grid, int nlive, hipStream_t s);
+                      int* key_count, const int* key_slot, int* cursor, int2* rows_out,
+                      const SetDesc* sets, uint32_t* pool, int* tile_counters, int grid,
+                      int nlive, hipStream_t s);
 int items_grid(int cus);
 
 hipError_t compact_active(const int* key_count, long long K, int* active_keys,

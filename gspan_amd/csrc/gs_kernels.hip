@@ -365,7 +365,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     int* key_count,
                                                     const int* __restrict__ key_slot, int* cursor,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
-                                                    uint32_t* pool, int* tile_counter, int dedup) {
+                                                    uint32_t* pool, int* tile_counters, int dedup) {
     static_assert(LDS_SLOTS == 2048, "lds_slot hashes to 11 bits");
     __shared__ long long s_off[GS_TILE + 1];
     __shared__ int s_key[LDS_SLOTS];
@@ -383,11 +383,29 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     }
     const long long W = woff[R];
     const long long ntiles = (W + GS_TILE - 1) / GS_TILE;
+    // XCD-aware stealing: workgroups are dealt to the 8 XCDs round-robin, so workgroup b
+    // first drains the b % 8-th contiguous eighth of the tiles (whose rows, patterns and
+    // child sets then stay in that XCD's L2), then helps the other eighths.
+    const int x0 = blockIdx.x % GS_XCDS;
+    int xi = 0;  // queues this workgroup has found empty (thread 0 only)
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1);
+        if (threadIdx.x == 0) {
+            long long tile = -1;
+            while (xi < GS_XCDS) {
+                const int x = (x0 + xi) % GS_XCDS;
+                const long long beg = ntiles * x / GS_XCDS, end = ntiles * (x + 1) / GS_XCDS;
+                const long long t = beg + atomicAdd(&tile_counters[x * GS_XCD_STRIDE], 1);
+                if (t < end) {
+                    tile = t;
+                    break;
+                }
+                ++xi;
+            }
+            s_tile = tile;
+        }
         __syncthreads();
         const long long tile = s_tile;
-        if (tile >= ntiles) break;
+        if (tile < 0) break;
         const long long t0 = tile * GS_TILE;
         const long long t1 = min(t0 + GS_TILE, W);
         if (threadIdx.x == 0) {  // row holding item t0
@@ -431,7 +449,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 o[q] = eval_item(g, pats[c.p], c, t - s_off[lo], ks);
             }
             int agg = -1;  // aggregation key: child key (count) / live slot (fill)
-            if (o[q].valid) agg = FILL ? key_slot[o[q].key] : o[q].key;
+            if (o[q].valid) agg = (FILL && key_slot) ? key_slot[o[q].key] : o[q].key;
             if (agg >= 0) {
                 hs[q] = lds_slot(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
@@ -443,11 +461,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
             const int key = s_key[h];
             if (key < 0) continue;
-            if (FILL) {
-                if (rows_out) s_base[h] = atomicAdd(&cursor[key], s_cnt[h]);
-            } else {
-                atomicAdd(&key_count[key], s_cnt[h]);
-            }
+            if (FILL && rows_out) s_base[h] = atomicAdd(&cursor[key], s_cnt[h]);
+            if (key_count) atomicAdd(&key_count[key], s_cnt[h]);
         }
         __syncthreads();
         // phase 3 (fill): write child rows at their reserved positions, insert images
@@ -991,22 +1006,27 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
 
 hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                        const long long* woff, long long b0, long long R, KeySpace ks,
-                       int* key_count, int* tile_counter, int grid, hipStream_t s) {
+                       int* key_count, int* tile_counters, int grid, hipStream_t s) {
     if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
+                       (long long)GS_XCDS * GS_XCD_STRIDE);
     hipLaunchKernelGGL(k_items<false>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, tile_counter, 0);
+                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, tile_counters, 0);
     return hipGetLastError();
 }
 
 hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                       const long long* woff, long long b0, long long R, KeySpace ks,
-                      const int* key_slot, int* cursor, int2* rows_out, const SetDesc* sets,
-                      uint32_t* pool, int* tile_counter, int grid, int nlive, hipStream_t s) {
+                      int* key_count, const int* key_slot, int* cursor, int2* rows_out,
+                      const SetDesc* sets, uint32_t* pool, int* tile_counters, int grid,
+                      int nlive, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     // pair keys slot * GS_TILE + row must stay below 2^31
     const int dedup = nlive < (1 << 21) ? 1 : 0;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
+                       (long long)GS_XCDS * GS_XCD_STRIDE);
     hipLaunchKernelGGL(k_items<true>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, nullptr, key_slot, cursor, rows_out, sets, pool, tile_counter, dedup);
+                       ks, key_count, key_slot, cursor, rows_out, sets, pool, tile_counters, dedup);
     return hipGetLastError();
 }
 

@@ -207,7 +207,8 @@ struct gs_ctx {
     DevPool pool;
     DBuf d_offsets, d_src, d_adj, d_vlab;
     DevGraph dg{};
-    DBuf d_misc;  // [0] tile counter, [1] n_active, [2] err, [4..5] int64 total
+    DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [4..5] int64 total
+    DBuf d_tiles; // per-XCD tile queues of the item kernels
     int grid_items = 0;
     std::string err;
     gs_stats st{};
@@ -222,7 +223,7 @@ struct gs_ctx {
     int k = 0, min_sup = 0, mni_mode = 0;
     std::vector<Report> reports;
 
-    int* tile_counter() const { return d_misc.as<int>(); }
+    int* tile_counters() const { return d_tiles.as<int>(); }
     int* n_active() const { return d_misc.as<int>() + 1; }
     int* err_flag() const { return d_misc.as<int>() + 2; }
     long long* total64() const { return reinterpret_cast<long long*>(d_misc.as<int>() + 4); }
@@ -441,6 +442,45 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     DBuf d_kc(P, sizeof(int) * K);
     HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
 
+    // Single pass at the last level: when a bitmap for every potential child key fits in a
+    // quarter of the free budget, the fill pass counts children and fills the MNI sets in
+    // one go (no count pass); canonicality is then checked on the keys that occurred.
+    const long long bitmap_words = (((long long)c->n + 31) / 32 + 3) / 4 * 4;
+    std::vector<SetDesc> all_sets;
+    long long all_words = 0;
+    bool single = false;
+    if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
+        long long words = 0;
+        for (int i = 0; i < B; ++i) {
+            const long long nb = (long long)(pd[i].r - 1) * c->EL;
+            const long long nf = (long long)(pd[i].r + 1) * c->EL * c->VL;
+            words += (nb * pd[i].nv + nf * (pd[i].nv + 1)) * bitmap_words;
+        }
+        const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
+        single = (size_t)words * 4 <= avail / 4;
+    }
+    DBuf d_all_sets, d_all_pool;
+    if (single) {
+        all_sets.resize(K);
+        for (int i = 0; i < B; ++i) {
+            const int K_p = (int)key_space(pd[i].r, c->EL, c->VL);
+            const int nb = (pd[i].r - 1) * c->EL;
+            for (int sl = 0; sl < K_p; ++sl) {
+                SetDesc& d = all_sets[pd[i].key_base + sl];
+                d.kind = GS_SET_BITMAP;
+                d.size = (int)bitmap_words;
+                d.nv = pd[i].nv + (sl < nb ? 0 : 1);
+                d.off = all_words;
+                d.pad = 0;
+                all_words += (long long)d.nv * bitmap_words;
+            }
+        }
+        d_all_sets = DBuf(P, sizeof(SetDesc) * K);
+        d_all_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(all_words, 4));
+        h2d(c, d_all_sets.as<SetDesc>(), all_sets.data(), K);
+        HIPCHECK(gsk::fill_u32(d_all_pool.as<uint32_t>(), 0u, all_words, c->stream));
+    }
+
     long long W_total = 0;
     std::vector<long long> Wc_host(nchunks, 0);
     auto prep_chunk = [&](int ci) -> long long {
@@ -469,20 +509,28 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         ++c->ev_used;
     };
 
-    // ---- pass 1: count children per key
+    // ---- pass 1: count children per key (single pass: also fill every key's sets)
     for (int ci = 0; ci < nchunks; ++ci) {
         const long long b0 = (long long)ci * chunk;
         const long long Rc = prep_chunk(ci);
         d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
-        HIPCHECK(gsk::fill_i32(c->tile_counter(), 0, 1, c->stream));
         time_begin();
-        HIPCHECK(gsk::items_count(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
-                                  d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
-                                  c->tile_counter(), c->grid_items, c->stream));
-        time_end(0);
-        c->st.count_launches++;
+        if (single) {
+            HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                                     d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
+                                     nullptr, nullptr, d_all_sets.as<SetDesc>(),
+                                     d_all_pool.as<uint32_t>(), c->tile_counters(),
+                                     c->grid_items, (int)K, c->stream));
+            time_end(1);
+            c->st.fill_launches++;
+        } else {
+            HIPCHECK(gsk::items_count(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
+                                      c->tile_counters(), c->grid_items, c->stream));
+            time_end(0);
+            c->st.count_launches++;
+        }
     }
-    c->st.items += W_total;
     c->st.parent_rows += R;
 
     // ---- candidates and their canonicality
@@ -497,6 +545,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     sync(c);
     resolve_timing(c);
     for (long long w : Wc_host) W_total += w;
+    c->st.items += W_total;
     if (na_err[1] & 1)
         throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
     if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
@@ -532,7 +581,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         emitted += acnt[a];
         if (ci.canonical || all_live) {
             SetDesc d;
-            plan_set(c, ci.count, ci.pat.nv(), d, pool_words);
+            if (single) d = all_sets[key];
+            else plan_set(c, ci.count, ci.pat.nv(), d, pool_words);
             sets.push_back(d);
             live_keys.push_back(key);
             if (write_rows) {
@@ -563,31 +613,34 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         return STEP_OK;
     }
 
-    // ---- pass 2: write child rows and fill the MNI sets
-    DBuf d_slot(P, sizeof(int) * K);
-    DBuf d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(int) * NL);
+    // ---- pass 2: write child rows and fill the MNI sets (two-pass mode)
     DBuf d_sets(P, sizeof(SetDesc) * NL);
-    DBuf d_pool(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
     DBuf d_counts(P, sizeof(int) * NL * GS_MAXV);
-    if (write_rows) out.rows = DBuf(P, sizeof(int2) * std::max<long long>(rows_total, 1));
-    HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
-    h2d(c, d_live.as<int>(), live_keys.data(), NL);
-    h2d(c, d_cursor.as<int>(), cursor.data(), NL);
+    DBuf d_pool;
     h2d(c, d_sets.as<SetDesc>(), sets.data(), NL);
-    HIPCHECK(gsk::scatter_slots(d_slot.as<int>(), d_live.as<int>(), NL, c->stream));
-    HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
-    for (int ci = 0; ci < nchunks; ++ci) {
-        const long long b0 = (long long)ci * chunk;
-        const long long Rc = nchunks > 1 ? prep_chunk(ci) : std::min(chunk, R - b0);
-        HIPCHECK(gsk::fill_i32(c->tile_counter(), 0, 1, c->stream));
-        time_begin();
-        HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
-                                 d_woff.as<long long>(), b0, Rc, ks, d_slot.as<int>(),
-                                 d_cursor.as<int>(), write_rows ? out.rows.as<int2>() : nullptr,
-                                 d_sets.as<SetDesc>(), d_pool.as<uint32_t>(), c->tile_counter(),
-                                 c->grid_items, NL, c->stream));
-        time_end(1);
-        c->st.fill_launches++;
+    if (!single) {
+        DBuf d_slot(P, sizeof(int) * K);
+        DBuf d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(int) * NL);
+        d_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
+        if (write_rows) out.rows = DBuf(P, sizeof(int2) * std::max<long long>(rows_total, 1));
+        HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
+        h2d(c, d_live.as<int>(), live_keys.data(), NL);
+        h2d(c, d_cursor.as<int>(), cursor.data(), NL);
+        HIPCHECK(gsk::scatter_slots(d_slot.as<int>(), d_live.as<int>(), NL, c->stream));
+        HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
+        for (int ci = 0; ci < nchunks; ++ci) {
+            const long long b0 = (long long)ci * chunk;
+            const long long Rc = nchunks > 1 ? prep_chunk(ci) : std::min(chunk, R - b0);
+            time_begin();
+            HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                                     d_woff.as<long long>(), b0, Rc, ks, nullptr,
+                                     d_slot.as<int>(), d_cursor.as<int>(),
+                                     write_rows ? out.rows.as<int2>() : nullptr,
+                                     d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
+                                     c->tile_counters(), c->grid_items, NL, c->stream));
+            time_end(1);
+            c->st.fill_launches++;
+        }
     }
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
@@ -596,11 +649,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         for (int i = 0; i < B; ++i)
             bytes += (double)pats[i].row_count *
                      (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
-        c->st.alg_bytes_count += bytes;
+        if (!single) c->st.alg_bytes_count += bytes;
         c->st.alg_bytes_fill += bytes + 8.0 * (double)emitted;
     }
-    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(), d_counts.as<int>(),
-                             c->stream));
+    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
+                             single ? d_all_pool.as<uint32_t>() : d_pool.as<uint32_t>(),
+                             d_counts.as<int>(), c->stream));
     std::vector<int> counts((size_t)NL * GS_MAXV);
     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
     sync(c);
@@ -938,6 +992,7 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->d_adj = DBuf(P, sizeof(uint2) * std::max(nnz, 1));
         x->d_vlab = DBuf(P, sizeof(int) * std::max(n, 1));
         x->d_misc = DBuf(P, 64);
+        x->d_tiles = DBuf(P, sizeof(int) * GS_XCDS * GS_XCD_STRIDE);
         h2d(x, x->d_offsets.as<int>(), offsets, n + 1);
         h2d(x, x->d_src.as<int>(), src.data(), nnz);
         h2d(x, x->d_adj.as<uint2>(), adj.data(), nnz);
@@ -1162,6 +1217,7 @@ int gs_destroy(gs_ctx* c) {
     c->d_adj.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
+    c->d_tiles.reset();
     c->pool.release_all();
     for (auto& e : c->ev_pool) {
         (void)hipEventDestroy(e.first);

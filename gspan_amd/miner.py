@@ -37,7 +37,8 @@ class Miner:
     """One device context holding one data graph (``gs_create`` .. ``gs_destroy``)."""
 
     def __init__(self, g: CSRGraph, mode: str = "batched", mni: str = "gpu", device: int = 0,
-                 timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0):
+                 timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0,
+                 two_pass: bool = False):
         self._lib = L.lib()
         opts = L.GsOpts()
         self._lib.gs_default_opts(ctypes.byref(opts))
@@ -47,6 +48,7 @@ class Miner:
         opts.timing = 1 if timing else 0
         opts.budget_bytes = budget_bytes
         opts.chunk_rows = chunk_rows
+        opts.two_pass = 1 if two_pass else 0
         self.mni_mode = opts.mni_mode
         self.graph = g
         h = ctypes.c_void_p()

@@ -55,6 +55,9 @@ typedef struct gs_opts {
     int timing;                      /* 1: time every extension kernel with HIP events */
     unsigned long long budget_bytes; /* device bytes for mining buffers; 0 = 85% free  */
     long long chunk_rows;            /* parent rows per extension pass; 0 = 16M        */
+    int two_pass;                    /* 1: always count, then fill (no single-pass last
+                                        level)                                          */
+    int reserved;
 } gs_opts;
 
 typedef struct gs_stats {

@@ -135,9 +135,14 @@ def test_mining_rmat_small(oracle_mod):
 
 def test_chunked_and_split_runs_agree(oracle_mod):
     g = synthetic("er", n=3000, m=12000, n_vlabels=4, n_elabels=2, seed=4)
-    _announce("mine ER 3k chunked / budget-split")
-    want, _ = oracle_mod.fsm(g, 3, 20)
+    _announce("mine ER 3k chunked / budget-split / two-pass")
+    want, st = oracle_mod.fsm(g, 3, 20)
     with Miner(g, chunk_rows=1000) as mn:
         assert mn.mine(3, 20) == want
     with Miner(g, budget_bytes=48 << 20) as mn:
         assert mn.mine(3, 20) == want
+    for mode in ("batched", "plain"):
+        with Miner(g, mode=mode, two_pass=True) as mn:
+            assert mn.mine(3, 20) == want
+            s = mn.stats()
+            assert s["extended"] == st["extended"] and s["count_launches"] > 0
