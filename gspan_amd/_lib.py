@@ -24,6 +24,9 @@ GS_ERR_INTERNAL = 6
 
 GS_MODE_PLAIN = 0
 GS_MODE_BATCHED = 1
+GS_MODE_PREDICTED = 2
+GS_MLP_FEATURES = 32
+GS_MLP_PARAMS = 20865
 GS_MNI_GPU = 0
 GS_MNI_OMP_LITERAL = 1
 GS_MAX_PATTERN_EDGES = 7
@@ -59,6 +62,8 @@ class GsStats(ctypes.Structure):
         ("t_count_ms", ctypes.c_double),
         ("t_mine_ms", ctypes.c_double),
         ("peak_device_bytes", ctypes.c_ulonglong),
+        ("predicted_steps", ctypes.c_longlong),
+        ("fallbacks", ctypes.c_longlong),
     ]
 
     def as_dict(self) -> dict:
@@ -107,6 +112,14 @@ def lib():
     L.gs_extend.argtypes = [vp, P(c_int), c_int, P(P(c_int)), P(c_ll), c_int, CHILD_CB, vp,
                             P(c_ll)]
     L.gs_is_canonical_batch.argtypes = [vp, P(c_int), P(c_int), c_int, P(c_int)]
+    c_f = ctypes.c_float
+    L.gs_predictor_log.argtypes = [vp, c_int, c_ll]
+    L.gs_predictor_samples.argtypes = [vp, P(c_ll)]
+    L.gs_predictor_train.argtypes = [vp, c_int, c_f, P(c_f)]
+    L.gs_predictor_get_weights.argtypes = [vp, P(c_f)]
+    L.gs_predictor_set_weights.argtypes = [vp, P(c_f)]
+    L.gs_mlp_forward.argtypes = [vp, P(c_f), c_ll, P(c_f)]
+    L.gs_mlp_gradients.argtypes = [vp, P(c_f), P(c_f), c_ll, P(c_f), P(c_f)]
     L.gs_result_count.argtypes = [vp, P(c_ll), P(c_ll)]
     L.gs_result_copy.argtypes = [vp, P(c_int), P(c_int), P(c_int)]
     L.gs_get_stats.argtypes = [vp, P(GsStats)]
@@ -116,7 +129,9 @@ def lib():
     L.gs_destroy.argtypes = [vp]
     for name in ("gs_default_opts", "gs_create", "gs_mine", "gs_support", "gs_extend",
                  "gs_is_canonical_batch", "gs_result_count", "gs_result_copy", "gs_get_stats",
-                 "gs_reset_stats", "gs_destroy"):
+                 "gs_reset_stats", "gs_destroy", "gs_predictor_log", "gs_predictor_samples",
+                 "gs_predictor_train", "gs_predictor_get_weights", "gs_predictor_set_weights",
+                 "gs_mlp_forward", "gs_mlp_gradients"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

@@ -21,6 +21,11 @@
 // Canonicality kernel: one wave per candidate, projections double-buffered in LDS.
 #define GS_CANON_CAP 2048
 
+// Child-count predictor: MLP GS_FEATURES -> GS_HIDDEN -> GS_HIDDEN -> 1.
+#define GS_FEATURES 32
+#define GS_HIDDEN 128
+#define GS_MLP_PARAMS (GS_FEATURES * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN + 1)
+
 // Set kinds for MNI image sets.
 #define GS_SET_BITMAP 0
 #define GS_SET_HASH 1
@@ -37,6 +42,8 @@ struct PatDev {
     int cfrom[GS_MAXE], cto[GS_MAXE], cel[GS_MAXE];
     int vlab[GS_MAXV];  // label rank of every pattern vertex
     int rm[GS_MAXE];    // rightmost path (code edge indices, last forward edge first)
+    int support;           // MNI support of the pattern
+    int pad0;
     long long row_begin;   // first row in the parent level array
     long long row_count;   // rows of this pattern
     long long brow_begin;  // first row in the batch row space
@@ -80,6 +87,16 @@ struct DevGraph {
     const int* vlab;     // n
 };
 
+// MLP parameters on the device: w32 holds every parameter in fp32 (W1[32x128], b1, W2[128x128],
+// b2, W3[128], b3); w16_all a bf16 copy in the same layout.  Layers read bf16 weights and
+// fp32 biases.
+struct MlpDev {
+    float* w32;
+    uint16_t* w16_all;
+    const uint16_t* w16[3];
+    const float* b[3];
+};
+
 // ------------------------------------------------------------ kernel launchers
 // All launches go to `stream`; every size argument is checked by the caller.
 namespace gsk {
@@ -100,11 +117,13 @@ hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                        const long long* woff, long long b0, long long R, KeySpace ks,
                        int* key_count, int* tile_counters, int grid, hipStream_t s);
 // key_count (nullable): also count children per key; key_slot == nullptr: slot = key.
+// cap_end (nullable): rows of slot s may only be written below cap_end[s]; a child that
+// does not fit, or a hash set that fills up, sets *overflow instead (predicted sizes).
 hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                       const long long* woff, long long b0, long long R, KeySpace ks,
-                      int* key_count, const int* key_slot, int* cursor, int2* rows_out,
-                      const SetDesc* sets, uint32_t* pool, int* tile_counters, int grid,
-                      int nlive, hipStream_t s);
+                      int* key_count, const int* key_slot, int* cursor, const int* cap_end,
+                      int2* rows_out, const SetDesc* sets, uint32_t* pool, int* overflow,
+                      int* tile_counters, int grid, int nlive, hipStream_t s);
 int items_grid(int cus);
 
 hipError_t compact_active(const int* key_count, long long K, int* active_keys,
@@ -120,5 +139,19 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool, int*
                       hipStream_t s);
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
                         int mni_mode, const SetDesc* set, uint32_t* pool, hipStream_t s);
+
+// predictor
+hipError_t features(const PatDev* pats, int npats, KeySpace ks, long long K, const float* H,
+                    const float* vfreq, const float* efreq, const float* dmean, float nverts,
+                    float* X, unsigned char* possible, hipStream_t s);
+hipError_t log_targets(const int* counts, long long K, float* t, hipStream_t s);
+hipError_t mlp_forward(const float* X, long long N, const MlpDev& w, float* h1, float* h2,
+                       float* y, hipStream_t s);
+hipError_t mlp_backward(const float* X, const float* t, long long N, const MlpDev& w,
+                        const float* h1, const float* h2, const float* y, float* dy, float* dh2,
+                        float* dh1, float* grads, float* loss, hipStream_t s);
+hipError_t mlp_adam(MlpDev& w, float* grads, float* mo, float* ve, int step, float lr,
+                    hipStream_t s);
+hipError_t mlp_sync_bf16(MlpDev& w, hipStream_t s);
 
 }  // namespace gsk

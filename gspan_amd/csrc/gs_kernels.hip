@@ -34,7 +34,10 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 
 // Inserts graph vertex x into set v of descriptor d.  Bits / slots only go from empty
 // to full, so a stale plain read can only send us to the atomic, never skip a new key.
-__device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int v, uint32_t x) {
+// A table sized from exact counts (>= 2x the rows) never fills; one sized from predicted
+// counts can, and then raises *overflow after probing every slot once.
+__device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int v, uint32_t x,
+                                           int* overflow = nullptr) {
     if (d.kind == GS_SET_BITMAP) {
         uint32_t* w = pool + d.off + (long long)v * d.size + (x >> 5);
         uint32_t bit = 1u << (x & 31);
@@ -44,7 +47,7 @@ __device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int
         uint32_t mask = (uint32_t)d.size - 1;
         uint32_t key = x + 1;  // 0 marks an empty slot
         uint32_t h = hash32(x) & mask;
-        for (;;) {
+        for (int probe = 0; probe < d.size; ++probe) {
             uint32_t cur = __hip_atomic_load(base + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (cur == key) return;
             if (cur == 0) {
@@ -53,6 +56,7 @@ __device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int
             }
             h = (h + 1) & mask;
         }
+        if (overflow) atomicOr(overflow, 1);
     }
 }
 
@@ -364,8 +368,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     long long b0, long long R, KeySpace ks,
                                                     int* key_count,
                                                     const int* __restrict__ key_slot, int* cursor,
+                                                    const int* __restrict__ cap_end,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
-                                                    uint32_t* pool, int* tile_counters, int dedup) {
+                                                    uint32_t* pool, int* overflow,
+                                                    int* tile_counters, int dedup) {
     static_assert(LDS_SLOTS == 2048, "lds_slot hashes to 11 bits");
     __shared__ long long s_off[GS_TILE + 1];
     __shared__ int s_key[LDS_SLOTS];
@@ -475,12 +481,15 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 const PatDev& P = pats[c.p];
                 if (rows_out) {
                     const long long prow = P.row_begin + (b0 + rlo + row[q] - P.brow_begin);
-                    rows_out[s_base[hs[q]] + rank[q]] = make_int2((int)prow, o[q].eid);
+                    const int at = s_base[hs[q]] + rank[q];
+                    if (!cap_end || at < cap_end[slot]) rows_out[at] = make_int2((int)prow, o[q].eid);
+                    else atomicOr(overflow, 1);
                 }
                 const SetDesc d = sets[slot];
                 if (lead[q])
-                    for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)c.phi[v]);
-                if (o[q].newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o[q].newv);
+                    for (int v = 0; v < P.nv; ++v)
+                        set_insert(d, pool, v, (uint32_t)c.phi[v], overflow);
+                if (o[q].newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o[q].newv, overflow);
             }
         }
         __syncthreads();
@@ -1011,22 +1020,24 @@ hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
                        (long long)GS_XCDS * GS_XCD_STRIDE);
     hipLaunchKernelGGL(k_items<false>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, tile_counters, 0);
+                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       tile_counters, 0);
     return hipGetLastError();
 }
 
 hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
                       const long long* woff, long long b0, long long R, KeySpace ks,
-                      int* key_count, const int* key_slot, int* cursor, int2* rows_out,
-                      const SetDesc* sets, uint32_t* pool, int* tile_counters, int grid,
-                      int nlive, hipStream_t s) {
+                      int* key_count, const int* key_slot, int* cursor, const int* cap_end,
+                      int2* rows_out, const SetDesc* sets, uint32_t* pool, int* overflow,
+                      int* tile_counters, int grid, int nlive, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     // pair keys slot * GS_TILE + row must stay below 2^31
     const int dedup = nlive < (1 << 21) ? 1 : 0;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
                        (long long)GS_XCDS * GS_XCD_STRIDE);
     hipLaunchKernelGGL(k_items<true>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, key_count, key_slot, cursor, rows_out, sets, pool, tile_counters, dedup);
+                       ks, key_count, key_slot, cursor, cap_end, rows_out, sets, pool, overflow,
+                       tile_counters, dedup);
     return hipGetLastError();
 }
 

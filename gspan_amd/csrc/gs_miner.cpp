@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -207,7 +208,7 @@ struct gs_ctx {
     DevPool pool;
     DBuf d_offsets, d_src, d_adj, d_vlab;
     DevGraph dg{};
-    DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [4..5] int64 total
+    DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
     DBuf d_tiles; // per-XCD tile queues of the item kernels
     int grid_items = 0;
     std::string err;
@@ -222,10 +223,20 @@ struct gs_ctx {
     // mining state
     int k = 0, min_sup = 0, mni_mode = 0;
     std::vector<Report> reports;
+    // child-count predictor: label statistics, parameters, Adam state, logged samples
+    DBuf d_H, d_vfreq, d_efreq, d_dmean;
+    DBuf d_w32, d_w16, d_adam_m, d_adam_v, d_grads;
+    MlpDev mlp{};
+    int mlp_step = 0;
+    bool mlp_trained = false;
+    bool logging = false;
+    long long log_cap = 0, n_logged = 0;
+    DBuf d_logX, d_logT;
 
     int* tile_counters() const { return d_tiles.as<int>(); }
     int* n_active() const { return d_misc.as<int>() + 1; }
     int* err_flag() const { return d_misc.as<int>() + 2; }
+    int* overflow_flag() const { return d_misc.as<int>() + 3; }
     long long* total64() const { return reinterpret_cast<long long*>(d_misc.as<int>() + 4); }
 };
 
@@ -266,6 +277,7 @@ PatDev to_dev(const HPat& h) {
         d.cel[j] = j < h.m ? h.el[j] : 0;
     }
     h.labels(d.vlab);
+    d.support = h.support;
     d.row_begin = h.row_begin;
     d.row_count = h.row_count;
     return d;
@@ -372,6 +384,91 @@ void plan_set(const gs_ctx* c, long long rows, int nv, SetDesc& d, long long& po
     pool_words += (long long)nv * d.size;
     pool_words = (pool_words + 3) / 4 * 4;
     d.pad = 0;
+}
+
+// Label statistics for the predictor's features: vertex-label and edge-label frequencies,
+// mean degree per vertex label, and H[a][e][b] = mean number of neighbours with edge label e
+// and vertex label b of a vertex labelled a.
+void init_label_stats(gs_ctx* c, const std::vector<int>& vr, const std::vector<int>& src,
+                      const std::vector<uint2>& adj) {
+    const int VL = c->VL, EL = c->EL;
+    std::vector<double> vcount(VL, 0), ecount(EL, 0), deg(VL, 0);
+    for (int v = 0; v < c->n; ++v) vcount[vr[v]] += 1;
+    const long long HS = (long long)VL * EL * VL;
+    const bool useH = HS <= (1LL << 24);
+    std::vector<float> H(useH ? HS : 1, 0.f);
+    for (int i = 0; i < c->nnz; ++i) {
+        const int a = vr[src[i]], e = (int)(adj[i].y >> 16), b = (int)(adj[i].y & 0xffffu);
+        ecount[e] += 1;
+        deg[a] += 1;
+        if (useH) H[((long long)a * EL + e) * VL + b] += 1.f;
+    }
+    std::vector<float> vf(VL), ef(EL), dm(VL);
+    for (int a = 0; a < VL; ++a) {
+        vf[a] = c->n ? (float)(vcount[a] / c->n) : 0.f;
+        dm[a] = vcount[a] > 0 ? (float)(deg[a] / vcount[a]) : 0.f;
+        if (useH && vcount[a] > 0)
+            for (long long q = 0; q < (long long)EL * VL; ++q)
+                H[(long long)a * EL * VL + q] /= (float)vcount[a];
+    }
+    for (int e = 0; e < EL; ++e) ef[e] = c->nnz ? (float)(ecount[e] / c->nnz) : 0.f;
+    DevPool* P = &c->pool;
+    c->d_H = useH ? DBuf(P, sizeof(float) * HS) : DBuf();
+    c->d_vfreq = DBuf(P, sizeof(float) * VL);
+    c->d_efreq = DBuf(P, sizeof(float) * EL);
+    c->d_dmean = DBuf(P, sizeof(float) * VL);
+    if (useH) h2d(c, c->d_H.as<float>(), H.data(), (size_t)HS);
+    h2d(c, c->d_vfreq.as<float>(), vf.data(), VL);
+    h2d(c, c->d_efreq.as<float>(), ef.data(), EL);
+    h2d(c, c->d_dmean.as<float>(), dm.data(), VL);
+}
+
+void mlp_bind(gs_ctx* c) {
+    float* w = c->d_w32.as<float>();
+    uint16_t* h = c->d_w16.as<uint16_t>();
+    const long long oW1 = 0, ob1 = oW1 + GS_FEATURES * GS_HIDDEN, oW2 = ob1 + GS_HIDDEN,
+                    ob2 = oW2 + GS_HIDDEN * GS_HIDDEN, oW3 = ob2 + GS_HIDDEN, ob3 = oW3 + GS_HIDDEN;
+    c->mlp.w32 = w;
+    c->mlp.w16_all = h;
+    c->mlp.w16[0] = h + oW1;
+    c->mlp.w16[1] = h + oW2;
+    c->mlp.w16[2] = h + oW3;
+    c->mlp.b[0] = w + ob1;
+    c->mlp.b[1] = w + ob2;
+    c->mlp.b[2] = w + ob3;
+}
+
+// He-uniform initial weights from a fixed-seed generator (splitmix64), zero biases.
+void mlp_init(gs_ctx* c) {
+    DevPool* P = &c->pool;
+    c->d_w32 = DBuf(P, sizeof(float) * GS_MLP_PARAMS);
+    c->d_w16 = DBuf(P, sizeof(uint16_t) * GS_MLP_PARAMS);
+    c->d_adam_m = DBuf(P, sizeof(float) * GS_MLP_PARAMS);
+    c->d_adam_v = DBuf(P, sizeof(float) * GS_MLP_PARAMS);
+    c->d_grads = DBuf(P, sizeof(float) * GS_MLP_PARAMS);
+    mlp_bind(c);
+    std::vector<float> w(GS_MLP_PARAMS, 0.f);
+    unsigned long long st = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() {
+        unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) / 9007199254740992.0;  // [0, 1)
+    };
+    auto fill = [&](long long off, int fan_in, long long count) {
+        const double lim = std::sqrt(6.0 / fan_in);
+        for (long long i = 0; i < count; ++i) w[off + i] = (float)((2.0 * rnd() - 1.0) * lim);
+    };
+    fill(0, GS_FEATURES, GS_FEATURES * GS_HIDDEN);
+    fill(GS_FEATURES * GS_HIDDEN + GS_HIDDEN, GS_HIDDEN, GS_HIDDEN * GS_HIDDEN);
+    fill(GS_FEATURES * GS_HIDDEN + 2 * GS_HIDDEN + GS_HIDDEN * GS_HIDDEN, GS_HIDDEN, GS_HIDDEN);
+    h2d(c, c->d_w32.as<float>(), w.data(), GS_MLP_PARAMS);
+    HIPCHECK(gsk::fill_i32(c->d_adam_m.as<int>(), 0, GS_MLP_PARAMS, c->stream));
+    HIPCHECK(gsk::fill_i32(c->d_adam_v.as<int>(), 0, GS_MLP_PARAMS, c->stream));
+    HIPCHECK(gsk::mlp_sync_bf16(c->mlp, c->stream));
+    c->mlp_step = 0;
+    c->mlp_trained = false;
 }
 
 // Folds the event pairs recorded since the last host synchronisation into the stats.
@@ -509,6 +606,167 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         ++c->ev_used;
     };
 
+    // ---- learned sizes: one fill pass below the last level from predicted child counts
+    if (write_rows && !all_live && c->opts.mode == GS_MODE_PREDICTED && c->mlp_trained &&
+        K < (1 << 21)) {
+        // predicted counts of every candidate child key
+        std::vector<float> y(K);
+        std::vector<unsigned char> poss(K);
+        {
+            DBuf d_X(P, sizeof(float) * K * GS_FEATURES), d_poss(P, K);
+            DBuf d_h1(P, sizeof(float) * K * GS_HIDDEN), d_h2(P, sizeof(float) * K * GS_HIDDEN);
+            DBuf d_y(P, sizeof(float) * K);
+            HIPCHECK(gsk::features(d_pats.as<PatDev>(), B, ks, K, c->d_H.as<float>(),
+                                   c->d_vfreq.as<float>(), c->d_efreq.as<float>(),
+                                   c->d_dmean.as<float>(), (float)c->n, d_X.as<float>(),
+                                   d_poss.as<unsigned char>(), c->stream));
+            HIPCHECK(gsk::mlp_forward(d_X.as<float>(), K, c->mlp, d_h1.as<float>(),
+                                      d_h2.as<float>(), d_y.as<float>(), c->stream));
+            d2h(c, y.data(), d_y.as<float>(), K);
+            d2h(c, poss.data(), d_poss.as<unsigned char>(), K);
+            sync(c);
+        }
+        // capacities: 1.25x the prediction plus 16 rows; keys the rules exclude get none
+        std::vector<int> row_off(K), cap_end(K);
+        std::vector<SetDesc> psets(K);
+        long long rows_total = 0, pool_words = 0;
+        for (long long key = 0; key < K; ++key) {
+            int p = int(std::upper_bound(pd.begin(), pd.end(), (int)key,
+                                         [](int kk, const PatDev& d) { return kk < d.key_base; }) -
+                        pd.begin()) - 1;
+            const int slot = (int)(key - pd[p].key_base);
+            const int nv_c = pd[p].nv + (slot < (pd[p].r - 1) * c->EL ? 0 : 1);
+            long long cap = 0;
+            if (poss[key]) {
+                const double pred = std::max(0.0, std::expm1((double)y[key]));
+                cap = std::min<long long>(1LL << 30, (long long)std::ceil(1.25 * pred) + 16);
+            }
+            row_off[key] = (int)std::min<long long>(rows_total, INT_MAX);
+            rows_total += cap;
+            cap_end[key] = (int)std::min<long long>(rows_total, INT_MAX);
+            SetDesc& d = psets[key];
+            if (cap > 0) {
+                plan_set(c, cap, nv_c, d, pool_words);
+            } else {
+                d = SetDesc{0, GS_SET_HASH, 1, 0, 0};
+            }
+        }
+        const size_t need = (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
+                            (size_t)K * (sizeof(SetDesc) + 12);
+        if (rows_total <= INT_MAX && c->pool.in_use() + need <= c->budget) {
+            DBuf rows(P, sizeof(int2) * std::max<long long>(rows_total, 1));
+            DBuf d_cur(P, sizeof(int) * K), d_end(P, sizeof(int) * K);
+            DBuf d_psets(P, sizeof(SetDesc) * K);
+            DBuf d_ppool(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
+            h2d(c, d_cur.as<int>(), row_off.data(), K);
+            h2d(c, d_end.as<int>(), cap_end.data(), K);
+            h2d(c, d_psets.as<SetDesc>(), psets.data(), K);
+            HIPCHECK(gsk::fill_u32(d_ppool.as<uint32_t>(), 0u, pool_words, c->stream));
+            HIPCHECK(gsk::fill_i32(c->overflow_flag(), 0, 1, c->stream));
+            long long Wp = 0;
+            std::vector<long long> Wc_p(nchunks, 0);
+            for (int ci = 0; ci < nchunks; ++ci) {
+                const long long b0 = (long long)ci * chunk;
+                const long long Rc = prep_chunk(ci);
+                d2h(c, &Wc_p[ci], d_woff.as<long long>() + Rc, 1);
+                time_begin();
+                HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                                         d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
+                                         nullptr, d_cur.as<int>(), d_end.as<int>(),
+                                         rows.as<int2>(), d_psets.as<SetDesc>(),
+                                         d_ppool.as<uint32_t>(), c->overflow_flag(),
+                                         c->tile_counters(), c->grid_items, (int)K, c->stream));
+                time_end(1);
+                c->st.fill_launches++;
+            }
+            DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K),
+                d_flags(P, sizeof(int) * K);
+            HIPCHECK(gsk::fill_i32(c->n_active(), 0, 2, c->stream));
+            HIPCHECK(gsk::compact_active(d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>(),
+                                         c->n_active(), c->stream));
+            HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(),
+                                (int)K, d_flags.as<int>(), c->err_flag(), c->stream));
+            int na_err_ov[3] = {0, 0, 0};
+            d2h(c, na_err_ov, c->n_active(), 3);  // n_active, err, overflow
+            sync(c);
+            resolve_timing(c);
+            for (long long w : Wc_p) Wp += w;
+            c->st.items += Wp;
+            c->st.parent_rows += R;
+            if (na_err_ov[1] & 1)
+                throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
+            if (na_err_ov[1] & 2)
+                throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
+            if (na_err_ov[2] == 0) {
+                const int A = na_err_ov[0];
+                std::vector<int> akeys(A), acnt(A), aflag(A);
+                d2h(c, akeys.data(), d_active.as<int>(), A);
+                d2h(c, acnt.data(), d_acnt.as<int>(), A);
+                d2h(c, aflag.data(), d_flags.as<int>(), A);
+                sync(c);
+                std::vector<int> order(A);
+                for (int i = 0; i < A; ++i) order[i] = i;
+                std::sort(order.begin(), order.end(),
+                          [&](int a, int b) { return akeys[a] < akeys[b]; });
+                std::vector<SetDesc> live_sets;
+                long long emitted = 0;
+                for (int oi = 0; oi < A; ++oi) {
+                    const int a = order[oi];
+                    const int key = akeys[a];
+                    int p = int(std::upper_bound(pd.begin(), pd.end(), key,
+                                                 [](int kk, const PatDev& d) {
+                                                     return kk < d.key_base;
+                                                 }) -
+                                pd.begin()) - 1;
+                    ChildInfo ci;
+                    ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
+                    ci.count = acnt[a];
+                    ci.canonical = aflag[a];
+                    ci.support = 0;
+                    ci.row_off = row_off[key];
+                    emitted += acnt[a];
+                    if (ci.canonical) live_sets.push_back(psets[key]);
+                    out.children.push_back(ci);
+                }
+                out.emitted = emitted;
+                const int NL = (int)live_sets.size();
+                if (NL > 0) {
+                    DBuf d_ls(P, sizeof(SetDesc) * NL), d_counts(P, sizeof(int) * NL * GS_MAXV);
+                    h2d(c, d_ls.as<SetDesc>(), live_sets.data(), NL);
+                    HIPCHECK(gsk::set_counts(d_ls.as<SetDesc>(), NL, d_ppool.as<uint32_t>(),
+                                             d_counts.as<int>(), c->stream));
+                    std::vector<int> counts((size_t)NL * GS_MAXV);
+                    d2h(c, counts.data(), d_counts.as<int>(), counts.size());
+                    sync(c);
+                    int li = 0;
+                    for (auto& ch : out.children) {
+                        if (!ch.canonical) continue;
+                        ch.support = support_of(ch.pat, &counts[(size_t)li * GS_MAXV], c->mni_mode);
+                        ch.pat.support = ch.support;
+                        ch.pat.row_begin = ch.row_off;
+                        ch.pat.row_count = ch.count;
+                        ++li;
+                    }
+                }
+                {
+                    double bytes = 16.0 * (double)Wp + 8.0 * (double)emitted;
+                    for (int i = 0; i < B; ++i)
+                        bytes += (double)pats[i].row_count *
+                                 (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
+                    c->st.alg_bytes_fill += bytes;
+                }
+                out.rows = std::move(rows);
+                c->st.predicted_steps++;
+                c->st.batches++;
+                return STEP_OK;
+            }
+            // a child outgrew its predicted capacity: redo this step exactly
+            c->st.fallbacks++;
+            out.children.clear();
+            HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
+        }
+    }
+
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
     for (int ci = 0; ci < nchunks; ++ci) {
         const long long b0 = (long long)ci * chunk;
@@ -518,9 +776,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         if (single) {
             HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
-                                     nullptr, nullptr, d_all_sets.as<SetDesc>(),
-                                     d_all_pool.as<uint32_t>(), c->tile_counters(),
-                                     c->grid_items, (int)K, c->stream));
+                                     nullptr, nullptr, nullptr, d_all_sets.as<SetDesc>(),
+                                     d_all_pool.as<uint32_t>(), c->overflow_flag(),
+                                     c->tile_counters(), c->grid_items, (int)K, c->stream));
             time_end(1);
             c->st.fill_launches++;
         } else {
@@ -532,6 +790,22 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         }
     }
     c->st.parent_rows += R;
+
+    // training samples for the predictor: features and exact counts of every candidate key
+    if (c->logging && !single && c->n_logged < c->log_cap) {
+        const long long take = std::min(K, c->log_cap - c->n_logged);
+        DBuf d_X(P, sizeof(float) * K * GS_FEATURES);
+        HIPCHECK(gsk::features(d_pats.as<PatDev>(), B, ks, K, c->d_H.as<float>(),
+                               c->d_vfreq.as<float>(), c->d_efreq.as<float>(),
+                               c->d_dmean.as<float>(), (float)c->n, d_X.as<float>(), nullptr,
+                               c->stream));
+        HIPCHECK(hipMemcpyAsync(c->d_logX.as<float>() + c->n_logged * GS_FEATURES,
+                                d_X.as<float>(), sizeof(float) * take * GS_FEATURES,
+                                hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(gsk::log_targets(d_kc.as<int>(), take, c->d_logT.as<float>() + c->n_logged,
+                                  c->stream));
+        c->n_logged += take;
+    }
 
     // ---- candidates and their canonicality
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
@@ -634,10 +908,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             time_begin();
             HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, nullptr,
-                                     d_slot.as<int>(), d_cursor.as<int>(),
+                                     d_slot.as<int>(), d_cursor.as<int>(), nullptr,
                                      write_rows ? out.rows.as<int2>() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
-                                     c->tile_counters(), c->grid_items, NL, c->stream));
+                                     c->overflow_flag(), c->tile_counters(), c->grid_items, NL,
+                                     c->stream));
             time_end(1);
             c->st.fill_launches++;
         }
@@ -997,6 +1272,8 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         h2d(x, x->d_src.as<int>(), src.data(), nnz);
         h2d(x, x->d_adj.as<uint2>(), adj.data(), nnz);
         h2d(x, x->d_vlab.as<int>(), vr.data(), n);
+        init_label_stats(x, vr, src, adj);
+        mlp_init(x);
         sync(x);
         x->dg.n = n;
         x->dg.nnz = nnz;
@@ -1173,6 +1450,127 @@ int gs_is_canonical_batch(gs_ctx* c, const int* codes5, const int* lens, int nco
     });
 }
 
+int gs_predictor_log(gs_ctx* c, int on, long long capacity) {
+    if (!c || capacity < 0) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        c->logging = on != 0;
+        if (on && capacity != c->log_cap) {
+            c->d_logX = DBuf(&c->pool, sizeof(float) * std::max<long long>(capacity, 1) * GS_FEATURES);
+            c->d_logT = DBuf(&c->pool, sizeof(float) * std::max<long long>(capacity, 1));
+            c->log_cap = capacity;
+            c->n_logged = 0;
+        }
+    });
+}
+
+int gs_predictor_samples(const gs_ctx* c, long long* count) {
+    if (!c || !count) return GS_ERR_INVALID;
+    *count = c->n_logged;
+    return GS_OK;
+}
+
+namespace {
+// One forward + backward over n samples already on the device; gradients into d_grads,
+// mean squared error into *loss_dev.
+void mlp_grad_device(gs_ctx* c, const float* X, const float* T, long long n, DBuf& h1, DBuf& h2,
+                     DBuf& y, DBuf& dy, DBuf& dh2, DBuf& dh1, float* loss_dev) {
+    HIPCHECK(gsk::fill_i32(c->d_grads.as<int>(), 0, GS_MLP_PARAMS, c->stream));
+    HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(loss_dev), 0, 1, c->stream));
+    HIPCHECK(gsk::mlp_forward(X, n, c->mlp, h1.as<float>(), h2.as<float>(), y.as<float>(),
+                              c->stream));
+    HIPCHECK(gsk::mlp_backward(X, T, n, c->mlp, h1.as<float>(), h2.as<float>(), y.as<float>(),
+                               dy.as<float>(), dh2.as<float>(), dh1.as<float>(),
+                               c->d_grads.as<float>(), loss_dev, c->stream));
+}
+
+struct MlpScratch {
+    DBuf h1, h2, y, dy, dh2, dh1, loss;
+    MlpScratch(DevPool* P, long long n)
+        : h1(P, sizeof(float) * n * GS_HIDDEN), h2(P, sizeof(float) * n * GS_HIDDEN),
+          y(P, sizeof(float) * n), dy(P, sizeof(float) * n), dh2(P, sizeof(float) * n * GS_HIDDEN),
+          dh1(P, sizeof(float) * n * GS_HIDDEN), loss(P, sizeof(float)) {}
+};
+}  // namespace
+
+int gs_predictor_train(gs_ctx* c, int epochs, float lr, float* loss) {
+    if (!c || epochs < 0 || !(lr > 0)) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        const long long n = c->n_logged;
+        if (n == 0) throw GsError(GS_ERR_INVALID, "no logged samples to train on");
+        MlpScratch sc(&c->pool, n);
+        float l = 0.f;
+        for (int e = 0; e < epochs; ++e) {
+            mlp_grad_device(c, c->d_logX.as<float>(), c->d_logT.as<float>(), n, sc.h1, sc.h2,
+                            sc.y, sc.dy, sc.dh2, sc.dh1, sc.loss.as<float>());
+            HIPCHECK(gsk::mlp_adam(c->mlp, c->d_grads.as<float>(), c->d_adam_m.as<float>(),
+                                   c->d_adam_v.as<float>(), ++c->mlp_step, lr, c->stream));
+        }
+        // loss at the final weights
+        mlp_grad_device(c, c->d_logX.as<float>(), c->d_logT.as<float>(), n, sc.h1, sc.h2, sc.y,
+                        sc.dy, sc.dh2, sc.dh1, sc.loss.as<float>());
+        d2h(c, &l, sc.loss.as<float>(), 1);
+        sync(c);
+        if (loss) *loss = l;
+        c->mlp_trained = true;
+    });
+}
+
+int gs_predictor_get_weights(const gs_ctx* c, float* weights) {
+    if (!c || !weights) return GS_ERR_INVALID;
+    gs_ctx* m = const_cast<gs_ctx*>(c);
+    return guard(m, [&]() {
+        d2h(m, weights, c->d_w32.as<float>(), GS_MLP_PARAMS);
+        sync(m);
+    });
+}
+
+int gs_predictor_set_weights(gs_ctx* c, const float* weights) {
+    if (!c || !weights) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        h2d(c, c->d_w32.as<float>(), weights, GS_MLP_PARAMS);
+        HIPCHECK(gsk::mlp_sync_bf16(c->mlp, c->stream));
+        HIPCHECK(gsk::fill_i32(c->d_adam_m.as<int>(), 0, GS_MLP_PARAMS, c->stream));
+        HIPCHECK(gsk::fill_i32(c->d_adam_v.as<int>(), 0, GS_MLP_PARAMS, c->stream));
+        sync(c);
+        c->mlp_step = 0;
+        c->mlp_trained = true;
+    });
+}
+
+int gs_mlp_forward(gs_ctx* c, const float* x, long long n, float* y) {
+    if (!c || n < 0 || (n > 0 && (!x || !y))) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        if (n == 0) return;
+        DBuf dX(&c->pool, sizeof(float) * n * GS_FEATURES);
+        MlpScratch sc(&c->pool, n);
+        h2d(c, dX.as<float>(), x, (size_t)n * GS_FEATURES);
+        HIPCHECK(gsk::mlp_forward(dX.as<float>(), n, c->mlp, sc.h1.as<float>(), sc.h2.as<float>(),
+                                  sc.y.as<float>(), c->stream));
+        d2h(c, y, sc.y.as<float>(), (size_t)n);
+        sync(c);
+    });
+}
+
+int gs_mlp_gradients(gs_ctx* c, const float* x, const float* t, long long n, float* grads,
+                     float* loss) {
+    if (!c || n <= 0 || !x || !t || !grads) return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        DBuf dX(&c->pool, sizeof(float) * n * GS_FEATURES), dT(&c->pool, sizeof(float) * n);
+        MlpScratch sc(&c->pool, n);
+        h2d(c, dX.as<float>(), x, (size_t)n * GS_FEATURES);
+        h2d(c, dT.as<float>(), t, (size_t)n);
+        mlp_grad_device(c, dX.as<float>(), dT.as<float>(), n, sc.h1, sc.h2, sc.y, sc.dy, sc.dh2,
+                        sc.dh1, sc.loss.as<float>());
+        d2h(c, grads, c->d_grads.as<float>(), GS_MLP_PARAMS);
+        float l = 0.f;
+        d2h(c, &l, sc.loss.as<float>(), 1);
+        sync(c);
+        if (loss) *loss = l;
+    });
+}
+
 int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
     if (!c || !count || !total_edges) return GS_ERR_INVALID;
     long long e = 0;
@@ -1218,6 +1616,9 @@ int gs_destroy(gs_ctx* c) {
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();
+    for (DBuf* b : {&c->d_H, &c->d_vfreq, &c->d_efreq, &c->d_dmean, &c->d_w32, &c->d_w16,
+                    &c->d_adam_m, &c->d_adam_v, &c->d_grads, &c->d_logX, &c->d_logT})
+        b->reset();
     c->pool.release_all();
     for (auto& e : c->ev_pool) {
         (void)hipEventDestroy(e.first);

@@ -22,6 +22,10 @@ def _ip(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
 def _code_arr(code) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(code, dtype=np.int32).reshape(-1, 5))
 
@@ -42,7 +46,8 @@ class Miner:
         self._lib = L.lib()
         opts = L.GsOpts()
         self._lib.gs_default_opts(ctypes.byref(opts))
-        opts.mode = {"plain": L.GS_MODE_PLAIN, "batched": L.GS_MODE_BATCHED}[mode]
+        opts.mode = {"plain": L.GS_MODE_PLAIN, "batched": L.GS_MODE_BATCHED,
+                     "predicted": L.GS_MODE_PREDICTED}[mode]
         opts.mni_mode = {"gpu": L.GS_MNI_GPU, "omp-literal": L.GS_MNI_OMP_LITERAL}[mni]
         opts.device = device
         opts.timing = 1 if timing else 0
@@ -149,6 +154,46 @@ class Miner:
         self._check(self._lib.gs_is_canonical_batch(self._h, _ip(flat), _ip(lens), len(codes),
                                                     _ip(flags)))
         return [bool(x) for x in flags]
+
+    # ---- child-count predictor
+    def predictor_log(self, on: bool = True, capacity: int = 1 << 20):
+        """Log (features, log1p(count)) of every candidate child key of exact count passes."""
+        self._check(self._lib.gs_predictor_log(self._h, 1 if on else 0, capacity))
+
+    def predictor_samples(self) -> int:
+        n = ctypes.c_longlong()
+        self._check(self._lib.gs_predictor_samples(self._h, ctypes.byref(n)))
+        return n.value
+
+    def predictor_train(self, epochs: int = 300, lr: float = 3e-3) -> float:
+        loss = ctypes.c_float()
+        self._check(self._lib.gs_predictor_train(self._h, epochs, lr, ctypes.byref(loss)))
+        return loss.value
+
+    def predictor_weights(self) -> np.ndarray:
+        w = np.zeros(L.GS_MLP_PARAMS, np.float32)
+        self._check(self._lib.gs_predictor_get_weights(self._h, _fp(w)))
+        return w
+
+    def set_predictor_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.shape == (L.GS_MLP_PARAMS,)
+        self._check(self._lib.gs_predictor_set_weights(self._h, _fp(w)))
+
+    def mlp_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, L.GS_MLP_FEATURES)
+        y = np.zeros(x.shape[0], np.float32)
+        self._check(self._lib.gs_mlp_forward(self._h, _fp(x), x.shape[0], _fp(y)))
+        return y
+
+    def mlp_gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, L.GS_MLP_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(L.GS_MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self._check(self._lib.gs_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0], _fp(g),
+                                               ctypes.byref(loss)))
+        return g, loss.value
 
     def stats(self) -> dict:
         s = L.GsStats()

@@ -40,6 +40,9 @@ extern "C" {
 
 #define GS_MODE_PLAIN 0   /* reference DFS order, one parent pattern per launch set   */
 #define GS_MODE_BATCHED 1 /* level-synchronous: every pattern of a DFS level at once   */
+#define GS_MODE_PREDICTED 2 /* batched; below the last level child buffers are sized by
+                               the learned predictor and filled in one pass (exact two-pass
+                               fallback on any misprediction)                          */
 
 #define GS_MNI_GPU 0         /* support_gpu semantics (gspan.cu:165-190), the shipped fsm */
 #define GS_MNI_OMP_LITERAL 1 /* support_omp semantics (gspan.cu:321-374)                  */
@@ -76,6 +79,8 @@ typedef struct gs_stats {
     double t_count_ms;      /* summed count-kernel time (timing = 1)                     */
     double t_mine_ms;       /* wall time of the last gs_mine                             */
     unsigned long long peak_device_bytes;
+    long long predicted_steps; /* steps run in one pass from predicted sizes               */
+    long long fallbacks;       /* predicted steps redone exactly after a misprediction     */
 } gs_stats;
 
 /* Receives one frequent pattern.  Return 0 to continue, non-zero to stop mining. */
@@ -112,6 +117,26 @@ int gs_extend(gs_ctx* ctx, const int* code5, int m, const int* const* layers,
 /* Canonicality of ncodes codes; code i has lens[i] edges and starts at 5*sum(lens[:i]). */
 int gs_is_canonical_batch(gs_ctx* ctx, const int* codes5, const int* lens, int ncodes,
                           int* flags);
+
+/* Child-count predictor (MLP 32 -> 128 -> 128 -> 1, ReLU, bf16 weights, fp32 accumulation;
+ * no counterpart in the reference, whose count pass is gspan.cu:737-755).
+ * gs_predictor_log: while on, every exact count pass appends (features, log1p(count)) of its
+ * candidate child keys to a device buffer of at most `capacity` samples.
+ * gs_predictor_train: full-batch Adam on the logged samples with the library's own forward
+ * and backward kernels; *loss = final mean squared error (log space).
+ * Weights are GS_MLP_PARAMS floats: W1[32][128], b1[128], W2[128][128], b2[128], W3[128], b3. */
+#define GS_MLP_FEATURES 32
+#define GS_MLP_PARAMS 20865
+int gs_predictor_log(gs_ctx* ctx, int on, long long capacity);
+int gs_predictor_samples(const gs_ctx* ctx, long long* count);
+int gs_predictor_train(gs_ctx* ctx, int epochs, float lr, float* loss);
+int gs_predictor_get_weights(const gs_ctx* ctx, float* weights);
+int gs_predictor_set_weights(gs_ctx* ctx, const float* weights);
+/* The network on caller data: y[n] for x[n][32]; gradients (GS_MLP_PARAMS floats) of the mean
+ * squared error against t[n] at the current weights, without updating them. */
+int gs_mlp_forward(gs_ctx* ctx, const float* x, long long n, float* y);
+int gs_mlp_gradients(gs_ctx* ctx, const float* x, const float* t, long long n, float* grads,
+                     float* loss);
 
 /* Results of the last gs_mine, in report order, as flat arrays (for callers that want
  * them in bulk instead of through a callback; gs_mine may then be called with cb = NULL).
