@@ -96,6 +96,7 @@ def main() -> int:
     ap.add_argument("--k", type=int, default=0)
     ap.add_argument("--min-sup", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-predicted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
@@ -164,6 +165,38 @@ def main() -> int:
                  "batches": pst["batches"]}
         log(f"plain: {pdt:.3f}s")
 
+    predicted = None
+    if not args.no_predicted:
+        # learned sizes: log exact counts of one two-pass run, train the predictor on them
+        # with the library's own kernels, then mine with predicted sizes (exact fallback)
+        with Miner(g, mode="batched", two_pass=True) as ml:
+            ml.predictor_log(True, 1 << 20)
+            ml.mine_raw(k, ms)
+            samples = ml.predictor_samples()
+            t = time.perf_counter()
+            loss = ml.predictor_train(epochs=300, lr=3e-3)
+            train_s = time.perf_counter() - t
+            w = ml.predictor_weights()
+        with Miner(g, mode="predicted", timing=True) as mq:
+            mq.set_predictor_weights(w)
+            for _ in range(args.warmup):
+                mq.mine_raw(k, ms)
+            mq.reset_stats()
+            qt = []
+            for _ in range(args.steps):
+                t = time.perf_counter()
+                qres = mq.mine_raw(k, ms)
+                qt.append(time.perf_counter() - t)
+            qst = mq.stats()
+        if Miner.to_patterns(*qres) != pats:
+            log("ERROR: predicted and batched results differ")
+            return 1
+        predicted = {"value": qst["extended"] / sum(qt), "ms_per_step": sum(qt) / len(qt) * 1e3,
+                     "predicted_steps_per_step": qst["predicted_steps"] / args.steps,
+                     "fallbacks_per_step": qst["fallbacks"] / args.steps,
+                     "train_samples": samples, "train_loss": loss, "train_s": train_s}
+        log(f"predicted: {predicted['ms_per_step']:.2f} ms/step")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -198,6 +231,8 @@ def main() -> int:
         "mining_wall_s": total_s / steps,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "predicted": predicted,
+        "predicted_vs_plain": (predicted["value"] / plain["value"]) if (plain and predicted) else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "peak_device_bytes": st["peak_device_bytes"],
