@@ -176,9 +176,9 @@ def main() -> int:
             t = time.perf_counter()
             loss = ml.predictor_train(epochs=300, lr=3e-3)
             train_s = time.perf_counter() - t
-            w = ml.predictor_weights()
+            weights = ml.predictor_weights()
         with Miner(g, mode="predicted", timing=True) as mq:
-            mq.set_predictor_weights(w)
+            mq.set_predictor_weights(weights)
             for _ in range(args.warmup):
                 mq.mine_raw(k, ms)
             mq.reset_stats()

@@ -120,6 +120,7 @@ def lib():
     L.gs_predictor_set_weights.argtypes = [vp, P(c_f)]
     L.gs_mlp_forward.argtypes = [vp, P(c_f), c_ll, P(c_f)]
     L.gs_mlp_gradients.argtypes = [vp, P(c_f), P(c_f), c_ll, P(c_f), P(c_f)]
+    L.gs_prune_graph.argtypes = [vp, c_int, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]
     L.gs_result_count.argtypes = [vp, P(c_ll), P(c_ll)]
     L.gs_result_copy.argtypes = [vp, P(c_int), P(c_int), P(c_int)]
     L.gs_get_stats.argtypes = [vp, P(GsStats)]
@@ -131,7 +132,7 @@ def lib():
                  "gs_is_canonical_batch", "gs_result_count", "gs_result_copy", "gs_get_stats",
                  "gs_reset_stats", "gs_destroy", "gs_predictor_log", "gs_predictor_samples",
                  "gs_predictor_train", "gs_predictor_get_weights", "gs_predictor_set_weights",
-                 "gs_mlp_forward", "gs_mlp_gradients"):
+                 "gs_mlp_forward", "gs_mlp_gradients", "gs_prune_graph"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

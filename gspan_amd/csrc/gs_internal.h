@@ -140,6 +140,12 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool, int*
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
                         int mni_mode, const SetDesc* set, uint32_t* pool, hipStream_t s);
 
+// graph.cc:216-266 on the device graph; newid/newoff are exclusive scans (n+1 entries)
+hipError_t prune_graph(const DevGraph& g, int VL, int min_sup, const int* vl_vals,
+                       const int* el_vals, int* hist, int* keep, int* kdeg, long long* newid,
+                       long long* newoff, long long* scan_tmp, long long scan_tmp_elems,
+                       int* out_offsets, int* out_vlabels, int* out_edges, hipStream_t s);
+
 // predictor
 hipError_t features(const PatDev* pats, int npats, KeySpace ks, long long K, const float* H,
                     const float* vfreq, const float* efreq, const float* dmean, float nverts,

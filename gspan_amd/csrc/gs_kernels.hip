@@ -1111,3 +1111,77 @@ hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, Layer
 }
 
 }  // namespace gsk
+
+// ------------------------------------------------------------------ prune_graph
+// graph.cc:216-266: keep the vertices whose label occurs at least min_sup times, renumber
+// them in ascending id order, keep the edges between kept vertices in their CSR order.
+namespace {
+
+__global__ void k_label_hist(const int* __restrict__ vlab, int n, int* hist) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) atomicAdd(&hist[vlab[v]], 1);
+}
+
+__global__ void k_keep(const int* __restrict__ vlab, int n, const int* __restrict__ hist,
+                       int min_sup, int* __restrict__ keep) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) keep[v] = hist[vlab[v]] >= min_sup ? 1 : 0;
+}
+
+// kept-edge count of every vertex (0 for dropped vertices)
+__global__ void k_kept_degree(DevGraph g, const int* __restrict__ keep, int* __restrict__ kdeg) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.n) return;
+    int c = 0;
+    if (keep[v])
+        for (int i = g.offsets[v]; i < g.offsets[v + 1]; ++i) c += keep[g.adj[i].x];
+    kdeg[v] = c;
+}
+
+__global__ void k_prune_scatter(DevGraph g, const int* __restrict__ keep,
+                                const long long* __restrict__ newid,
+                                const long long* __restrict__ newoff,
+                                const int* __restrict__ vl_vals, const int* __restrict__ el_vals,
+                                int* __restrict__ out_offsets, int* __restrict__ out_vlabels,
+                                int* __restrict__ out_edges) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.n || !keep[v]) return;
+    const int nv = (int)newid[v];
+    out_vlabels[nv] = vl_vals[g.vlab[v]];
+    out_offsets[nv + 1] = (int)newoff[v + 1];
+    long long at = newoff[v];
+    for (int i = g.offsets[v]; i < g.offsets[v + 1]; ++i) {
+        const uint2 a = g.adj[i];
+        if (!keep[a.x]) continue;
+        out_edges[3 * at + 0] = nv;
+        out_edges[3 * at + 1] = (int)newid[a.x];
+        out_edges[3 * at + 2] = el_vals[a.y >> 16];
+        ++at;
+    }
+}
+
+}  // namespace
+
+namespace gsk {
+
+hipError_t prune_graph(const DevGraph& g, int VL, int min_sup, const int* vl_vals,
+                       const int* el_vals, int* hist, int* keep, int* kdeg, long long* newid,
+                       long long* newoff, long long* scan_tmp, long long scan_tmp_elems,
+                       int* out_offsets, int* out_vlabels, int* out_edges, hipStream_t s) {
+    const unsigned nb = (unsigned)((g.n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, hist, 0, (long long)VL);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out_offsets, 0, 1LL);
+    if (g.n == 0) return hipGetLastError();
+    hipLaunchKernelGGL(k_label_hist, dim3(nb), dim3(256), 0, s, g.vlab, g.n, hist);
+    hipLaunchKernelGGL(k_keep, dim3(nb), dim3(256), 0, s, g.vlab, g.n, hist, min_sup, keep);
+    hipError_t e = scan_exclusive(keep, newid, g.n, scan_tmp, scan_tmp_elems, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_kept_degree, dim3(nb), dim3(256), 0, s, g, keep, kdeg);
+    e = scan_exclusive(kdeg, newoff, g.n, scan_tmp, scan_tmp_elems, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_prune_scatter, dim3(nb), dim3(256), 0, s, g, keep, newid, newoff, vl_vals,
+                       el_vals, out_offsets, out_vlabels, out_edges);
+    return hipGetLastError();
+}
+
+}  // namespace gsk

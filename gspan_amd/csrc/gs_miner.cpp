@@ -1571,6 +1571,45 @@ int gs_mlp_gradients(gs_ctx* c, const float* x, const float* t, long long n, flo
     });
 }
 
+int gs_prune_graph(gs_ctx* c, int min_sup, int* n_out, int* nnz_out, int* offsets_out,
+                   int* vlabels_out, int* edges3_out) {
+    if (!c || !n_out || !nnz_out || !offsets_out || (c->n > 0 && !vlabels_out) ||
+        (c->nnz > 0 && !edges3_out))
+        return GS_ERR_INVALID;
+    return guard(c, [&]() {
+        HIPCHECK(hipSetDevice(c->device));
+        DevPool* P = &c->pool;
+        const long long n = c->n;
+        DBuf d_vl(P, sizeof(int) * c->VL), d_el(P, sizeof(int) * c->EL);
+        DBuf d_hist(P, sizeof(int) * c->VL), d_keep(P, sizeof(int) * std::max(n, 1LL)),
+            d_kdeg(P, sizeof(int) * std::max(n, 1LL));
+        DBuf d_newid(P, sizeof(long long) * (n + 1)), d_newoff(P, sizeof(long long) * (n + 1));
+        const long long te = gsk::scan_tmp_elems(n);
+        DBuf d_tmp(P, sizeof(long long) * te);
+        DBuf d_off(P, sizeof(int) * (n + 1)), d_vlab(P, sizeof(int) * std::max(n, 1LL)),
+            d_edges(P, sizeof(int) * 3 * std::max(c->nnz, 1));
+        h2d(c, d_vl.as<int>(), c->vl_vals.data(), c->VL);
+        h2d(c, d_el.as<int>(), c->el_vals.data(), c->EL);
+        HIPCHECK(gsk::prune_graph(c->dg, c->VL, min_sup, d_vl.as<int>(), d_el.as<int>(),
+                                  d_hist.as<int>(), d_keep.as<int>(), d_kdeg.as<int>(),
+                                  d_newid.as<long long>(), d_newoff.as<long long>(),
+                                  d_tmp.as<long long>(), te, d_off.as<int>(), d_vlab.as<int>(),
+                                  d_edges.as<int>(), c->stream));
+        long long sizes[2] = {0, 0};  // kept vertices, kept edges
+        if (n > 0) {
+            d2h(c, &sizes[0], d_newid.as<long long>() + n, 1);
+            d2h(c, &sizes[1], d_newoff.as<long long>() + n, 1);
+        }
+        sync(c);
+        *n_out = (int)sizes[0];
+        *nnz_out = (int)sizes[1];
+        d2h(c, offsets_out, d_off.as<int>(), (size_t)sizes[0] + 1);
+        d2h(c, vlabels_out, d_vlab.as<int>(), (size_t)sizes[0]);
+        d2h(c, edges3_out, d_edges.as<int>(), (size_t)sizes[1] * 3);
+        sync(c);
+    });
+}
+
 int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
     if (!c || !count || !total_edges) return GS_ERR_INVALID;
     long long e = 0;

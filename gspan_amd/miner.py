@@ -155,6 +155,17 @@ class Miner:
                                                     _ip(flags)))
         return [bool(x) for x in flags]
 
+    def prune_graph(self, min_sup: int) -> CSRGraph:
+        """prune_graph (graph.cc:216-266) of this context's graph, computed on the GPU."""
+        g = self.graph
+        off = np.zeros(g.nr_nodes + 1, np.int32)
+        vl = np.zeros(max(g.nr_nodes, 1), np.int32)
+        ed = np.zeros((max(g.nr_edges, 1), 3), np.int32)
+        n, nnz = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.gs_prune_graph(self._h, min_sup, ctypes.byref(n), ctypes.byref(nnz),
+                                             _ip(off), _ip(vl), _ip(ed)))
+        return CSRGraph(off[:n.value + 1].copy(), vl[:n.value].copy(), ed[:nnz.value].copy())
+
     # ---- child-count predictor
     def predictor_log(self, on: bool = True, capacity: int = 1 << 20):
         """Log (features, log1p(count)) of every candidate child key of exact count passes."""

@@ -138,6 +138,13 @@ int gs_mlp_forward(gs_ctx* ctx, const float* x, long long n, float* y);
 int gs_mlp_gradients(gs_ctx* ctx, const float* x, const float* t, long long n, float* grads,
                      float* loss);
 
+/* prune_graph (graph.cc:216-266) of the context's graph on the device: vertices whose label
+ * occurs fewer than min_sup times are dropped, the rest renumbered in ascending id order,
+ * edges between kept vertices kept in CSR order.  Outputs must hold n+1, n and 3*nnz ints
+ * (the input sizes); *n_out / *nnz_out receive the pruned sizes. */
+int gs_prune_graph(gs_ctx* ctx, int min_sup, int* n_out, int* nnz_out, int* offsets_out,
+                   int* vlabels_out, int* edges3_out);
+
 /* Results of the last gs_mine, in report order, as flat arrays (for callers that want
  * them in bulk instead of through a callback; gs_mine may then be called with cb = NULL).
  * gs_result_count: number of patterns and total code edges.  gs_result_copy fills

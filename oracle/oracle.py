@@ -166,6 +166,37 @@ def pattern_graph(code):
     return off, vl, ed
 
 
+def prune_graph(g):
+    """Restatement of prune_graph (graph.cc:216-266): vertices whose label occurs fewer than
+    min_sup times are dropped, the others renumbered in ascending id order; each kept
+    vertex keeps its edges to kept vertices in CSR order.  Returns a function of min_sup."""
+    def run(min_sup: int):
+        from gspan_amd.graph import CSRGraph
+        counts = np.bincount(g.vlabels - g.vlabels.min()) if g.nr_nodes else np.zeros(0, np.int64)
+        keep = counts[g.vlabels - g.vlabels.min()] >= min_sup if g.nr_nodes else np.zeros(0, bool)
+        newid = np.cumsum(keep) - 1
+        e = g.edges
+        own = np.repeat(np.arange(g.nr_nodes), np.diff(g.offsets))
+        ke = keep[own] & keep[e[:, 1]] if e.shape[0] else np.zeros(0, bool)
+        out = np.stack([newid[e[ke, 0]], newid[e[ke, 1]], e[ke, 2]], axis=1).astype(np.int32)
+        kv = np.nonzero(keep)[0]
+        deg = np.bincount(newid[own[ke]], minlength=kv.shape[0]) if kv.shape[0] else np.zeros(0)
+        off = np.zeros(kv.shape[0] + 1, np.int64)
+        np.cumsum(deg, out=off[1:])
+        return CSRGraph(off.astype(np.int32), g.vlabels[kv], out.reshape(-1, 3))
+    return run
+
+
+def ref_prune(g, min_sup: int, tmpdir: str):
+    """The reference's own prune_graph (oracle/_ref) on g; returns the pruned CSRGraph."""
+    from gspan_amd.graph import CSRGraph
+    src = os.path.join(tmpdir, "in.bin")
+    dst = os.path.join(tmpdir, "out.bin")
+    g.save(src)
+    ref_lib().ref_prune_file(src.encode(), min_sup, dst.encode())
+    return CSRGraph.load(dst)
+
+
 def format_report(patterns) -> str:
     """The reference's stdout lines (gspan.cu:67-77)."""
     lines = []

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import brute
-from gspan_amd.graph import synthetic, csr_from_undirected
+from gspan_amd.graph import CSRGraph, synthetic, csr_from_undirected
 from gspan_amd.miner import Miner, format_report
 
 pytestmark = pytest.mark.gpu
@@ -146,3 +146,38 @@ def test_chunked_and_split_runs_agree(oracle_mod):
             assert mn.mine(3, 20) == want
             s = mn.stats()
             assert s["extended"] == st["extended"] and s["count_launches"] > 0
+
+
+def test_prune_graph_matches_oracle(oracle_mod):
+    """gs_prune_graph (graph.cc:216-266 on the device) against the oracle's restatement,
+    at every threshold that changes the kept set; then mining the pruned graph must give
+    the unpruned graph's report (MNI support is bounded by each pattern label's count)."""
+    cases = [("empty", CSRGraph(np.zeros(1, np.int32), np.zeros(0, np.int32),
+                                np.zeros((0, 3), np.int32)))]
+    for s in range(4):
+        cases.append((f"er{s}", synthetic("er", n=200 + 50 * s, m=600 + 100 * s,
+                                          n_vlabels=3 + s, n_elabels=1 + s % 2, seed=20 + s,
+                                          vlabel_zipf=1.2 if s % 2 else None)))
+    cases.append(("rmat12", synthetic("rmat", scale=12, edge_factor=8, n_vlabels=20,
+                                      vlabel_zipf=1.1, n_elabels=2, seed=3)))
+    for name, g in cases:
+        _announce(f"prune_graph {name} n={g.nr_nodes} nnz={g.nr_edges}")
+        run = oracle_mod.prune_graph(g)
+        sups = sorted({0, 1, g.nr_nodes + 1} | set(np.bincount(g.vlabels).tolist()))
+        with Miner(g) as mn:
+            for ms in sups:
+                got, want = mn.prune_graph(ms), run(ms)
+                assert got.nr_nodes == want.nr_nodes and got.nr_edges == want.nr_edges, (name, ms)
+                np.testing.assert_array_equal(got.offsets, want.offsets)
+                np.testing.assert_array_equal(got.vlabels, want.vlabels)
+                np.testing.assert_array_equal(got.edges, want.edges)
+    g = cases[-2][1]
+    counts = np.bincount(g.vlabels)
+    ms = int(np.sort(counts[counts > 0])[1]) + 1  # drops at least the two rarest labels
+    _announce(f"mine pruned vs unpruned ms={ms}")
+    with Miner(g) as mn:
+        pruned = mn.prune_graph(ms)
+        want = mn.mine(3, ms)
+    assert pruned.nr_nodes < g.nr_nodes
+    with Miner(pruned) as mp:
+        assert mp.mine(3, ms) == want
