@@ -99,7 +99,20 @@ def main() -> int:
     ap.add_argument("--no-predicted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (a plain copy kernel of "
+                         "known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": n, "dispatches": 5}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has a single MI355X, so "
               f"--gpus {args.gpus} is not run (the path would shard by DFS subtree).",

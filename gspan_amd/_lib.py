@@ -29,6 +29,9 @@ GS_MLP_FEATURES = 32
 GS_MLP_PARAMS = 20865
 GS_MNI_GPU = 0
 GS_MNI_OMP_LITERAL = 1
+GS_SETS_AUTO = 0
+GS_SETS_BITS = 1
+GS_SETS_BYTES = 2
 GS_MAX_PATTERN_EDGES = 7
 
 
@@ -41,7 +44,7 @@ class GsOpts(ctypes.Structure):
         ("budget_bytes", ctypes.c_ulonglong),
         ("chunk_rows", ctypes.c_longlong),
         ("two_pass", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("set_kind", ctypes.c_int),
     ]
 
 

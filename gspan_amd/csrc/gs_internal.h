@@ -26,9 +26,12 @@
 #define GS_HIDDEN 128
 #define GS_MLP_PARAMS (GS_FEATURES * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN + 1)
 
-// Set kinds for MNI image sets.
+// Set kinds for MNI image sets: one bit per graph vertex (inserted with an atomic OR),
+// an open-addressing table (compare-and-swap), or one byte per graph vertex (a plain store
+// of 1: every writer stores the same value, so racing stores need no read-modify-write).
 #define GS_SET_BITMAP 0
 #define GS_SET_HASH 1
+#define GS_SET_BYTEMAP 2
 
 // One pattern of a batch, as the extension kernels see it.  Labels are dense ranks
 // (order-preserving), so every comparison of the reference's rules is unchanged.
@@ -65,8 +68,8 @@ struct LayerPtrs {
 // MNI set descriptor of one child key that survived canonicality.
 struct SetDesc {
     long long off;  // first u32 word in the set pool
-    int kind;       // GS_SET_BITMAP / GS_SET_HASH
-    int size;       // words per vertex set (bitmap) or slots (hash, power of two)
+    int kind;       // GS_SET_BITMAP / GS_SET_HASH / GS_SET_BYTEMAP
+    int size;       // words per vertex set (bitmap, bytemap) or slots (hash, power of two)
     int nv;         // vertices of the child pattern
     int pad;
 };

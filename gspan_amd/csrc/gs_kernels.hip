@@ -38,7 +38,9 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
 // counts can, and then raises *overflow after probing every slot once.
 __device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int v, uint32_t x,
                                            int* overflow = nullptr) {
-    if (d.kind == GS_SET_BITMAP) {
+    if (d.kind == GS_SET_BYTEMAP) {
+        reinterpret_cast<unsigned char*>(pool + d.off + (long long)v * d.size)[x] = 1;
+    } else if (d.kind == GS_SET_BITMAP) {
         uint32_t* w = pool + d.off + (long long)v * d.size + (x >> 5);
         uint32_t bit = 1u << (x & 31);
         if (!(*w & bit)) atomicOr(w, bit);
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets,
     int c = 0;
     if (v < d.nv) {
         const uint32_t* base = pool + d.off + (long long)v * d.size;
-        if (d.kind == GS_SET_BITMAP) {
+        if (d.kind != GS_SET_HASH) {  // bitmap bits, or bytemap bytes that are 0 or 1
             for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += __popc(base[i]);
         } else {
             for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += base[i] != 0;

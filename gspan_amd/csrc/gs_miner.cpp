@@ -367,18 +367,32 @@ int support_of(const HPat& h, const int* counts, int mni_mode) {
 
 // Set memory per key: a bitmap over all graph vertices, or a power-of-two open-addressing
 // table of at least twice the key's rows when that is smaller.
-void plan_set(const gs_ctx* c, long long rows, int nv, SetDesc& d, long long& pool_words) {
-    const long long bitmap_words = (((long long)c->n + 31) / 32 + 3) / 4 * 4;
+// u32 words of one vertex set over all graph vertices: a bit or a byte per vertex, padded
+// to 16 bytes.
+long long bitmap_words(const gs_ctx* c) { return (((long long)c->n + 31) / 32 + 3) / 4 * 4; }
+long long bytemap_words(const gs_ctx* c) { return ((long long)c->n + 15) / 16 * 4; }
+
+// Byte-per-vertex sets unless the caller asked for bits.
+bool want_bytes(const gs_ctx* c) { return c->opts.set_kind != GS_SETS_BITS; }
+
+// Sets of a child with `rows` rows: a hash table of >= 2*rows slots when that is smaller
+// than a bitmap, else a bytemap (bytes) or a bitmap.
+void plan_set(const gs_ctx* c, long long rows, int nv, bool bytes, SetDesc& d,
+              long long& pool_words) {
+    const long long bw = bitmap_words(c);
     long long cap = 16;
     const long long want = std::min<long long>(2 * rows, 2LL * c->n);
     while (cap < want) cap <<= 1;
     d.nv = nv;
-    if (cap < bitmap_words) {
+    if (cap < bw) {
         d.kind = GS_SET_HASH;
         d.size = (int)cap;
+    } else if (bytes) {
+        d.kind = GS_SET_BYTEMAP;
+        d.size = (int)bytemap_words(c);
     } else {
         d.kind = GS_SET_BITMAP;
-        d.size = (int)bitmap_words;
+        d.size = (int)bw;
     }
     d.off = pool_words;
     pool_words += (long long)nv * d.size;
@@ -421,6 +435,7 @@ void init_label_stats(gs_ctx* c, const std::vector<int>& vr, const std::vector<i
     h2d(c, c->d_vfreq.as<float>(), vf.data(), VL);
     h2d(c, c->d_efreq.as<float>(), ef.data(), EL);
     h2d(c, c->d_dmean.as<float>(), dm.data(), VL);
+    sync(c);  // the copies read H, vf, ef and dm, which end with this function
 }
 
 void mlp_bind(gs_ctx* c) {
@@ -467,6 +482,7 @@ void mlp_init(gs_ctx* c) {
     HIPCHECK(gsk::fill_i32(c->d_adam_m.as<int>(), 0, GS_MLP_PARAMS, c->stream));
     HIPCHECK(gsk::fill_i32(c->d_adam_v.as<int>(), 0, GS_MLP_PARAMS, c->stream));
     HIPCHECK(gsk::mlp_sync_bf16(c->mlp, c->stream));
+    sync(c);  // the upload reads w, which ends with this function
     c->mlp_step = 0;
     c->mlp_trained = false;
 }
@@ -542,19 +558,27 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     // Single pass at the last level: when a bitmap for every potential child key fits in a
     // quarter of the free budget, the fill pass counts children and fills the MNI sets in
     // one go (no count pass); canonicality is then checked on the keys that occurred.
-    const long long bitmap_words = (((long long)c->n + 31) / 32 + 3) / 4 * 4;
     std::vector<SetDesc> all_sets;
     long long all_words = 0;
     bool single = false;
+    int single_kind = GS_SET_BITMAP;
+    long long set_words = bitmap_words(c);
     if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
-        long long words = 0;
+        long long sets_per_batch = 0;  // vertex sets over every potential child key
         for (int i = 0; i < B; ++i) {
             const long long nb = (long long)(pd[i].r - 1) * c->EL;
             const long long nf = (long long)(pd[i].r + 1) * c->EL * c->VL;
-            words += (nb * pd[i].nv + nf * (pd[i].nv + 1)) * bitmap_words;
+            sets_per_batch += nb * pd[i].nv + nf * (pd[i].nv + 1);
         }
         const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
-        single = (size_t)words * 4 <= avail / 4;
+        if (want_bytes(c) &&
+            (size_t)(sets_per_batch * bytemap_words(c)) * 4 <= avail / 4) {
+            single = true;
+            single_kind = GS_SET_BYTEMAP;
+            set_words = bytemap_words(c);
+        } else if (c->opts.set_kind != GS_SETS_BYTES) {
+            single = (size_t)(sets_per_batch * bitmap_words(c)) * 4 <= avail / 4;
+        }
     }
     DBuf d_all_sets, d_all_pool;
     if (single) {
@@ -564,12 +588,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             const int nb = (pd[i].r - 1) * c->EL;
             for (int sl = 0; sl < K_p; ++sl) {
                 SetDesc& d = all_sets[pd[i].key_base + sl];
-                d.kind = GS_SET_BITMAP;
-                d.size = (int)bitmap_words;
+                d.kind = single_kind;
+                d.size = (int)set_words;
                 d.nv = pd[i].nv + (sl < nb ? 0 : 1);
                 d.off = all_words;
                 d.pad = 0;
-                all_words += (long long)d.nv * bitmap_words;
+                all_words += (long long)d.nv * set_words;
             }
         }
         d_all_sets = DBuf(P, sizeof(SetDesc) * K);
@@ -646,7 +670,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             cap_end[key] = (int)std::min<long long>(rows_total, INT_MAX);
             SetDesc& d = psets[key];
             if (cap > 0) {
-                plan_set(c, cap, nv_c, d, pool_words);
+                plan_set(c, cap, nv_c, want_bytes(c), d, pool_words);
             } else {
                 d = SetDesc{0, GS_SET_HASH, 1, 0, 0};
             }
@@ -835,7 +859,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     std::sort(order.begin(), order.end(), [&](int a, int b) { return akeys[a] < akeys[b]; });
 
     // ---- plan child rows and sets of the live keys
-    std::vector<int> live_keys;
+    std::vector<int> live_keys, live_child;
     std::vector<int> cursor;
     std::vector<SetDesc> sets;
     out.children.reserve(A);
@@ -854,10 +878,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         ci.row_off = -1;
         emitted += acnt[a];
         if (ci.canonical || all_live) {
-            SetDesc d;
-            if (single) d = all_sets[key];
-            else plan_set(c, ci.count, ci.pat.nv(), d, pool_words);
-            sets.push_back(d);
+            live_child.push_back((int)out.children.size());
             live_keys.push_back(key);
             if (write_rows) {
                 ci.row_off = rows_total;
@@ -875,8 +896,26 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         if (B > 1) return STEP_SPLIT;
         throw GsError(GS_ERR_BUDGET, "child rows of one pattern exceed 2^31");
     }
-    const size_t need = (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
-                        (size_t)NL * (sizeof(SetDesc) + 40) + (size_t)K * 4;
+    auto plan_sets = [&](bool bytes) {
+        sets.clear();
+        pool_words = 0;
+        for (int li = 0; li < NL; ++li) {
+            SetDesc d;
+            const ChildInfo& ch = out.children[live_child[li]];
+            if (single) d = all_sets[live_keys[li]];
+            else plan_set(c, ch.count, ch.pat.nv(), bytes, d, pool_words);
+            sets.push_back(d);
+        }
+    };
+    auto need_of = [&]() {
+        return (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
+               (size_t)NL * (sizeof(SetDesc) + 40) + (size_t)K * 4;
+    };
+    plan_sets(want_bytes(c));
+    if (!single && want_bytes(c) && c->opts.set_kind == GS_SETS_AUTO &&
+        c->pool.in_use() + need_of() > c->budget)
+        plan_sets(false);  // bytemaps do not fit: bitmaps take an eighth of the memory
+    const size_t need = need_of();
     if (c->pool.in_use() + need > c->budget) {
         if (B > 1) return STEP_SPLIT;
         throw GsError(GS_ERR_BUDGET, "children of one pattern exceed the device budget (" +
@@ -1044,7 +1083,7 @@ void mine_seeds(gs_ctx* c) {
         h.row_begin = rows_total;
         h.row_count = acnt[a];
         SetDesc d;
-        plan_set(c, acnt[a], 2, d, pool_words);
+        plan_set(c, acnt[a], 2, want_bytes(c), d, pool_words);
         sets.push_back(d);
         live.push_back(key);
         cursor.push_back((int)rows_total);
@@ -1052,6 +1091,12 @@ void mine_seeds(gs_ctx* c) {
         seeds.push_back(h);
     }
     if (rows_total > INT_MAX) throw GsError(GS_ERR_BUDGET, "single-edge rows exceed 2^31");
+    if (want_bytes(c) && c->opts.set_kind == GS_SETS_AUTO &&
+        c->pool.in_use() + (size_t)pool_words * 4 + (size_t)rows_total * 8 > c->budget) {
+        pool_words = 0;  // bytemaps do not fit: plan bitmaps
+        for (size_t i = 0; i < sets.size(); ++i)
+            plan_set(c, seeds[i].row_count, 2, false, sets[i], pool_words);
+    }
     const int NL = (int)live.size();
     c->st.candidates += NL;
     c->st.canonical += NL;
@@ -1333,7 +1378,7 @@ int gs_support(gs_ctx* c, const int* code5, int m, const int* const* layers,
         h2d(c, d_pat.as<PatDev>(), &d, 1);
         long long words = 0;
         SetDesc sd;
-        plan_set(c, counts[m - 1], h.nv(), sd, words);
+        plan_set(c, counts[m - 1], h.nv(), want_bytes(c), sd, words);
         DBuf d_set(&c->pool, sizeof(SetDesc)), d_pool(&c->pool, 4 * std::max(words, 4LL));
         DBuf d_counts(&c->pool, sizeof(int) * GS_MAXV);
         h2d(c, d_set.as<SetDesc>(), &sd, 1);

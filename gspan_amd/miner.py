@@ -42,7 +42,7 @@ class Miner:
 
     def __init__(self, g: CSRGraph, mode: str = "batched", mni: str = "gpu", device: int = 0,
                  timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0,
-                 two_pass: bool = False):
+                 two_pass: bool = False, sets: str = "auto"):
         self._lib = L.lib()
         opts = L.GsOpts()
         self._lib.gs_default_opts(ctypes.byref(opts))
@@ -54,6 +54,8 @@ class Miner:
         opts.budget_bytes = budget_bytes
         opts.chunk_rows = chunk_rows
         opts.two_pass = 1 if two_pass else 0
+        opts.set_kind = {"auto": L.GS_SETS_AUTO, "bits": L.GS_SETS_BITS,
+                         "bytes": L.GS_SETS_BYTES}[sets]
         self.mni_mode = opts.mni_mode
         self.graph = g
         h = ctypes.c_void_p()

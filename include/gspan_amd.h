@@ -60,8 +60,14 @@ typedef struct gs_opts {
     long long chunk_rows;            /* parent rows per extension pass; 0 = 16M        */
     int two_pass;                    /* 1: always count, then fill (no single-pass last
                                         level)                                          */
-    int reserved;
+    int set_kind;                    /* MNI image sets: GS_SETS_AUTO (byte per vertex
+                                        where the budget allows, else bit per vertex),
+                                        GS_SETS_BITS or GS_SETS_BYTES                  */
 } gs_opts;
+
+#define GS_SETS_AUTO 0
+#define GS_SETS_BITS 1
+#define GS_SETS_BYTES 2
 
 typedef struct gs_stats {
     long long extended;     /* child rows emitted by extension (gspan.cu:708,717,730)    */

@@ -148,6 +148,21 @@ def test_chunked_and_split_runs_agree(oracle_mod):
             assert s["extended"] == st["extended"] and s["count_launches"] > 0
 
 
+@pytest.mark.parametrize("sets", ["bits", "bytes"])
+def test_set_kinds_match_oracle(oracle_mod, sets):
+    """MNI image sets as bitmaps (atomic OR) and as bytemaps (plain byte stores), in the
+    single-pass last level and in the two-pass path, give the oracle's supports."""
+    g = synthetic("rmat", scale=11, edge_factor=8, n_vlabels=6, vlabel_zipf=1.1, n_elabels=2,
+                  seed=7)
+    _announce(f"mine rmat scale 11 k=3 min_sup=40 sets={sets}")
+    want, st = oracle_mod.fsm(g, 3, 40)
+    for mode in ("batched", "plain"):
+        for two_pass in (False, True):
+            with Miner(g, mode=mode, sets=sets, two_pass=two_pass) as mn:
+                assert mn.mine(3, 40) == want, (mode, two_pass)
+                assert mn.stats()["extended"] == st["extended"]
+
+
 def test_prune_graph_matches_oracle(oracle_mod):
     """gs_prune_graph (graph.cc:216-266 on the device) against the oracle's restatement,
     at every threshold that changes the kept set; then mining the pruned graph must give
