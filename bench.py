@@ -90,8 +90,9 @@ def cpu_baseline(g, w: dict, k: int, target_s: float) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    # warmup runs also let the launch-shape tuner try each shape once per bucket
+    ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--workload", default="er10k", choices=sorted(WORKLOADS))
     ap.add_argument("--k", type=int, default=0)
     ap.add_argument("--min-sup", type=int, default=0)
@@ -142,6 +143,7 @@ def main() -> int:
             times.append(time.perf_counter() - t)
             log(f"step {i}: {times[-1]:.3f}s")
         st = mn.stats()
+        tuning = mn.tuner_report()
     pats = Miner.to_patterns(*res)
     steps = args.steps
     ext_per_step = st["extended"] / steps
@@ -247,6 +249,7 @@ def main() -> int:
         "predicted": predicted,
         "predicted_vs_plain": (predicted["value"] / plain["value"]) if (plain and predicted) else None,
         "roofline": roofline,
+        "autotune": tuning,
         "cpu_baseline": cpu,
         "peak_device_bytes": st["peak_device_bytes"],
     }

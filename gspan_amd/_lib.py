@@ -45,6 +45,8 @@ class GsOpts(ctypes.Structure):
         ("chunk_rows", ctypes.c_longlong),
         ("two_pass", ctypes.c_int),
         ("set_kind", ctypes.c_int),
+        ("autotune", ctypes.c_int),
+        ("tile", ctypes.c_int),
     ]
 
 
@@ -124,6 +126,7 @@ def lib():
     L.gs_mlp_forward.argtypes = [vp, P(c_f), c_ll, P(c_f)]
     L.gs_mlp_gradients.argtypes = [vp, P(c_f), P(c_f), c_ll, P(c_f), P(c_f)]
     L.gs_prune_graph.argtypes = [vp, c_int, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]
+    L.gs_tuner_report.argtypes = [vp, ctypes.c_char_p, c_int, P(c_int)]
     L.gs_result_count.argtypes = [vp, P(c_ll), P(c_ll)]
     L.gs_result_copy.argtypes = [vp, P(c_int), P(c_int), P(c_int)]
     L.gs_get_stats.argtypes = [vp, P(GsStats)]
@@ -135,7 +138,7 @@ def lib():
                  "gs_is_canonical_batch", "gs_result_count", "gs_result_copy", "gs_get_stats",
                  "gs_reset_stats", "gs_destroy", "gs_predictor_log", "gs_predictor_samples",
                  "gs_predictor_train", "gs_predictor_get_weights", "gs_predictor_set_weights",
-                 "gs_mlp_forward", "gs_mlp_gradients", "gs_prune_graph"):
+                 "gs_mlp_forward", "gs_mlp_gradients", "gs_prune_graph", "gs_tuner_report"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "gspan_amd.h"
+
 // Largest pattern the kernels handle: 7 edges, 8 vertices.  Projections in the
 // canonicality kernel pack 8 three-bit vertex ids and an 8-bit used-edge mask into 32 bits.
 #define GS_MAXE 7
@@ -24,7 +26,9 @@
 // Child-count predictor: MLP GS_FEATURES -> GS_HIDDEN -> GS_HIDDEN -> 1.
 #define GS_FEATURES 32
 #define GS_HIDDEN 128
-#define GS_MLP_PARAMS (GS_FEATURES * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN + 1)
+#define GS_MLP_PARAM_COUNT \
+    (GS_FEATURES * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN * GS_HIDDEN + GS_HIDDEN + GS_HIDDEN + 1)
+static_assert(GS_MLP_PARAMS == GS_MLP_PARAM_COUNT, "parameter count of the public header");
 
 // Set kinds for MNI image sets: one bit per graph vertex (inserted with an atomic OR),
 // an open-addressing table (compare-and-swap), or one byte per graph vertex (a plain store
@@ -115,19 +119,27 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
                           long long tmp_elems, hipStream_t s);
 long long scan_tmp_elems(long long n);
 
+// Launch shape of the extension kernels: items per stolen tile (256, 512, 1024) and
+// persistent workgroups.
+struct ItemsLaunch {
+    int tile;
+    int grid;
+};
+
 // tile_counters: GS_XCDS * GS_XCD_STRIDE ints, reset by the launchers.
-hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
-                       const long long* woff, long long b0, long long R, KeySpace ks,
-                       int* key_count, int* tile_counters, int grid, hipStream_t s);
+hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
+                       KeySpace ks, int* key_count, int* tile_counters, hipStream_t s);
 // key_count (nullable): also count children per key; key_slot == nullptr: slot = key.
 // cap_end (nullable): rows of slot s may only be written below cap_end[s]; a child that
 // does not fit, or a hash set that fills up, sets *overflow instead (predicted sizes).
-hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
-                      const long long* woff, long long b0, long long R, KeySpace ks,
-                      int* key_count, const int* key_slot, int* cursor, const int* cap_end,
-                      int2* rows_out, const SetDesc* sets, uint32_t* pool, int* overflow,
-                      int* tile_counters, int grid, int nlive, hipStream_t s);
-int items_grid(int cus);
+hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                      const RowCtx* ctx, const long long* woff, long long b0, long long R,
+                      KeySpace ks, int* key_count, const int* key_slot, int* cursor,
+                      const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
+                      int* overflow, int* tile_counters, int nlive, hipStream_t s);
+// resident workgroups per CU of the count (fill = false) or fill kernel at a tile size
+int items_blocks_per_cu(bool fill, int tile);
 
 hipError_t compact_active(const int* key_count, long long K, int* active_keys,
                           int* active_counts, int* n_active, hipStream_t s);

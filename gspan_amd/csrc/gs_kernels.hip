@@ -340,30 +340,34 @@ __global__ __launch_bounds__(GS_BLOCK) void k_scan_down(const int* in, long long
 
 // Per-tile key aggregation in LDS: an open-addressing table (key -> items in this tile)
 // so that a tile issues one global atomic per distinct child key, however hot the key.
-constexpr int ITEMS_PER_THREAD = GS_TILE / GS_BLOCK;
-constexpr int LDS_SLOTS = 2 * GS_TILE;  // > distinct keys of a tile, so probing ends
+// A table has 2*TILE slots, more than the distinct keys of a tile, so probing ends.
+constexpr int log2i(int x) { return x <= 1 ? 0 : 1 + log2i(x / 2); }
 
+template <int SLOTS>
 __device__ __forceinline__ int lds_slot(int* s_key, int key) {
-    unsigned h = ((unsigned)key * 0x9E3779B1u) >> 21;  // 11 bits = LDS_SLOTS
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - log2i(SLOTS));
     for (;;) {
         const int prev = atomicCAS(&s_key[h], -1, key);
         if (prev == -1 || prev == key) return (int)h;
-        h = (h + 1) & (LDS_SLOTS - 1);
+        h = (h + 1) & (SLOTS - 1);
     }
 }
 
 // Inserts key into an LDS set; true for the one caller that inserted it.
+template <int SLOTS>
 __device__ __forceinline__ bool lds_first(int* s_set, int key) {
-    unsigned h = ((unsigned)key * 0x9E3779B1u) >> 21;
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - log2i(SLOTS));
     for (;;) {
         const int prev = atomicCAS(&s_set[h], -1, key);
         if (prev == -1) return true;
         if (prev == key) return false;
-        h = (h + 1) & (LDS_SLOTS - 1);
+        h = (h + 1) & (SLOTS - 1);
     }
 }
 
-template <bool FILL>
+// TILE items per stolen tile (256, 512 or 1024); LDS use grows with TILE, so the tile
+// size trades the number of steals against workgroups resident per CU.
+template <bool FILL, int TILE>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
                                                     const long long* __restrict__ woff,
@@ -374,8 +378,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
                                                     uint32_t* pool, int* overflow,
                                                     int* tile_counters, int dedup) {
-    static_assert(LDS_SLOTS == 2048, "lds_slot hashes to 11 bits");
-    __shared__ long long s_off[GS_TILE + 1];
+    constexpr int LDS_SLOTS = 2 * TILE;
+    constexpr int ITEMS_PER_THREAD = TILE / GS_BLOCK;
+    static_assert(TILE % GS_BLOCK == 0 && (TILE & (TILE - 1)) == 0, "tile: power of two");
+    __shared__ long long s_off[TILE + 1];
     __shared__ int s_key[LDS_SLOTS];
     __shared__ int s_cnt[LDS_SLOTS];
     __shared__ int s_base[FILL ? LDS_SLOTS : 1];
@@ -390,7 +396,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         if (FILL) s_pair[h] = -1;
     }
     const long long W = woff[R];
-    const long long ntiles = (W + GS_TILE - 1) / GS_TILE;
+    const long long ntiles = (W + TILE - 1) / TILE;
     // XCD-aware stealing: workgroups are dealt to the 8 XCDs round-robin, so workgroup b
     // first drains the b % 8-th contiguous eighth of the tiles (whose rows, patterns and
     // child sets then stay in that XCD's L2), then helps the other eighths.
@@ -414,8 +420,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         __syncthreads();
         const long long tile = s_tile;
         if (tile < 0) break;
-        const long long t0 = tile * GS_TILE;
-        const long long t1 = min(t0 + GS_TILE, W);
+        const long long t0 = tile * TILE;
+        const long long t1 = min(t0 + TILE, W);
         if (threadIdx.x == 0) {  // row holding item t0
             long long lo = 0, hi = R - 1;
             while (lo < hi) {
@@ -428,8 +434,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         __syncthreads();
         const long long rlo = s_rlo;
         // every row has >= 1 item (u_n always has a neighbour), so a tile spans at most
-        // GS_TILE rows
-        const int nrows = (int)min((long long)GS_TILE, R - rlo);
+        // TILE rows
+        const int nrows = (int)min((long long)TILE, R - rlo);
         for (int i = threadIdx.x; i <= nrows; i += blockDim.x) s_off[i] = woff[rlo + i];
         __syncthreads();
 
@@ -459,9 +465,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             int agg = -1;  // aggregation key: child key (count) / live slot (fill)
             if (o[q].valid) agg = (FILL && key_slot) ? key_slot[o[q].key] : o[q].key;
             if (agg >= 0) {
-                hs[q] = lds_slot(s_key, agg);
+                hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
-                if (FILL && dedup) lead[q] = lds_first(s_pair, agg * GS_TILE + row[q]);
+                if (FILL && dedup) lead[q] = lds_first<LDS_SLOTS>(s_pair, agg * TILE + row[q]);
             }
         }
         __syncthreads();
@@ -1015,41 +1021,65 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
     return hipGetLastError();
 }
 
-hipError_t items_count(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
-                       const long long* woff, long long b0, long long R, KeySpace ks,
-                       int* key_count, int* tile_counters, int grid, hipStream_t s) {
-    if (R <= 0) return hipSuccess;
+template <bool FILL>
+hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
+                        KeySpace ks, int* key_count, const int* key_slot, int* cursor,
+                        const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
+                        int* overflow, int* tile_counters, int dedup, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
                        (long long)GS_XCDS * GS_XCD_STRIDE);
-    hipLaunchKernelGGL(k_items<false>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, key_count, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                       tile_counters, 0);
+#define GS_LAUNCH_ITEMS(T)                                                                     \
+    hipLaunchKernelGGL((k_items<FILL, T>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, \
+                       woff, b0, R, ks, key_count, key_slot, cursor, cap_end, rows_out, sets,  \
+                       pool, overflow, tile_counters, dedup)
+    switch (lc.tile) {
+        case 256: GS_LAUNCH_ITEMS(256); break;
+        case 512: GS_LAUNCH_ITEMS(512); break;
+        case 1024: GS_LAUNCH_ITEMS(1024); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef GS_LAUNCH_ITEMS
     return hipGetLastError();
 }
 
-hipError_t items_fill(const DevGraph& g, const PatDev* pats, const RowCtx* ctx,
-                      const long long* woff, long long b0, long long R, KeySpace ks,
-                      int* key_count, const int* key_slot, int* cursor, const int* cap_end,
-                      int2* rows_out, const SetDesc* sets, uint32_t* pool, int* overflow,
-                      int* tile_counters, int grid, int nlive, hipStream_t s) {
+hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
+                       KeySpace ks, int* key_count, int* tile_counters, hipStream_t s) {
     if (R <= 0) return hipSuccess;
-    // pair keys slot * GS_TILE + row must stay below 2^31
-    const int dedup = nlive < (1 << 21) ? 1 : 0;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
-                       (long long)GS_XCDS * GS_XCD_STRIDE);
-    hipLaunchKernelGGL(k_items<true>, dim3(grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, woff, b0, R,
-                       ks, key_count, key_slot, cursor, cap_end, rows_out, sets, pool, overflow,
-                       tile_counters, dedup);
-    return hipGetLastError();
+    return launch_items<false>(lc, g, pats, ctx, woff, b0, R, ks, key_count, nullptr, nullptr,
+                               nullptr, nullptr, nullptr, nullptr, nullptr, tile_counters, 0, s);
 }
 
-int items_grid(int cus) {
+hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                      const RowCtx* ctx, const long long* woff, long long b0, long long R,
+                      KeySpace ks, int* key_count, const int* key_slot, int* cursor,
+                      const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
+                      int* overflow, int* tile_counters, int nlive, hipStream_t s) {
+    if (R <= 0) return hipSuccess;
+    // pair keys slot * TILE + row must stay below 2^31
+    const int dedup = (long long)nlive * lc.tile < (1LL << 31) ? 1 : 0;
+    return launch_items<true>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot, cursor,
+                              cap_end, rows_out, sets, pool, overflow, tile_counters, dedup, s);
+}
+
+int items_blocks_per_cu(bool fill, int tile) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_items<true>, GS_BLOCK, 0) !=
-            hipSuccess ||
-        per_cu <= 0)
-        per_cu = 4;
-    return cus * per_cu;
+    hipError_t e = hipErrorInvalidValue;
+#define GS_OCC(F, T) \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_items<F, T>, GS_BLOCK, 0)
+    if (fill) {
+        if (tile == 256) GS_OCC(true, 256);
+        else if (tile == 512) GS_OCC(true, 512);
+        else if (tile == 1024) GS_OCC(true, 1024);
+    } else {
+        if (tile == 256) GS_OCC(false, 256);
+        else if (tile == 512) GS_OCC(false, 512);
+        else if (tile == 1024) GS_OCC(false, 1024);
+    }
+#undef GS_OCC
+    if (e != hipSuccess || per_cu <= 0) per_cu = 2;
+    return per_cu;
 }
 
 hipError_t compact_active(const int* key_count, long long K, int* active_keys,

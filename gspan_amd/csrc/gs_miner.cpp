@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "gs_internal.h"
+#include "gs_tuner.h"
 #include "gspan_amd.h"
 
 namespace {
@@ -210,15 +211,22 @@ struct gs_ctx {
     DevGraph dg{};
     DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
     DBuf d_tiles; // per-XCD tile queues of the item kernels
-    int grid_items = 0;
+    LaunchTuner tuner;  // launch shape of the item kernels
     std::string err;
     gs_stats st{};
     size_t budget = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // timing=1: event pairs recorded around count/fill launches, resolved at the next
-    // host synchronisation so that timing adds no synchronisation of its own
+    // event pairs recorded around count/fill launches (timing=1, or a launch the tuner
+    // measures), resolved at the next host synchronisation so that timing adds no
+    // synchronisation of its own
+    struct PendingEvent {
+        int ev;                 // index in ev_pool
+        int kind;               // 0 = count, 1 = fill
+        int bucket, cfg;        // tuner bucket / shape, or -1
+        const long long* items; // work items of the launch, valid at resolution
+    };
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
-    std::vector<std::pair<int, int>> ev_pending;  // (pool index, 0 = count / 1 = fill)
+    std::vector<PendingEvent> ev_pending;
     size_t ev_used = 0;
     // mining state
     int k = 0, min_sup = 0, mni_mode = 0;
@@ -491,8 +499,9 @@ void mlp_init(gs_ctx* c) {
 void resolve_timing(gs_ctx* c) {
     for (auto& pk : c->ev_pending) {
         float ms = 0;
-        HIPCHECK(hipEventElapsedTime(&ms, c->ev_pool[pk.first].first, c->ev_pool[pk.first].second));
-        (pk.second == 0 ? c->st.t_count_ms : c->st.t_fill_ms) += ms;
+        HIPCHECK(hipEventElapsedTime(&ms, c->ev_pool[pk.ev].first, c->ev_pool[pk.ev].second));
+        if (c->opts.timing) (pk.kind == 0 ? c->st.t_count_ms : c->st.t_fill_ms) += ms;
+        c->tuner.record(pk.bucket, pk.cfg, ms, pk.items ? *pk.items : 0);
     }
     c->ev_pending.clear();
     c->ev_used = 0;
@@ -613,8 +622,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      d_tmp.as<long long>(), tmp_elems, c->stream));
         return Rc;
     };
+    // launch shape from the tuner, and events around the launch when they are wanted
+    int t_bucket = -1, t_cfg = 0;
+    auto shape = [&](bool fill, bool single_pass, long long Rc) {
+        return c->tuner.choose(fill, single_pass, m, Rc, &t_bucket, &t_cfg);
+    };
     auto time_begin = [&]() {
-        if (!c->opts.timing) return;
+        if (!c->opts.timing && t_bucket < 0) return;
         if (c->ev_used == c->ev_pool.size()) {
             hipEvent_t a, b;
             HIPCHECK(hipEventCreate(&a));
@@ -623,10 +637,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         }
         HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].first, c->stream));
     };
-    auto time_end = [&](int kind) {
-        if (!c->opts.timing) return;
+    auto time_end = [&](int kind, const long long* items) {
+        if (!c->opts.timing && t_bucket < 0) return;
         HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].second, c->stream));
-        c->ev_pending.emplace_back((int)c->ev_used, kind);
+        c->ev_pending.push_back({(int)c->ev_used, kind, t_bucket, t_cfg, items});
         ++c->ev_used;
     };
 
@@ -693,14 +707,15 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 const long long b0 = (long long)ci * chunk;
                 const long long Rc = prep_chunk(ci);
                 d2h(c, &Wc_p[ci], d_woff.as<long long>() + Rc, 1);
+                const gsk::ItemsLaunch lc = shape(true, false, Rc);
                 time_begin();
-                HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+                HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                          d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
                                          nullptr, d_cur.as<int>(), d_end.as<int>(),
                                          rows.as<int2>(), d_psets.as<SetDesc>(),
                                          d_ppool.as<uint32_t>(), c->overflow_flag(),
-                                         c->tile_counters(), c->grid_items, (int)K, c->stream));
-                time_end(1);
+                                         c->tile_counters(), (int)K, c->stream));
+                time_end(1, &Wc_p[ci]);
                 c->st.fill_launches++;
             }
             DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K),
@@ -796,20 +811,21 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         const long long b0 = (long long)ci * chunk;
         const long long Rc = prep_chunk(ci);
         d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
+        const gsk::ItemsLaunch lc = shape(single, single, Rc);
         time_begin();
         if (single) {
-            HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+            HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
                                      nullptr, nullptr, nullptr, d_all_sets.as<SetDesc>(),
                                      d_all_pool.as<uint32_t>(), c->overflow_flag(),
-                                     c->tile_counters(), c->grid_items, (int)K, c->stream));
-            time_end(1);
+                                     c->tile_counters(), (int)K, c->stream));
+            time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
         } else {
-            HIPCHECK(gsk::items_count(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+            HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                       d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
-                                      c->tile_counters(), c->grid_items, c->stream));
-            time_end(0);
+                                      c->tile_counters(), c->stream));
+            time_end(0, &Wc_host[ci]);
             c->st.count_launches++;
         }
     }
@@ -944,15 +960,15 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         for (int ci = 0; ci < nchunks; ++ci) {
             const long long b0 = (long long)ci * chunk;
             const long long Rc = nchunks > 1 ? prep_chunk(ci) : std::min(chunk, R - b0);
+            const gsk::ItemsLaunch lc = shape(true, false, Rc);
             time_begin();
-            HIPCHECK(gsk::items_fill(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
+            HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, nullptr,
                                      d_slot.as<int>(), d_cursor.as<int>(), nullptr,
                                      write_rows ? out.rows.as<int2>() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
-                                     c->overflow_flag(), c->tile_counters(), c->grid_items, NL,
-                                     c->stream));
-            time_end(1);
+                                     c->overflow_flag(), c->tile_counters(), NL, c->stream));
+            time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
         }
     }
@@ -1232,6 +1248,7 @@ int gs_default_opts(gs_opts* o) {
     std::memset(o, 0, sizeof(*o));
     o->mode = GS_MODE_BATCHED;
     o->mni_mode = GS_MNI_GPU;
+    o->autotune = 1;
     return GS_OK;
 }
 
@@ -1287,7 +1304,10 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, x->device));
         x->cus = prop.multiProcessorCount;
-        x->grid_items = gsk::items_grid(x->cus);
+        if (x->opts.tile != 0 && x->opts.tile != 256 && x->opts.tile != 512 &&
+            x->opts.tile != 1024)
+            throw GsError(GS_ERR_INVALID, "tile must be 0, 256, 512 or 1024");
+        x->tuner.init(x->cus, x->opts.autotune != 0, x->opts.tile);
         size_t fr = 0, tot = 0;
         HIPCHECK(hipMemGetInfo(&fr, &tot));
         x->budget = x->opts.budget_bytes ? (size_t)x->opts.budget_bytes : (size_t)(fr * 0.85);
@@ -1653,6 +1673,18 @@ int gs_prune_graph(gs_ctx* c, int min_sup, int* n_out, int* nnz_out, int* offset
         d2h(c, edges3_out, d_edges.as<int>(), (size_t)sizes[1] * 3);
         sync(c);
     });
+}
+
+int gs_tuner_report(const gs_ctx* c, char* buf, int cap, int* len) {
+    if (!c || !len || cap < 0 || (cap > 0 && !buf)) return GS_ERR_INVALID;
+    const std::string s = c->tuner.report();
+    *len = (int)s.size();
+    if (cap > 0) {
+        const size_t k = std::min(s.size(), (size_t)cap - 1);
+        std::memcpy(buf, s.data(), k);
+        buf[k] = 0;
+    }
+    return GS_OK;
 }
 
 int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {

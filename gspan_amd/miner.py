@@ -42,7 +42,8 @@ class Miner:
 
     def __init__(self, g: CSRGraph, mode: str = "batched", mni: str = "gpu", device: int = 0,
                  timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0,
-                 two_pass: bool = False, sets: str = "auto"):
+                 two_pass: bool = False, sets: str = "auto", autotune: bool = True,
+                 tile: int = 0):
         self._lib = L.lib()
         opts = L.GsOpts()
         self._lib.gs_default_opts(ctypes.byref(opts))
@@ -56,6 +57,8 @@ class Miner:
         opts.two_pass = 1 if two_pass else 0
         opts.set_kind = {"auto": L.GS_SETS_AUTO, "bits": L.GS_SETS_BITS,
                          "bytes": L.GS_SETS_BYTES}[sets]
+        opts.autotune = 1 if autotune else 0
+        opts.tile = tile
         self.mni_mode = opts.mni_mode
         self.graph = g
         h = ctypes.c_void_p()
@@ -207,6 +210,15 @@ class Miner:
         self._check(self._lib.gs_mlp_gradients(self._h, _fp(x), _fp(t), x.shape[0], _fp(g),
                                                ctypes.byref(loss)))
         return g, loss.value
+
+    def tuner_report(self) -> dict:
+        """The launch-shape tuner's per-bucket choice and time per item of each shape."""
+        import json
+        n = ctypes.c_int()
+        self._check(self._lib.gs_tuner_report(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        self._check(self._lib.gs_tuner_report(self._h, buf, n.value + 1, ctypes.byref(n)))
+        return json.loads(buf.value.decode())
 
     def stats(self) -> dict:
         s = L.GsStats()

@@ -63,6 +63,10 @@ typedef struct gs_opts {
     int set_kind;                    /* MNI image sets: GS_SETS_AUTO (byte per vertex
                                         where the budget allows, else bit per vertex),
                                         GS_SETS_BITS or GS_SETS_BYTES                  */
+    int autotune;                    /* 1 (default): choose the extension kernels' tile
+                                        size and workgroups per CU by timing launches  */
+    int tile;                        /* 0, or a fixed tile size (256, 512, 1024) with
+                                        full occupancy (turns the tuner off)           */
 } gs_opts;
 
 #define GS_SETS_AUTO 0
@@ -150,6 +154,12 @@ int gs_mlp_gradients(gs_ctx* ctx, const float* x, const float* t, long long n, f
  * (the input sizes); *n_out / *nnz_out receive the pruned sizes. */
 int gs_prune_graph(gs_ctx* ctx, int min_sup, int* n_out, int* nnz_out, int* offsets_out,
                    int* vlabels_out, int* edges3_out);
+
+/* The launch-shape tuner's measurements as a JSON object: per bucket (pass, single-pass
+ * last level, parent pattern size) the chosen [tile, workgroups] and the time per work item
+ * of every shape tried.  Writes at most cap bytes including the terminating NUL; *len
+ * receives the full length.  No counterpart in the reference (its extension is OpenMP). */
+int gs_tuner_report(const gs_ctx* ctx, char* buf, int cap, int* len);
 
 /* Results of the last gs_mine, in report order, as flat arrays (for callers that want
  * them in bulk instead of through a callback; gs_mine may then be called with cb = NULL).

@@ -163,6 +163,30 @@ def test_set_kinds_match_oracle(oracle_mod, sets):
                 assert mn.stats()["extended"] == st["extended"]
 
 
+def test_tile_shapes_and_tuner(oracle_mod):
+    """Every tile size of the extension kernels gives the oracle's report; the tuner times
+    each launch shape once per bucket and then keeps the fastest, without changing results."""
+    g = synthetic("er", n=10000, m=40000, n_vlabels=5, n_elabels=1, seed=1)
+    _announce("ER 10k k=3 tiles 256/512/1024 + tuner")
+    want, st = oracle_mod.fsm(g, 3, 200)
+    for tile in (256, 512, 1024):
+        with Miner(g, tile=tile) as mn:
+            assert mn.mine(3, 200) == want, tile
+            assert mn.stats()["extended"] == st["extended"]
+    with Miner(g, autotune=True) as mn:
+        for _ in range(8):  # enough runs to try every shape in every bucket
+            assert mn.mine(3, 200) == want
+        rep = mn.tuner_report()
+    assert rep, "no launch was large enough to tune"
+    for name, b in rep.items():
+        shapes = b["ns_per_item"]
+        assert {s[0] for s in shapes} == {256, 512, 1024}, name
+        tried = [s for s in shapes if s[2] >= 0]
+        assert len(tried) == len(shapes), (name, shapes)
+        best = min(tried, key=lambda s: s[2])
+        assert b["best"] == [best[0], best[1]], (name, b)
+
+
 def test_prune_graph_matches_oracle(oracle_mod):
     """gs_prune_graph (graph.cc:216-266 on the device) against the oracle's restatement,
     at every threshold that changes the kept set; then mining the pruned graph must give

@@ -21,6 +21,9 @@ def test_default_opts():
     o = L.GsOpts()
     assert L.lib().gs_default_opts(ctypes.byref(o)) == L.GS_OK
     assert o.mode == L.GS_MODE_BATCHED and o.mni_mode == L.GS_MNI_GPU
+    assert o.autotune == 1 and o.tile == 0 and o.set_kind == L.GS_SETS_AUTO
+    n = ctypes.c_int()
+    assert L.lib().gs_tuner_report(None, None, 0, ctypes.byref(n)) == L.GS_ERR_INVALID
 
 
 def _create(offsets, vlabels, edges, n, nnz):
