@@ -96,6 +96,8 @@ def main() -> int:
     ap.add_argument("--workload", default="er10k", choices=sorted(WORKLOADS))
     ap.add_argument("--k", type=int, default=0)
     ap.add_argument("--min-sup", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="fixed tile size of the extension kernels (turns the tuner off)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-predicted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -130,7 +132,7 @@ def main() -> int:
     gstats = g.stats()
     log(f"graph {args.workload}: {gstats} ({time.perf_counter() - t0:.1f}s)")
 
-    with Miner(g, mode="batched", timing=True) as mn:
+    with Miner(g, mode="batched", timing=True, tile=args.tile) as mn:
         for i in range(args.warmup):
             t = time.perf_counter()
             res = mn.mine_raw(k, ms)

@@ -49,6 +49,14 @@ struct PatDev {
     int cfrom[GS_MAXE], cto[GS_MAXE], cel[GS_MAXE];
     int vlab[GS_MAXV];  // label rank of every pattern vertex
     int rm[GS_MAXE];    // rightmost path (code edge indices, last forward edge first)
+    // Rule constants of the item kernels.  Forward segment j scans the neighbours of
+    // pattern vertex seg_v[j] (j = 0: v_n; j >= 1: rmpath[j-1].from); an entry (el, tl)
+    // of segment j passes when seg_el[j] < el || (seg_el[j] == el && seg_tl[j] <= tl)
+    // (gspan.cu:476-477; seg_el[0] = -1 passes everything).  Backward probe i (1..r-1)
+    // looks for an entry of v_n to bw_v[i] = rmpath[i].from that passes
+    // bw_el[i] < el || (bw_el[i] == el && bw_le[i]) (gspan.cu:436-440).
+    int seg_v[GS_MAXE], seg_el[GS_MAXE], seg_tl[GS_MAXE];
+    int bw_v[GS_MAXE], bw_el[GS_MAXE], bw_le[GS_MAXE];
     int support;           // MNI support of the pattern
     int pad0;
     long long row_begin;   // first row in the parent level array
@@ -58,12 +66,16 @@ struct PatDev {
 
 // Per parent row context written by the prep kernel and read by the item kernels.
 // phi: image of every pattern vertex; fcum[j]: forward items through segment j
-// (segment 0 = neighbours of u_n, segment i+1 = neighbours of rmpath[i].from).
+// (segment 0 = neighbours of u_n, segment i+1 = neighbours of rmpath[i].from); the CSR
+// entry of forward item f in segment j is base[j] + f.
 struct RowCtx {
     int phi[GS_MAXV];
-    int p;  // pattern index in the batch
     int fcum[GS_MAXE];
+    int p;  // pattern index in the batch
+    int base[GS_MAXE];
+    int pad;
 };
+static_assert(sizeof(RowCtx) == 96, "RowCtx is six 16-byte loads");
 
 struct LayerPtrs {
     const int2* layer[GS_MAXE];  // layer[l] = rows of code edge l: {prev, eid}

@@ -62,16 +62,64 @@ __device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int
     }
 }
 
+// A row context in registers: eight-lane integer vectors, indexed by compare-and-select
+// (a per-lane index into an array would put the array in scratch memory).
+typedef int int8v __attribute__((ext_vector_type(8)));
+
+struct RowRegs {
+    int8v phi, fcum, base;  // fcum[7] = base[7] = 0
+    int p;
+};
+
+__device__ __forceinline__ RowRegs load_row(const RowCtx* c) {
+    const int4* q = reinterpret_cast<const int4*>(c);
+    const int4 a = q[0], b = q[1], d = q[2], e = q[3], f = q[4], h = q[5];
+    RowRegs r;
+    r.phi = int8v{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    r.fcum = int8v{d.x, d.y, d.z, d.w, e.x, e.y, e.z, 0};
+    r.p = e.w;
+    r.base = int8v{f.x, f.y, f.z, f.w, h.x, h.y, h.z, 0};
+    return r;
+}
+
+__device__ __forceinline__ int pick(const int8v& v, int i) {
+    int r = v[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) r = (i == k) ? v[k] : r;
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ int pick(const int (&a)[N], int i) {
+    int r = a[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) r = (i == k) ? a[k] : r;
+    return r;
+}
+
+// Element i of a pattern table: when the pattern is the same for the whole wave (UNI)
+// its tables are wave-uniform, read with scalar loads and selected per lane; otherwise
+// one vector load.
+template <bool UNI, int N>
+__device__ __forceinline__ int pget(const int (&a)[N], int i) {
+    if (UNI) return pick(a, i);
+    return a[i];
+}
+
 // has_edge (gspan.cu:396-401) on a row given by its vertex images: some pattern edge
 // with label el joins u and w.
-__device__ __forceinline__ bool row_has_edge(const PatDev& P, const int* phi, int u, int w,
+template <bool UNI>
+__device__ __forceinline__ bool row_has_edge(const PatDev& P, const RowRegs& c, int u, int w,
                                              int el) {
-    for (int j = 0; j < P.m; ++j) {
-        if (P.cel[j] != el) continue;
-        int a = phi[P.cfrom[j]], b = phi[P.cto[j]];
-        if ((a == u && b == w) || (a == w && b == u)) return true;
+    bool hit = false;
+#pragma unroll
+    for (int j = 0; j < GS_MAXE; ++j) {
+        if (j < P.m && P.cel[j] == el) {
+            const int a = pick(c.phi, P.cfrom[j]), b = pick(c.phi, P.cto[j]);
+            hit |= (a == u && b == w) || (a == w && b == u);
+        }
     }
-    return false;
+    return hit;
 }
 
 struct ItemOut {
@@ -84,30 +132,31 @@ struct ItemOut {
 // One work item of a parent row: item q < r-1 is the backward probe of rmpath index
 // q+1 (gspan.cu:428-445); the rest walk the neighbours of u_n (first forward,
 // gspan.cu:447-462) and of every rightmost-path source (other forward, gspan.cu:464-483).
-__device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowCtx& c,
-                                             long long q, KeySpace ks) {
+// c is the row's context in registers.
+template <bool UNI>
+__device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowRegs& c,
+                                             int q, KeySpace ks) {
     ItemOut o{false, 0, 0, -1};
     const int r = P.r;
     if (q < r - 1) {
-        const int i = (int)q + 1;
-        const int k = P.rm[i];
-        const int u_n = c.phi[P.v_n];
-        const int target = c.phi[P.cfrom[k]];
-        const int e_el = P.cel[k];
-        const bool le = P.vlab[P.cto[k]] <= P.vlab[P.v_n];
-        int lo = g.offsets[u_n];
-        const int end = g.offsets[u_n + 1];
+        const int i = q + 1;
+        const int u_n = pick(c.phi, P.v_n);
+        const int target = pick(c.phi, pget<UNI>(P.bw_v, i));
+        const int e_el = pget<UNI>(P.bw_el, i);
+        const bool le = pget<UNI>(P.bw_le, i) != 0;
+        int lo = c.base[0];  // neighbours of u_n: forward segment 0
+        const int end = lo + c.fcum[0];
         int hi = end;
         while (lo < hi) {  // first entry with to >= target; adjacency sorted by (to, label)
-            int mid = (lo + hi) >> 1;
+            const int mid = (lo + hi) >> 1;
             if ((int)g.adj[mid].x < target) lo = mid + 1;
             else hi = mid;
         }
         for (int e = lo; e < end; ++e) {
-            uint2 a = g.adj[e];
+            const uint2 a = g.adj[e];
             if ((int)a.x != target) break;
-            int el = (int)(a.y >> 16);
-            if (row_has_edge(P, c.phi, u_n, target, el)) continue;
+            const int el = (int)(a.y >> 16);
+            if (row_has_edge<UNI>(P, c, u_n, target, el)) continue;
             if (e_el < el || (e_el == el && le)) {
                 o.valid = true;
                 o.eid = e;
@@ -117,21 +166,20 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
         }
         return o;
     }
-    const long long f = q - (r - 1);
-    int j = 0;
-    while (j < r && f >= c.fcum[j]) ++j;
-    const int u = (j == 0) ? c.phi[P.v_n] : c.phi[P.cfrom[P.rm[j - 1]]];
-    const int e = g.offsets[u] + (int)(f - (j ? c.fcum[j - 1] : 0));
+    const int f = q - (r - 1);
+    int j = 0;  // segment: fcum is non-decreasing
+#pragma unroll
+    for (int s = 0; s < GS_MAXE - 1; ++s) j += (s < r && f >= c.fcum[s]) ? 1 : 0;
+    const int e = pick(c.base, j) + f;
     const uint2 a = g.adj[e];
     const int to = (int)a.x, el = (int)(a.y >> 16), tl = (int)(a.y & 0xffffu);
     if (tl < P.min_label) return o;
-    for (int v = 0; v < P.nv; ++v)
-        if (c.phi[v] == to) return o;
-    if (j > 0) {
-        const int k = P.rm[j - 1];
-        const int e_el = P.cel[k];
-        if (!(e_el < el || (e_el == el && P.vlab[P.cto[k]] <= tl))) return o;
-    }
+    bool seen = false;
+#pragma unroll
+    for (int v = 0; v < GS_MAXV; ++v) seen |= v < P.nv && c.phi[v] == to;
+    if (seen) return o;
+    const int e_el = pget<UNI>(P.seg_el, j);
+    if (!(e_el < el || (e_el == el && pget<UNI>(P.seg_tl, j) <= tl))) return o;
     o.valid = true;
     o.eid = e;
     o.newv = to;
@@ -244,27 +292,41 @@ __global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __r
         else hi = mid - 1;
     }
     const PatDev& P = pats[lo];
-    RowCtx c;
-#pragma unroll
-    for (int v = 0; v < GS_MAXV; ++v) c.phi[v] = -1;
+    int8v phi = int8v{-1, -1, -1, -1, -1, -1, -1, -1};
     long long idx = P.row_begin + (b - P.brow_begin);
-    for (int l = P.m - 1; l >= 0; --l) {
-        int2 rr = L.layer[l][idx];
-        if (P.cfrom[l] < P.cto[l]) c.phi[P.cto[l]] = (int)g.adj[rr.y].x;
-        if (l == 0) c.phi[0] = g.src[rr.y];
+#pragma unroll
+    for (int l = GS_MAXE - 1; l >= 0; --l) {
+        if (l >= P.m) continue;
+        const int2 rr = L.layer[l][idx];
+        if (P.cfrom[l] < P.cto[l]) {
+            const int to = (int)g.adj[rr.y].x, v = P.cto[l];
+#pragma unroll
+            for (int k = 0; k < GS_MAXV; ++k) phi[k] = (v == k) ? to : phi[k];
+        }
+        if (l == 0) phi[0] = g.src[rr.y];
         idx = rr.x;
     }
-    c.p = lo;
+    int8v fcum = int8v{0, 0, 0, 0, 0, 0, 0, 0}, base = fcum;
     int fc = 0;
 #pragma unroll
     for (int j = 0; j < GS_MAXE; ++j) {
+        int bj = 0;
         if (j <= P.r) {
-            int u = (j == 0) ? c.phi[P.v_n] : c.phi[P.cfrom[P.rm[j - 1]]];
-            fc += g.offsets[u + 1] - g.offsets[u];
+            const int u = pick(phi, P.seg_v[j]);
+            const int o0 = g.offsets[u];
+            bj = o0 - fc;
+            fc += g.offsets[u + 1] - o0;
         }
-        c.fcum[j] = fc;
+        base[j] = bj;
+        fcum[j] = fc;
     }
-    ctx[lb] = c;
+    int4* out = reinterpret_cast<int4*>(ctx + lb);
+    out[0] = make_int4(phi[0], phi[1], phi[2], phi[3]);
+    out[1] = make_int4(phi[4], phi[5], phi[6], phi[7]);
+    out[2] = make_int4(fcum[0], fcum[1], fcum[2], fcum[3]);
+    out[3] = make_int4(fcum[4], fcum[5], fcum[6], lo);
+    out[4] = make_int4(base[0], base[1], base[2], base[3]);
+    out[5] = make_int4(base[4], base[5], base[6], 0);
     work[lb] = (P.r - 1) + fc;
 }
 
@@ -365,8 +427,29 @@ __device__ __forceinline__ bool lds_first(int* s_set, int key) {
     }
 }
 
+// Inserts key into the workgroup's persistent count table, counting new slots in *fill.
+template <int SLOTS>
+__device__ __forceinline__ int lds_slot_fill(int* s_key, int key, int* fill) {
+    unsigned h = ((unsigned)key * 0x9E3779B1u) >> (32 - log2i(SLOTS));
+    for (;;) {
+        const int prev = atomicCAS(&s_key[h], -1, key);
+        if (prev == -1) {
+            atomicAdd(fill, 1);
+            return (int)h;
+        }
+        if (prev == key) return (int)h;
+        h = (h + 1) & (SLOTS - 1);
+    }
+}
+
 // TILE items per stolen tile (256, 512 or 1024); LDS use grows with TILE, so the tile
 // size trades the number of steals against workgroups resident per CU.
+//
+// Child counts (key_count) are summed per workgroup in a second LDS table that lives
+// across the tiles the workgroup steals, and added to global memory when it is half full
+// and when the workgroup ends: the counters of a level's hot keys then see one atomic
+// per workgroup instead of one per tile (device-scope atomics are performed at the
+// memory side, so atomics on one hot address serialise there).
 template <bool FILL, int TILE>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
@@ -388,13 +471,21 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     // (live slot, row in tile) pairs already seen: only their first item inserts the
     // parent's vertex images, which are the same for every item of the pair
     __shared__ int s_pair[FILL ? LDS_SLOTS : 1];
+    // persistent count table: LDS_SLOTS slots, flushed once more than TILE are used, so a
+    // tile (at most TILE new keys) always finds room and probing ends
+    __shared__ int s_pkey[LDS_SLOTS];
+    __shared__ int s_pcnt[LDS_SLOTS];
+    __shared__ int s_pfill;
     __shared__ long long s_tile, s_rlo;
     if (R <= 0) return;
     for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
         s_key[h] = -1;
         s_cnt[h] = 0;
         if (FILL) s_pair[h] = -1;
+        s_pkey[h] = -1;
+        s_pcnt[h] = 0;
     }
+    if (threadIdx.x == 0) s_pfill = 0;
     const long long W = woff[R];
     const long long ntiles = (W + TILE - 1) / TILE;
     // XCD-aware stealing: workgroups are dealt to the 8 XCDs round-robin, so workgroup b
@@ -441,7 +532,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
 
         // phase 1: evaluate this thread's items, count them per key in LDS
         ItemOut o[ITEMS_PER_THREAD];
-        int row[ITEMS_PER_THREAD], hs[ITEMS_PER_THREAD], rank[ITEMS_PER_THREAD];
+        int row[ITEMS_PER_THREAD], pat[ITEMS_PER_THREAD], hs[ITEMS_PER_THREAD],
+            rank[ITEMS_PER_THREAD];
         bool lead[ITEMS_PER_THREAD];
 #pragma unroll
         for (int q = 0; q < ITEMS_PER_THREAD; ++q) {
@@ -451,6 +543,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             hs[q] = -1;
             rank[q] = 0;
             lead[q] = true;
+            RowRegs c;
+            int p = -1, f = 0;
             if (t < t1) {
                 int lo = 0, hi = nrows - 1;
                 while (lo < hi) {
@@ -459,8 +553,18 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     else hi = mid - 1;
                 }
                 row[q] = lo;
-                const RowCtx& c = ctx[rlo + lo];
-                o[q] = eval_item(g, pats[c.p], c, t - s_off[lo], ks);
+                c = load_row(ctx + rlo + lo);
+                p = c.p;
+                f = (int)(t - s_off[lo]);
+            }
+            pat[q] = p;
+            // rows of one pattern are contiguous, so a wave's items mostly share it: then
+            // the pattern's tables are read once per wave with scalar loads
+            const int p0 = __builtin_amdgcn_readfirstlane(p);
+            if (__all(p == p0 || p < 0)) {
+                if (t < t1) o[q] = eval_item<true>(g, pats[p0], c, f, ks);
+            } else if (t < t1) {
+                o[q] = eval_item<false>(g, pats[p], c, f, ks);
             }
             int agg = -1;  // aggregation key: child key (count) / live slot (fill)
             if (o[q].valid) agg = (FILL && key_slot) ? key_slot[o[q].key] : o[q].key;
@@ -471,14 +575,29 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             }
         }
         __syncthreads();
-        // phase 2: one global atomic per distinct key of the tile
+        // phase 2: one output reservation per distinct key of the tile; counts go to the
+        // workgroup's persistent table
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
             const int key = s_key[h];
             if (key < 0) continue;
             if (FILL && rows_out) s_base[h] = atomicAdd(&cursor[key], s_cnt[h]);
-            if (key_count) atomicAdd(&key_count[key], s_cnt[h]);
+            if (key_count) {
+                const int ph = lds_slot_fill<LDS_SLOTS>(s_pkey, key, &s_pfill);
+                atomicAdd(&s_pcnt[ph], s_cnt[h]);
+            }
         }
         __syncthreads();
+        if (key_count && s_pfill > TILE) {  // uniform: read after the barrier
+            for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+                if (s_pkey[h] >= 0) {
+                    atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
+                    s_pkey[h] = -1;
+                    s_pcnt[h] = 0;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) s_pfill = 0;  // read again only after later barriers
+        }
         // phase 3 (fill): write child rows at their reserved positions, insert images
         if (FILL) {
 #pragma unroll
@@ -486,7 +605,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 if (hs[q] < 0) continue;
                 const int slot = s_key[hs[q]];
                 const RowCtx& c = ctx[rlo + row[q]];
-                const PatDev& P = pats[c.p];
+                const PatDev& P = pats[pat[q]];
                 if (rows_out) {
                     const long long prow = P.row_begin + (b0 + rlo + row[q] - P.brow_begin);
                     const int at = s_base[hs[q]] + rank[q];
@@ -494,10 +613,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     else atomicOr(overflow, 1);
                 }
                 const SetDesc d = sets[slot];
-                if (lead[q])
-                    for (int v = 0; v < P.nv; ++v)
-                        set_insert(d, pool, v, (uint32_t)c.phi[v], overflow);
-                if (o[q].newv >= 0) set_insert(d, pool, P.nv, (uint32_t)o[q].newv, overflow);
+                const int nv = P.nv;
+                if (lead[q]) {
+#pragma unroll
+                    for (int v = 0; v < GS_MAXV; ++v)
+                        if (v < nv) set_insert(d, pool, v, (uint32_t)c.phi[v], overflow);
+                }
+                if (o[q].newv >= 0) set_insert(d, pool, nv, (uint32_t)o[q].newv, overflow);
             }
         }
         __syncthreads();
@@ -509,6 +631,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             if (FILL) s_pair[h] = -1;
         }
     }
+    // the loop ends after a barrier; the persistent table was last written before it
+    if (key_count)
+        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK)
+            if (s_pkey[h] >= 0) atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
 }
 
 // ------------------------------------------------------------------ keys

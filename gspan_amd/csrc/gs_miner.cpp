@@ -285,6 +285,27 @@ PatDev to_dev(const HPat& h) {
         d.cel[j] = j < h.m ? h.el[j] : 0;
     }
     h.labels(d.vlab);
+    for (int j = 0; j < GS_MAXE; ++j) {
+        d.seg_v[j] = 0;
+        d.seg_el[j] = -1;
+        d.seg_tl[j] = 0;
+        d.bw_v[j] = 0;
+        d.bw_el[j] = 0;
+        d.bw_le[j] = 0;
+    }
+    d.seg_v[0] = d.v_n;
+    for (int j = 1; j <= d.r && j < GS_MAXE; ++j) {
+        const int k = rm[j - 1];
+        d.seg_v[j] = d.cfrom[k];
+        d.seg_el[j] = d.cel[k];
+        d.seg_tl[j] = d.vlab[d.cto[k]];
+    }
+    for (int i = 1; i < d.r; ++i) {
+        const int k = rm[i];
+        d.bw_v[i] = d.cfrom[k];
+        d.bw_el[i] = d.cel[k];
+        d.bw_le[i] = d.vlab[d.cto[k]] <= d.vlab[d.v_n] ? 1 : 0;
+    }
     d.support = h.support;
     d.row_begin = h.row_begin;
     d.row_count = h.row_count;
