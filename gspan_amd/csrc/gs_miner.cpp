@@ -190,10 +190,76 @@ struct HPat {
     }
 };
 
+// Position of a pattern in the reference's report order (DFS pre-order of mine()): a
+// sequence of four numbers per code edge, compared lexicographically, a prefix first.
+struct OrderKey {
+    int n = 0;
+    int v[4 * GS_MAXE];
+    bool operator<(const OrderKey& o) const {
+        const int k = std::min(n, o.n);
+        for (int i = 0; i < k; ++i)
+            if (v[i] != o.v[i]) return v[i] < o.v[i];
+        return n < o.n;
+    }
+};
+
 struct Report {
-    std::vector<int> code5;  // original labels, reference placeholder form
-    std::vector<long long> order;
+    int m;
+    int code5[5 * GS_MAXE];  // original labels, reference placeholder form
+    OrderKey order;
     int support;
+};
+
+// Pinned host memory for the small transfers between host syncs.  A copy from or to
+// pageable memory is staged by the runtime and costs tens of microseconds of latency
+// each; through pinned memory the copies of one step are queued on the stream and
+// complete together at the next sync.  Blocks are kept for reuse; everything taken is
+// released by reset() once the stream has been synchronised.
+class HostStage {
+   public:
+    static constexpr size_t kBlock = 1 << 20;
+    static constexpr size_t kMaxStaged = 16 << 20;  // larger copies go direct
+
+    HostStage() = default;
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage() {
+        for (auto& b : blocks_) (void)hipHostFree(b.p);
+    }
+    void* take(size_t bytes) {
+        bytes = (bytes + 255) & ~size_t(255);
+        while (cur_ < blocks_.size() && blocks_[cur_].used + bytes > blocks_[cur_].cap) ++cur_;
+        if (cur_ == blocks_.size()) {
+            Block b{nullptr, std::max(bytes, kBlock), 0};
+            const hipError_t e = hipHostMalloc(&b.p, b.cap, hipHostMallocDefault);
+            if (e != hipSuccess)
+                throw GsError(GS_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+            blocks_.push_back(b);
+        }
+        Block& b = blocks_[cur_];
+        void* p = static_cast<char*>(b.p) + b.used;
+        b.used += bytes;
+        return p;
+    }
+    void reset() {
+        for (auto& b : blocks_) b.used = 0;
+        cur_ = 0;
+    }
+
+   private:
+    struct Block {
+        void* p;
+        size_t cap, used;
+    };
+    std::vector<Block> blocks_;
+    size_t cur_ = 0;
+};
+
+// A device-to-host copy into pinned memory, copied to its destination at the next sync.
+struct PendingD2H {
+    void* dst;
+    const void* staged;
+    size_t bytes;
 };
 
 }  // namespace
@@ -207,6 +273,8 @@ struct gs_ctx {
     std::vector<int> vl_vals, el_vals;  // rank -> original label
     int VL = 0, EL = 0;
     DevPool pool;
+    HostStage stage;
+    std::vector<PendingD2H> pending_d2h;
     DBuf d_offsets, d_src, d_adj, d_vlab;
     DevGraph dg{};
     DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
@@ -250,15 +318,41 @@ struct gs_ctx {
 
 namespace {
 
-void sync(gs_ctx* c) { HIPCHECK(hipStreamSynchronize(c->stream)); }
+// Waits for the stream; device-to-host copies issued since the last sync are then in
+// their destinations.
+void sync(gs_ctx* c) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (const PendingD2H& p : c->pending_d2h) std::memcpy(p.dst, p.staged, p.bytes);
+    c->pending_d2h.clear();
+    c->stage.reset();
+}
 
+// Host-to-device copy: src may be released as soon as this returns.
 template <class T>
 void h2d(gs_ctx* c, T* dst, const T* src, size_t count) {
-    if (count) HIPCHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = count * sizeof(T);
+    if (!bytes) return;
+    if (bytes <= HostStage::kMaxStaged) {
+        void* p = c->stage.take(bytes);
+        std::memcpy(p, src, bytes);
+        HIPCHECK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));  // src belongs to the caller
+    }
 }
+// Device-to-host copy: dst holds the data after the next sync(c).
 template <class T>
 void d2h(gs_ctx* c, T* dst, const T* src, size_t count) {
-    if (count) HIPCHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    const size_t bytes = count * sizeof(T);
+    if (!bytes) return;
+    if (bytes <= HostStage::kMaxStaged) {
+        void* p = c->stage.take(bytes);
+        HIPCHECK(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        c->pending_d2h.push_back(PendingD2H{dst, p, bytes});
+    } else {
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
 }
 
 int vl_rank(const gs_ctx* c, int label) {
@@ -338,32 +432,25 @@ HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
 // Order of the reference's report lines: seeds by (fl, el, tl); within a node, backward
 // children (ascending v_i, el) before forward children (descending v_i, ascending el, tl);
 // a pattern precedes its descendants (gspan.cu:1142-1168, 1185).
-std::vector<long long> order_key(const HPat& h) {
-    std::vector<long long> k;
-    k.reserve(4 * h.m);
-    k.push_back(h.fl0);
-    k.push_back(h.el[0]);
-    k.push_back(h.tl[0]);
-    k.push_back(0);
+OrderKey order_key(const HPat& h) {
+    OrderKey k;
+    auto put = [&](int a, int b, int c, int d) {
+        k.v[k.n++] = a;
+        k.v[k.n++] = b;
+        k.v[k.n++] = c;
+        k.v[k.n++] = d;
+    };
+    put(h.fl0, h.el[0], h.tl[0], 0);
     for (int j = 1; j < h.m; ++j) {
-        if (h.f[j] > h.t[j]) {
-            k.push_back(0);
-            k.push_back(h.t[j]);
-            k.push_back(h.el[j]);
-            k.push_back(0);
-        } else {
-            k.push_back(1);
-            k.push_back(-h.f[j]);
-            k.push_back(h.el[j]);
-            k.push_back(h.tl[j]);
-        }
+        if (h.f[j] > h.t[j]) put(0, h.t[j], h.el[j], 0);
+        else put(1, -h.f[j], h.el[j], h.tl[j]);
     }
     return k;
 }
 
 void add_report(gs_ctx* c, const HPat& h) {
     Report r;
-    r.code5.resize(5 * h.m);
+    r.m = h.m;
     for (int j = 0; j < h.m; ++j) {
         r.code5[5 * j + 0] = h.f[j];
         r.code5[5 * j + 1] = h.t[j];
@@ -1066,7 +1153,7 @@ void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& la
         // one parent at a time, in the order the reference visits them
         std::vector<size_t> idx(pats.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-        std::vector<std::vector<long long>> keys(pats.size());
+        std::vector<OrderKey> keys(pats.size());
         for (size_t i = 0; i < pats.size(); ++i) keys[i] = order_key(pats[i]);
         std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return keys[a] < keys[b]; });
         for (size_t i : idx) {
@@ -1176,18 +1263,30 @@ void set_error(gs_ctx* c, const std::string& msg) {
     if (c) c->err = msg;
 }
 
+// After a failed call: the destinations of its queued device-to-host copies may be gone,
+// so the copies are waited for and dropped.
+void drop_pending(gs_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->pending_d2h.clear();
+    c->stage.reset();
+}
+
 int guard(gs_ctx* c, const std::function<void()>& fn) {
     try {
         fn();
         return GS_OK;
     } catch (const GsError& e) {
         set_error(c, e.what());
+        drop_pending(c);
         return e.code;
     } catch (const std::bad_alloc&) {
         set_error(c, "host allocation failed");
+        drop_pending(c);
         return GS_ERR_INTERNAL;
     } catch (const std::exception& e) {
         set_error(c, e.what());
+        drop_pending(c);
         return GS_ERR_INTERNAL;
     }
 }
@@ -1398,7 +1497,7 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->st.peak_device_bytes = c->pool.peak();
         if (cb) {
             for (const Report& r : c->reports)
-                if (cb(r.code5.data(), (int)r.code5.size() / 5, r.support, user) != 0)
+                if (cb(r.code5, r.m, r.support, user) != 0)
                     throw GsError(GS_ERR_CALLBACK, "report callback stopped mining");
         }
     });
@@ -1456,7 +1555,7 @@ int gs_extend(gs_ctx* c, const int* code5, int m, const int* const* layers,
         // reference child order: backward ascending (v_i, el), forward descending v_i
         std::vector<size_t> idx(so.children.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
-        std::vector<std::vector<long long>> keys(idx.size());
+        std::vector<OrderKey> keys(idx.size());
         for (size_t i = 0; i < idx.size(); ++i) keys[i] = order_key(so.children[i].pat);
         std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return keys[a] < keys[b]; });
         long long rows_total = 0;
@@ -1711,7 +1810,7 @@ int gs_tuner_report(const gs_ctx* c, char* buf, int cap, int* len) {
 int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
     if (!c || !count || !total_edges) return GS_ERR_INVALID;
     long long e = 0;
-    for (const Report& r : c->reports) e += (long long)r.code5.size() / 5;
+    for (const Report& r : c->reports) e += r.m;
     *count = (long long)c->reports.size();
     *total_edges = e;
     return GS_OK;
@@ -1722,9 +1821,9 @@ int gs_result_copy(const gs_ctx* c, int* codes5, int* lens, int* supports) {
     long long at = 0;
     for (size_t i = 0; i < c->reports.size(); ++i) {
         const Report& r = c->reports[i];
-        std::memcpy(codes5 + at, r.code5.data(), sizeof(int) * r.code5.size());
-        at += (long long)r.code5.size();
-        lens[i] = (int)r.code5.size() / 5;
+        std::memcpy(codes5 + at, r.code5, sizeof(int) * 5 * r.m);
+        at += 5LL * r.m;
+        lens[i] = r.m;
         supports[i] = r.support;
     }
     return GS_OK;
