@@ -155,6 +155,11 @@ int items_blocks_per_cu(bool fill, int tile);
 
 hipError_t compact_active(const int* key_count, long long K, int* active_keys,
                           int* active_counts, int* n_active, hipStream_t s);
+// The same in ascending key order (flags / pos: K and K+1 elements; scan_tmp as for
+// scan_exclusive over K); *n_active is written, not added to.
+hipError_t compact_active_ordered(const int* key_count, long long K, int* active_keys,
+                                  int* active_counts, int* n_active, int* flags, long long* pos,
+                                  long long* scan_tmp, long long scan_tmp_elems, hipStream_t s);
 hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_keys,
                  const int* n_active, int max_active, int* flags, int* err, hipStream_t s);
 hipError_t canon_codes(const int* codes, int ncodes, int* flags, int* err, hipStream_t s);

@@ -639,6 +639,29 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
 
 // ------------------------------------------------------------------ keys
 
+// Ordered compaction of the keys with a non-zero count: flags, exclusive scan, scatter.
+__global__ __launch_bounds__(GS_BLOCK) void k_active_flags(const int* key_count, long long K,
+                                                           int* flags) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += stride)
+        flags[i] = key_count[i] > 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(GS_BLOCK) void k_active_scatter(const int* key_count,
+                                                             const long long* pos, long long K,
+                                                             int* active, int* active_counts,
+                                                             int* n_active) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += stride) {
+        const int c = key_count[i];
+        if (c > 0) {
+            active[pos[i]] = (int)i;
+            active_counts[pos[i]] = c;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n_active = (int)pos[K];
+}
+
 __global__ __launch_bounds__(GS_BLOCK) void k_compact_active(const int* key_count, long long K,
                                                              int* active, int* active_counts,
                                                              int* n_active) {
@@ -1212,6 +1235,22 @@ hipError_t compact_active(const int* key_count, long long K, int* active_keys,
                           int* active_counts, int* n_active, hipStream_t s) {
     hipLaunchKernelGGL(k_compact_active, dim3(grid_for(K, GS_BLOCK, 4096)), dim3(GS_BLOCK), 0, s,
                        key_count, K, active_keys, active_counts, n_active);
+    return hipGetLastError();
+}
+
+hipError_t compact_active_ordered(const int* key_count, long long K, int* active_keys,
+                                  int* active_counts, int* n_active, int* flags, long long* pos,
+                                  long long* scan_tmp, long long scan_tmp_elems, hipStream_t s) {
+    if (K <= 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_active, 0, 1LL);
+        return hipGetLastError();
+    }
+    const unsigned nb = (unsigned)grid_for(K, GS_BLOCK, 1 << 20);
+    hipLaunchKernelGGL(k_active_flags, dim3(nb), dim3(GS_BLOCK), 0, s, key_count, K, flags);
+    hipError_t e = scan_exclusive(flags, pos, K, scan_tmp, scan_tmp_elems, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_active_scatter, dim3(nb), dim3(GS_BLOCK), 0, s, key_count, pos, K,
+                       active_keys, active_counts, n_active);
     return hipGetLastError();
 }
 

@@ -615,6 +615,17 @@ void resolve_timing(gs_ctx* c) {
     c->ev_used = 0;
 }
 
+// Keys with a non-zero count, in ascending key order, into active / acnt; *n_active.
+void compact_ordered(gs_ctx* c, const int* key_count, long long K, int* active, int* acnt) {
+    DevPool* P = &c->pool;
+    const long long te = gsk::scan_tmp_elems(K);
+    DBuf flags(P, sizeof(int) * std::max(K, 1LL)), pos(P, sizeof(long long) * (K + 1)),
+        tmp(P, sizeof(long long) * te);
+    HIPCHECK(gsk::compact_active_ordered(key_count, K, active, acnt, c->n_active(),
+                                         flags.as<int>(), pos.as<long long>(),
+                                         tmp.as<long long>(), te, c->stream));
+}
+
 struct ChildInfo {
     HPat pat;
     long long count;
@@ -829,8 +840,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K),
                 d_flags(P, sizeof(int) * K);
             HIPCHECK(gsk::fill_i32(c->n_active(), 0, 2, c->stream));
-            HIPCHECK(gsk::compact_active(d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>(),
-                                         c->n_active(), c->stream));
+            compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
             HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(),
                                 (int)K, d_flags.as<int>(), c->err_flag(), c->stream));
             int na_err_ov[3] = {0, 0, 0};
@@ -851,10 +861,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 d2h(c, acnt.data(), d_acnt.as<int>(), A);
                 d2h(c, aflag.data(), d_flags.as<int>(), A);
                 sync(c);
-                std::vector<int> order(A);
+                std::vector<int> order(A);  // keys arrive in key order
                 for (int i = 0; i < A; ++i) order[i] = i;
-                std::sort(order.begin(), order.end(),
-                          [&](int a, int b) { return akeys[a] < akeys[b]; });
                 std::vector<SetDesc> live_sets;
                 long long emitted = 0;
                 for (int oi = 0; oi < A; ++oi) {
@@ -866,7 +874,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                                  }) -
                                 pd.begin()) - 1;
                     ChildInfo ci;
-                    ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
+                    if (aflag[a] || all_live)  // other children only count as candidates
+                        ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
                     ci.count = acnt[a];
                     ci.canonical = aflag[a];
                     ci.support = 0;
@@ -958,8 +967,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     // ---- candidates and their canonicality
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
     HIPCHECK(gsk::fill_i32(c->n_active(), 0, 2, c->stream));  // n_active and err
-    HIPCHECK(gsk::compact_active(d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>(),
-                                 c->n_active(), c->stream));
+    compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
     HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
                         d_flags.as<int>(), c->err_flag(), c->stream));
     int na_err[2] = {0, 0};
@@ -977,10 +985,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     d2h(c, acnt.data(), d_acnt.as<int>(), A);
     d2h(c, aflag.data(), d_flags.as<int>(), A);
     sync(c);
-    // key order = (parent, slot) order; compact_active leaves it unordered
+    // keys arrive in key order = (parent, slot) order
     std::vector<int> order(A);
     for (int i = 0; i < A; ++i) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return akeys[a] < akeys[b]; });
 
     // ---- plan child rows and sets of the live keys
     std::vector<int> live_keys, live_child;
@@ -995,7 +1002,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      [](int k, const PatDev& d) { return k < d.key_base; }) -
                     pd.begin()) - 1;
         ChildInfo ci;
-        ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
+        if (aflag[a] || all_live)  // other children only count as candidates
+            ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
         ci.count = acnt[a];
         ci.canonical = aflag[a];
         ci.support = 0;
@@ -1177,8 +1185,7 @@ void mine_seeds(gs_ctx* c) {
     HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->total64()), 0, 2, c->stream));
     HIPCHECK(gsk::seed_count(c->dg, c->EL, c->VL, d_kc.as<int>(), c->total64(), c->stream));
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K);
-    HIPCHECK(gsk::compact_active(d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>(),
-                                 c->n_active(), c->stream));
+    compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
     int A = 0;
     d2h(c, &A, c->n_active(), 1);
     sync(c);
@@ -1186,9 +1193,8 @@ void mine_seeds(gs_ctx* c) {
     d2h(c, akeys.data(), d_active.as<int>(), A);
     d2h(c, acnt.data(), d_acnt.as<int>(), A);
     sync(c);
-    std::vector<int> order(A);
+    std::vector<int> order(A);  // keys arrive in key order
     for (int i = 0; i < A; ++i) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return akeys[a] < akeys[b]; });
     const bool rows = c->k > 1;
     std::vector<HPat> seeds;
     std::vector<int> live, cursor;
