@@ -35,6 +35,9 @@ WORKLOADS = {
                   min_sup=200, k=4),
 }
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per fill launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this bench,
+# scaled by a 1 GiB copy's readings (profiles/k_items_er10k_k4.md); None where not measured
+FILL_TRAFFIC_BYTES = {"er10k": 288_776_563}
 
 
 def log(msg: str) -> None:
@@ -162,7 +165,10 @@ def main() -> int:
             st["count_launches"]
     achieved = (kbytes / klaunch) / (kms / klaunch * 1e-3) / 1e9 if kms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": (FILL_TRAFFIC_BYTES.get(args.workload)
+                            if kname.startswith("k_items<true>") and not args.k and not args.min_sup
+                            else None),
                 "kernel": kname, "launches_per_step": klaunch / steps,
                 "avg_launch_ms": kms / max(klaunch, 1),
                 "alg_bytes_per_launch": kbytes / max(klaunch, 1)}
