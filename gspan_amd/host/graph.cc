@@ -152,37 +152,5 @@ void CSRGraph::save(const char* filename) {
     std::fwrite(edges, sizeof(Edge), nr_edges, f);
 }
 
-// graph.cc:216-266: keep the vertices whose label occurs at least min_sup times,
-// renumbered in ascending order, with the edges between kept vertices
-void prune_graph(CSRGraph& pruned, const CSRGraph& g, int min_sup) {
-    std::map<int, int> freq;
-    for (int v = 0; v < g.nr_nodes; ++v) freq[g.vlabels[v]]++;
-    std::vector<int> renum(g.nr_nodes, -1);
-    int kept = 0;
-    for (int v = 0; v < g.nr_nodes; ++v)
-        if (freq[g.vlabels[v]] >= min_sup) renum[v] = kept++;
-    std::vector<Edge> out;
-    delete[] pruned.offsets;
-    delete[] pruned.vlabels;
-    delete[] pruned.edges;
-    pruned.nr_nodes = kept;
-    pruned.offsets = new int[kept + 1];
-    pruned.vlabels = new int[kept > 0 ? kept : 1];
-    pruned.offsets[0] = 0;
-    for (int v = 0; v < g.nr_nodes; ++v) {
-        if (renum[v] < 0) continue;
-        const int nv = renum[v];
-        pruned.vlabels[nv] = g.vlabels[v];
-        for (int i = g.offsets[v]; i < g.offsets[v + 1]; ++i) {
-            const Edge& e = g.edges[i];
-            if (renum[e.to] < 0) continue;
-            out.push_back(Edge{nv, renum[e.to], e.label});
-        }
-        pruned.offsets[nv + 1] = (int)out.size();
-    }
-    pruned.nr_edges = (int)out.size();
-    pruned.edges = new Edge[out.empty() ? 1 : out.size()];
-    std::copy(out.begin(), out.end(), pruned.edges);
-    std::printf("|V|: %d -> %d |E|: %d -> %d\n", g.nr_nodes, pruned.nr_nodes, g.nr_edges,
-                pruned.nr_edges);
-}
+// prune_graph (graph.cc:216-266) runs on the GPU: see gspan.cc.
+
