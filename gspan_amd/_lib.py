@@ -69,6 +69,7 @@ class GsStats(ctypes.Structure):
         ("peak_device_bytes", ctypes.c_ulonglong),
         ("predicted_steps", ctypes.c_longlong),
         ("fallbacks", ctypes.c_longlong),
+        ("spilled_rows", ctypes.c_longlong),
     ]
 
     def as_dict(self) -> dict:

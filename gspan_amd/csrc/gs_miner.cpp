@@ -634,9 +634,73 @@ struct ChildInfo {
     long long row_off;  // in the child row array (-1 when not written)
 };
 
+// Rows of one DFS level: in device memory, or -- when they do not fit the device
+// budget -- spilled to pinned host memory that the kernels reach through its device
+// mapping (slower, over the host link, but the level can be mined).
+class LevelRows {
+   public:
+    LevelRows() = default;
+    LevelRows(const LevelRows&) = delete;
+    LevelRows& operator=(const LevelRows&) = delete;
+    LevelRows(LevelRows&& o) noexcept { *this = std::move(o); }
+    LevelRows& operator=(LevelRows&& o) noexcept {
+        if (this != &o) {
+            release();
+            dev_ = std::move(o.dev_);
+            host_ = o.host_;
+            ptr_ = o.ptr_;
+            o.host_ = nullptr;
+            o.ptr_ = nullptr;
+        }
+        return *this;
+    }
+    ~LevelRows() { release(); }
+    void on_device(DevPool* P, long long rows) {
+        release();
+        dev_ = DBuf(P, sizeof(int2) * std::max(rows, 1LL));
+        ptr_ = dev_.as<int2>();
+    }
+    void on_host(long long rows) {
+        release();
+        hipError_t e = hipHostMalloc(&host_, sizeof(int2) * std::max(rows, 1LL),
+                                     hipHostMallocDefault);
+        if (e != hipSuccess)
+            throw GsError(GS_ERR_BUDGET, std::string("spilling level rows to host memory: ") +
+                                             hipGetErrorString(e));
+        void* d = nullptr;
+        e = hipHostGetDevicePointer(&d, host_, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(host_);
+            host_ = nullptr;
+            throw GsError(GS_ERR_HIP, std::string("hipHostGetDevicePointer: ") +
+                                          hipGetErrorString(e));
+        }
+        ptr_ = static_cast<int2*>(d);
+    }
+    void adopt(DBuf&& d) {
+        release();
+        dev_ = std::move(d);
+        ptr_ = dev_.as<int2>();
+    }
+    int2* ptr() const { return ptr_; }
+    bool spilled() const { return host_ != nullptr; }
+
+   private:
+    // every kernel that read or wrote the rows finished before the step's last sync
+    void release() {
+        if (host_) (void)hipHostFree(host_);
+        host_ = nullptr;
+        dev_ = DBuf();
+        ptr_ = nullptr;
+    }
+    DBuf dev_;
+    void* host_ = nullptr;
+    int2* ptr_ = nullptr;
+};
+
 struct StepOut {
     std::vector<ChildInfo> children;  // every candidate child (active key), key order
-    DBuf rows;                        // child rows of the written keys
+    LevelRows rows;                   // child rows of the written keys
     long long emitted = 0;
 };
 
@@ -911,7 +975,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                  (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
                     c->st.alg_bytes_fill += bytes;
                 }
-                out.rows = std::move(rows);
+                out.rows = LevelRows();
+                out.rows.adopt(std::move(rows));
                 c->st.predicted_steps++;
                 c->st.batches++;
                 return STEP_OK;
@@ -1048,10 +1113,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         c->pool.in_use() + need_of() > c->budget)
         plan_sets(false);  // bytemaps do not fit: bitmaps take an eighth of the memory
     const size_t need = need_of();
+    bool spill = false;
     if (c->pool.in_use() + need > c->budget) {
         if (B > 1) return STEP_SPLIT;
-        throw GsError(GS_ERR_BUDGET, "children of one pattern exceed the device budget (" +
-                                         std::to_string(need) + " bytes)");
+        // one pattern: its child rows go to pinned host memory; its sets stay on the
+        // device (a set is at most a byte per graph vertex; an allocation the device
+        // cannot hold fails with GS_ERR_HIP)
+        spill = write_rows && rows_total > 0;
     }
     if (NL == 0) {
         c->st.batches++;
@@ -1067,7 +1135,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         DBuf d_slot(P, sizeof(int) * K);
         DBuf d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(int) * NL);
         d_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
-        if (write_rows) out.rows = DBuf(P, sizeof(int2) * std::max<long long>(rows_total, 1));
+        if (write_rows) {
+            if (spill) {
+                out.rows.on_host(rows_total);
+                c->st.spilled_rows += rows_total;
+            } else {
+                out.rows.on_device(P, rows_total);
+            }
+        }
         HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
         h2d(c, d_live.as<int>(), live_keys.data(), NL);
         h2d(c, d_cursor.as<int>(), cursor.data(), NL);
@@ -1081,7 +1156,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, nullptr,
                                      d_slot.as<int>(), d_cursor.as<int>(), nullptr,
-                                     write_rows ? out.rows.as<int2>() : nullptr,
+                                     write_rows ? out.rows.ptr() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
                                      c->overflow_flag(), c->tile_counters(), NL, c->stream));
             time_end(1, &Wc_host[ci]);
@@ -1149,7 +1224,7 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers
         if (!last) next.push_back(ch.pat);
     }
     if (!last && !next.empty()) {
-        layers.push_back(so.rows.as<int2>());
+        layers.push_back(so.rows.ptr());
         mine_level(c, next, layers);
         layers.pop_back();
     }
@@ -1235,8 +1310,15 @@ void mine_seeds(gs_ctx* c) {
     DBuf d_sets(P, sizeof(SetDesc) * NL);
     DBuf d_pool(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
     DBuf d_counts(P, sizeof(int) * NL * GS_MAXV);
-    DBuf d_rows;
-    if (rows) d_rows = DBuf(P, sizeof(int2) * std::max<long long>(rows_total, 1));
+    LevelRows d_rows;
+    if (rows) {
+        if (c->pool.in_use() + (size_t)rows_total * sizeof(int2) > c->budget) {
+            d_rows.on_host(rows_total);  // spilled to pinned host memory
+            c->st.spilled_rows += rows_total;
+        } else {
+            d_rows.on_device(P, rows_total);
+        }
+    }
     HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
     h2d(c, d_live.as<int>(), live.data(), NL);
     h2d(c, d_cursor.as<int>(), cursor.data(), NL);
@@ -1244,7 +1326,7 @@ void mine_seeds(gs_ctx* c) {
     HIPCHECK(gsk::scatter_slots(d_slot.as<int>(), d_live.as<int>(), NL, c->stream));
     HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
     HIPCHECK(gsk::seed_fill(c->dg, c->EL, c->VL, d_slot.as<int>(), d_cursor.as<int>(),
-                            rows ? d_rows.as<int2>() : nullptr, d_sets.as<SetDesc>(),
+                            rows ? d_rows.ptr() : nullptr, d_sets.as<SetDesc>(),
                             d_pool.as<uint32_t>(), c->stream));
     HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(), d_counts.as<int>(),
                              c->stream));
@@ -1260,7 +1342,7 @@ void mine_seeds(gs_ctx* c) {
         freq.push_back(seeds[i]);
     }
     if (rows && !freq.empty()) {
-        std::vector<const int2*> layers{d_rows.as<int2>()};
+        std::vector<const int2*> layers{d_rows.ptr()};
         mine_level(c, freq, layers);
     }
 }
@@ -1567,7 +1649,7 @@ int gs_extend(gs_ctx* c, const int* code5, int m, const int* const* layers,
         long long rows_total = 0;
         for (auto& ch : so.children) rows_total += ch.count;
         std::vector<int2> rows((size_t)rows_total);
-        d2h(c, rows.data(), so.rows.as<int2>(), (size_t)rows_total);
+        d2h(c, rows.data(), so.rows.ptr(), (size_t)rows_total);
         sync(c);
         for (size_t i : idx) {
             const ChildInfo& ch = so.children[i];

@@ -91,6 +91,7 @@ typedef struct gs_stats {
     unsigned long long peak_device_bytes;
     long long predicted_steps; /* steps run in one pass from predicted sizes               */
     long long fallbacks;       /* predicted steps redone exactly after a misprediction     */
+    long long spilled_rows;    /* level rows kept in pinned host memory (budget exceeded)  */
 } gs_stats;
 
 /* Receives one frequent pattern.  Return 0 to continue, non-zero to stop mining. */

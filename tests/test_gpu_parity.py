@@ -187,6 +187,20 @@ def test_tile_shapes_and_tuner(oracle_mod):
         assert b["best"] == [best[0], best[1]], (name, b)
 
 
+def test_spill_rows_to_host_memory(oracle_mod):
+    """With a device budget too small for any level, every level's rows live in pinned
+    host memory that the kernels reach through its device mapping; results unchanged."""
+    g = synthetic("er", n=2000, m=8000, n_vlabels=3, n_elabels=2, seed=9)
+    _announce("mine ER 2k k=3 ms=15 with spilled level rows")
+    want, st = oracle_mod.fsm(g, 3, 15)
+    for mode in ("batched", "plain"):
+        with Miner(g, mode=mode, budget_bytes=1) as mn:
+            assert mn.mine(3, 15) == want, mode
+            s = mn.stats()
+            assert s["extended"] == st["extended"]
+            assert s["spilled_rows"] > 0
+
+
 def test_prune_graph_matches_oracle(oracle_mod):
     """gs_prune_graph (graph.cc:216-266 on the device) against the oracle's restatement,
     at every threshold that changes the kept set; then mining the pruned graph must give
