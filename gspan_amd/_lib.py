@@ -128,6 +128,10 @@ def lib():
     L.gs_mlp_gradients.argtypes = [vp, P(c_f), P(c_f), c_ll, P(c_f), P(c_f)]
     L.gs_prune_graph.argtypes = [vp, c_int, P(c_int), P(c_int), P(c_int), P(c_int), P(c_int)]
     L.gs_tuner_report.argtypes = [vp, ctypes.c_char_p, c_int, P(c_int)]
+    L.gs_text_parse.argtypes = [ctypes.c_char_p, c_ll, c_int, P(vp), P(c_int), P(c_int), P(c_ll)]
+    L.gs_text_copy.argtypes = [vp, P(c_int), P(c_int), P(c_int)]
+    L.gs_text_free.argtypes = [vp]
+    L.gs_text_free.restype = None
     L.gs_result_count.argtypes = [vp, P(c_ll), P(c_ll)]
     L.gs_result_copy.argtypes = [vp, P(c_int), P(c_int), P(c_int)]
     L.gs_get_stats.argtypes = [vp, P(GsStats)]
@@ -139,7 +143,8 @@ def lib():
                  "gs_is_canonical_batch", "gs_result_count", "gs_result_copy", "gs_get_stats",
                  "gs_reset_stats", "gs_destroy", "gs_predictor_log", "gs_predictor_samples",
                  "gs_predictor_train", "gs_predictor_get_weights", "gs_predictor_set_weights",
-                 "gs_mlp_forward", "gs_mlp_gradients", "gs_prune_graph", "gs_tuner_report"):
+                 "gs_mlp_forward", "gs_mlp_gradients", "gs_prune_graph", "gs_tuner_report",
+                 "gs_text_parse", "gs_text_copy"):
         getattr(L, name).restype = c_int
     _lib = L
     return L

@@ -175,6 +175,41 @@ def from_txt(path: str | os.PathLike, verbose: bool = False) -> CSRGraph:
     return csr_from_undirected(n, u, v, np.array(el, dtype=np.int64), labels)
 
 
+def from_txt_gpu(path: str | os.PathLike, device: int = 0, verbose: bool = False) -> CSRGraph:
+    """Graph::from_txt + CSRGraph(Graph&) (graph.cc:38-112) on the GPU (gs_text_parse).
+
+    The file must hold one record per line, as gSpan files do; a line the reference would
+    read across a line boundary raises ValueError naming the line.
+    """
+    import ctypes
+
+    from . import _lib as L
+    with open(path, "rb") as f:
+        data = f.read()
+    lib = L.lib()
+    h = ctypes.c_void_p()
+    n, nnz, bad = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(-1)
+    rc = lib.gs_text_parse(data, len(data), device, ctypes.byref(h), ctypes.byref(n),
+                           ctypes.byref(nnz), ctypes.byref(bad))
+    if rc == L.GS_ERR_INVALID and bad.value >= 0:
+        raise ValueError(f"{path}: line {bad.value + 1} is not one whole record")
+    if rc != L.GS_OK:
+        raise L.GsError(rc, "gs_text_parse failed")
+    try:
+        off = np.zeros(n.value + 1, np.int32)
+        vl = np.zeros(max(n.value, 1), np.int32)
+        ed = np.zeros((max(nnz.value, 1), 3), np.int32)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        rc = lib.gs_text_copy(h, ip(off), ip(vl), ip(ed))
+        if rc != L.GS_OK:
+            raise L.GsError(rc, "gs_text_copy failed")
+    finally:
+        lib.gs_text_free(h)
+    if verbose:
+        print(f"|V| = {n.value} |E| = {nnz.value // 2}")
+    return CSRGraph(off, vl[:n.value], ed[:nnz.value])
+
+
 def to_txt(g: CSRGraph, path: str | os.PathLike) -> None:
     """Writes the text format read by graph.cc:38-64 (each undirected edge once)."""
     e = g.edges

@@ -162,6 +162,19 @@ int gs_prune_graph(gs_ctx* ctx, int min_sup, int* n_out, int* nnz_out, int* offs
  * receives the full length.  No counterpart in the reference (its extension is OpenMP). */
 int gs_tuner_report(const gs_ctx* ctx, char* buf, int cap, int* len);
 
+/* Graph::from_txt (graph.cc:38-63) followed by CSRGraph(Graph&) (graph.cc:66-112), on the
+ * GPU: text holds the bytes of a graph file (len bytes), one record per line ("t # id",
+ * "v id label", "e u v label"; lines of other ops are skipped).  A line the reference would
+ * read across a line boundary (wrong token count, non-integer argument, an op letter after
+ * an unknown op) is rejected: GS_ERR_INVALID with its 0-based index in *bad_line.  On
+ * success *out holds the CSR arrays on the device and *n_out / *nnz_out their sizes. */
+typedef struct gs_text_graph gs_text_graph;
+int gs_text_parse(const char* text, long long len, int device, gs_text_graph** out, int* n_out,
+                  int* nnz_out, long long* bad_line);
+/* Copies the parsed graph: offsets n+1, vlabels n, edges3 3*nnz ints (from, to, label). */
+int gs_text_copy(const gs_text_graph* g, int* offsets, int* vlabels, int* edges3);
+void gs_text_free(gs_text_graph* g);
+
 /* Results of the last gs_mine, in report order, as flat arrays (for callers that want
  * them in bulk instead of through a callback; gs_mine may then be called with cb = NULL).
  * gs_result_count: number of patterns and total code edges.  gs_result_copy fills
