@@ -90,6 +90,44 @@ def cpu_baseline(g, w: dict, k: int, target_s: float) -> dict:
                       f"code {tuple(e5)} ({emitted} child rows)", "seconds": dt}
 
 
+def ingest_bench(args) -> int:
+    """Text -> CSR of the workload graph: GPU (gs_text_parse) vs the reference's convert."""
+    import subprocess
+    import tempfile
+    from gspan_amd import graph as G
+    w = dict(WORKLOADS[args.workload])
+    g = make_graph(w)
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "g.txt")
+        G.to_txt(g, p)
+        size = os.path.getsize(p)
+        log(f"text file {size / 1e6:.1f} MB, {g.nr_edges // 2} edges")
+        G.from_txt_gpu(p)  # warmup: context, code objects
+        times = []
+        for _ in range(max(args.steps, 1)):
+            t = time.perf_counter()
+            gg = G.from_txt_gpu(p)
+            times.append(time.perf_counter() - t)
+        assert gg.nr_nodes == g.nr_nodes and gg.nr_edges == g.nr_edges
+        np_ok = bool((gg.edges == g.edges).all() and (gg.offsets == g.offsets).all())
+        ref = None
+        conv = os.path.join(ROOT, "oracle", "_ref", "convert")
+        if os.path.exists(conv):
+            t = time.perf_counter()
+            subprocess.run([conv, p, os.path.join(td, "g.bin")], check=True,
+                           stdout=subprocess.DEVNULL)
+            ref_s = time.perf_counter() - t
+            rg = G.CSRGraph.load(os.path.join(td, "g.bin"))
+            ref = {"seconds": ref_s, "MB_per_s": size / ref_s / 1e6, "cores": 1,
+                   "kind": "reference", "same_csr": bool((rg.edges == gg.edges).all())}
+    best = min(times)
+    print(json.dumps({"metric": "text ingestion MB/s (file read, parse, CSR build, copy back)",
+                      "workload": args.workload, "bytes": size, "gpu_seconds": best,
+                      "gpu_MB_per_s": size / best / 1e6, "same_as_generator": np_ok,
+                      "cpu_baseline": ref}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +143,10 @@ def main() -> int:
     ap.add_argument("--no-predicted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--ingest", action="store_true",
+                    help="only time text ingestion (Graph::from_txt + CSRGraph(Graph&)) of the "
+                         "workload's graph on the GPU, beside the reference's own convert "
+                         "(oracle/_ref/convert, when built)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a plain copy kernel of "
                          "known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -126,6 +168,9 @@ def main() -> int:
         return 2
 
     from gspan_amd.miner import Miner
+
+    if args.ingest:
+        return ingest_bench(args)
 
     w = dict(WORKLOADS[args.workload])
     k = args.k or w["k"]
