@@ -85,3 +85,16 @@ def test_text_ingest_shuffled_records(tmp_path):
     p.write_text("\n".join(lines) + "\n")
     _announce("ingest shuffled records")
     same(G.from_txt_gpu(p), G.from_txt(p))
+
+
+def test_text_ingest_matches_reference_golden(tmp_path):
+    """Byte-identical to the reference's own convert on the committed fixtures."""
+    import glob
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    cases = sorted(glob.glob(os.path.join(gold, "convert_*.txt")))
+    assert cases
+    for txt in cases:
+        _announce(f"ingest golden {os.path.basename(txt)}")
+        G.from_txt_gpu(txt).save(tmp_path / "gpu.bin")
+        assert (tmp_path / "gpu.bin").read_bytes() == open(txt[:-4] + ".bin", "rb").read()

@@ -234,3 +234,24 @@ def test_prune_graph_matches_oracle(oracle_mod):
     assert pruned.nr_nodes < g.nr_nodes
     with Miner(pruned) as mp:
         assert mp.mine(3, ms) == want
+
+
+def test_prune_graph_matches_reference_golden():
+    """gs_prune_graph against the reference's own prune_graph outputs (tests/golden)."""
+    import glob
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    g = CSRGraph.load(os.path.join(gold, "prune_in.bin"))
+    cases = []
+    for p in glob.glob(os.path.join(gold, "prune_*.bin")):
+        tag = os.path.basename(p)[len("prune_"):-len(".bin")]
+        if tag.isdigit():
+            cases.append((int(tag), p))
+    assert len(cases) >= 4
+    _announce(f"prune golden, {len(cases)} thresholds")
+    with Miner(g) as mn:
+        for ms, p in sorted(cases):
+            want, got = CSRGraph.load(p), mn.prune_graph(ms)
+            np.testing.assert_array_equal(got.offsets, want.offsets)
+            np.testing.assert_array_equal(got.vlabels, want.vlabels)
+            np.testing.assert_array_equal(got.edges, want.edges)
