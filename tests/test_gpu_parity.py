@@ -121,6 +121,22 @@ def test_mining_er_config_k3(oracle_mod):
     assert format_report(got) == oracle_mod.format_report(want)
 
 
+def test_mining_er_bench_config_all_modes(oracle_mod):
+    """The bench workload itself (ER 10,000 / 40,000, 5 labels, min_sup 200, k = 4): the
+    report text and the extended-embedding total equal the oracle's in every mode."""
+    g = synthetic("er", n=10000, m=40000, n_vlabels=5, n_elabels=1, seed=1)
+    _announce("mine ER 10k/40k k=4 min_sup=200 (bench config), batched/plain/two-pass")
+    want, st = oracle_mod.fsm(g, 4, 200)
+    text = oracle_mod.format_report(want)
+    assert len(want) == 4415
+    for kw in (dict(mode="batched"), dict(mode="plain"), dict(mode="batched", two_pass=True),
+               dict(mode="batched", sets="bits")):
+        with Miner(g, **kw) as mn:
+            got = mn.mine(4, 200)
+            assert mn.stats()["extended"] == st["extended"], kw
+        assert format_report(got) == text, kw
+
+
 def test_mining_rmat_small(oracle_mod):
     g = synthetic("rmat", scale=11, edge_factor=8, n_vlabels=8, vlabel_zipf=1.1, n_elabels=2,
                   seed=1)
