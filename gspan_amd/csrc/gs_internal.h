@@ -72,7 +72,7 @@ struct RowCtx {
     int phi[GS_MAXV];
     int fcum[GS_MAXE];
     int p;  // pattern index in the batch
-    int base[GS_MAXE];
+    int base[GS_MAXE];  // index in DevGraph::adjl
     int pad;
 };
 static_assert(sizeof(RowCtx) == 96, "RowCtx is six 16-byte loads");
@@ -104,6 +104,16 @@ struct DevGraph {
     const int* src;      // nnz: Edge.from
     const uint2* adj;    // nnz: x = to, y = (el_rank << 16) | tl_rank
     const int* vlab;     // n
+    // the same entries with each vertex's range ordered by neighbour label (tl_rank), and
+    // the CSR index of each: forward segments start at the first neighbour whose label
+    // reaches the pattern's min_label, so entries the rules reject by label are never
+    // visited
+    const uint2* adjl;
+    const int* eidl;
+    // lstart[v * lstride + l]: first adjl index of v whose neighbour label is >= l
+    // (l = 0 .. VL); null when the table would be too large (then binary search)
+    const int* lstart;
+    int lstride;
 };
 
 // MLP parameters on the device: w32 holds every parameter in fp32 (W1[32x128], b1, W2[128x128],
@@ -125,8 +135,12 @@ hipError_t seed_count(const DevGraph& g, int EL, int VL, int* key_count, long lo
 hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int* cursor,
                      int2* rows_out, const SetDesc* sets, uint32_t* pool, hipStream_t s);
 
+// work[row] >= 1 items per row; *work_full (nullable) += the rows' items without the
+// label filter, (r - 1) + sum of segment degrees (the reference scans every neighbour);
+// partial: ceil(R / GS_BLOCK) elements of scratch for that sum
 hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
-                LayerPtrs L, RowCtx* ctx, int* work, hipStream_t s);
+                LayerPtrs L, RowCtx* ctx, int* work, unsigned long long* partial,
+                unsigned long long* work_full, hipStream_t s);
 hipError_t scan_exclusive(const int* in, long long* out, long long n, long long* tmp,
                           long long tmp_elems, hipStream_t s);
 long long scan_tmp_elems(long long n);

@@ -144,8 +144,8 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
         const int target = pick(c.phi, pget<UNI>(P.bw_v, i));
         const int e_el = pget<UNI>(P.bw_el, i);
         const bool le = pget<UNI>(P.bw_le, i) != 0;
-        int lo = c.base[0];  // neighbours of u_n: forward segment 0
-        const int end = lo + c.fcum[0];
+        int lo = g.offsets[u_n];  // all neighbours of u_n, ordered by (to, label)
+        const int end = g.offsets[u_n + 1];
         int hi = end;
         while (lo < hi) {  // first entry with to >= target; adjacency sorted by (to, label)
             const int mid = (lo + hi) >> 1;
@@ -167,11 +167,13 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
         return o;
     }
     const int f = q - (r - 1);
+    if (f >= pick(c.fcum, r)) return o;  // the empty item of a row with no forward item
     int j = 0;  // segment: fcum is non-decreasing
 #pragma unroll
     for (int s = 0; s < GS_MAXE - 1; ++s) j += (s < r && f >= c.fcum[s]) ? 1 : 0;
-    const int e = pick(c.base, j) + f;
-    const uint2 a = g.adj[e];
+    const int el_idx = pick(c.base, j) + f;  // in adjl: neighbours by label from min_label
+    const uint2 a = g.adjl[el_idx];
+    const int e = g.eidl[el_idx];  // the entry's CSR index
     const int to = (int)a.x, el = (int)(a.y >> 16), tl = (int)(a.y & 0xffffu);
     if (tl < P.min_label) return o;
     bool seen = false;
@@ -279,12 +281,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
 
 // ------------------------------------------------------------------ prep
 
-__global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __restrict__ pats,
-                                                   int npats, long long b0, long long R, LayerPtrs L,
-                                                   RowCtx* __restrict__ ctx, int* __restrict__ work) {
-    const long long lb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lb >= R) return;
-    const long long b = b0 + lb;  // row in the batch row space
+// One parent row: images phi, segment bases and item counts (RowCtx), work items; returns
+// the row's items without the label filter.
+__device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
+                                                       const PatDev* __restrict__ pats, int npats,
+                                                       long long b, LayerPtrs L,
+                                                       RowCtx* __restrict__ out_ctx,
+                                                       int* __restrict__ out_work) {
     int lo = 0, hi = npats - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -307,27 +310,82 @@ __global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __r
         idx = rr.x;
     }
     int8v fcum = int8v{0, 0, 0, 0, 0, 0, 0, 0}, base = fcum;
-    int fc = 0;
+    int fc = 0, deg = 0;
 #pragma unroll
     for (int j = 0; j < GS_MAXE; ++j) {
         int bj = 0;
         if (j <= P.r) {
             const int u = pick(phi, P.seg_v[j]);
-            const int o0 = g.offsets[u];
-            bj = o0 - fc;
-            fc += g.offsets[u + 1] - o0;
+            const int o0 = g.offsets[u], o1 = g.offsets[u + 1];
+            int first;  // first neighbour (by label) with label >= min_label
+            if (g.lstart) {
+                first = g.lstart[(long long)u * g.lstride + P.min_label];
+            } else {
+                first = o0;
+                int hi = o1;
+                while (first < hi) {
+                    const int mid = (first + hi) >> 1;
+                    if ((int)(g.adjl[mid].y & 0xffffu) < P.min_label) first = mid + 1;
+                    else hi = mid;
+                }
+            }
+            bj = first - fc;
+            fc += o1 - first;
+            deg += o1 - o0;
         }
         base[j] = bj;
         fcum[j] = fc;
     }
-    int4* out = reinterpret_cast<int4*>(ctx + lb);
+    int4* out = reinterpret_cast<int4*>(out_ctx);
     out[0] = make_int4(phi[0], phi[1], phi[2], phi[3]);
     out[1] = make_int4(phi[4], phi[5], phi[6], phi[7]);
     out[2] = make_int4(fcum[0], fcum[1], fcum[2], fcum[3]);
     out[3] = make_int4(fcum[4], fcum[5], fcum[6], lo);
     out[4] = make_int4(base[0], base[1], base[2], base[3]);
     out[5] = make_int4(base[4], base[5], base[6], 0);
-    work[lb] = (P.r - 1) + fc;
+    // at least one item per row (an empty one when every neighbour is filtered out), so
+    // that a tile of T items spans at most T rows
+    *out_work = max(1, (P.r - 1) + fc);
+    return (unsigned long long)((P.r - 1) + deg);
+}
+
+// partial[blockIdx] (nullable) = the block's items without the label filter; summed by
+// k_sum_partials (atomics on one counter from every block would serialise)
+__global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __restrict__ pats,
+                                                   int npats, long long b0, long long R, LayerPtrs L,
+                                                   RowCtx* __restrict__ ctx, int* __restrict__ work,
+                                                   unsigned long long* __restrict__ partial) {
+    __shared__ unsigned long long s_w[GS_BLOCK / 64];
+    const long long lb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long full = 0;
+    if (lb < R) full = prep_row(g, pats, npats, b0 + lb, L, ctx + lb, work + lb);
+    if (partial) {
+        for (int off = 32; off > 0; off >>= 1) full += __shfl_xor(full, off);
+        if (lane_id() == 0) s_w[threadIdx.x / 64] = full;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = 0;
+            for (int w = 0; w < GS_BLOCK / 64; ++w) t += s_w[w];
+            partial[blockIdx.x] = t;
+        }
+    }
+}
+
+// *total += sum of partial[0 .. n)
+__global__ __launch_bounds__(GS_BLOCK) void k_sum_partials(const unsigned long long* partial,
+                                                           long long n,
+                                                           unsigned long long* total) {
+    __shared__ unsigned long long s_w[GS_BLOCK / 64];
+    unsigned long long t = 0;
+    for (long long i = threadIdx.x; i < n; i += GS_BLOCK) t += partial[i];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if (lane_id() == 0) s_w[threadIdx.x / 64] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < GS_BLOCK / 64; ++w) s += s_w[w];
+        *total += s;
+    }
 }
 
 // ------------------------------------------------------------------ scan (int32 -> int64)
@@ -1146,11 +1204,15 @@ hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int
 }
 
 hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
-                LayerPtrs L, RowCtx* ctx, int* work, hipStream_t s) {
+                LayerPtrs L, RowCtx* ctx, int* work, unsigned long long* partial,
+                unsigned long long* work_full, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     long long grid = (R + GS_BLOCK - 1) / GS_BLOCK;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pats, npats, b0, R,
-                       L, ctx, work);
+                       L, ctx, work, work_full ? partial : nullptr);
+    if (work_full)
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(GS_BLOCK), 0, s, partial, grid,
+                           work_full);
     return hipGetLastError();
 }
 

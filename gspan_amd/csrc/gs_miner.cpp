@@ -275,7 +275,7 @@ struct gs_ctx {
     DevPool pool;
     HostStage stage;
     std::vector<PendingD2H> pending_d2h;
-    DBuf d_offsets, d_src, d_adj, d_vlab;
+    DBuf d_offsets, d_src, d_adj, d_vlab, d_adjl, d_eidl, d_lstart;
     DevGraph dg{};
     DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
     DBuf d_tiles; // per-XCD tile queues of the item kernels
@@ -314,6 +314,10 @@ struct gs_ctx {
     int* err_flag() const { return d_misc.as<int>() + 2; }
     int* overflow_flag() const { return d_misc.as<int>() + 3; }
     long long* total64() const { return reinterpret_cast<long long*>(d_misc.as<int>() + 4); }
+    // items of a step without the label filter (the algorithmic-bytes figure counts them)
+    unsigned long long* work_full64() const {
+        return reinterpret_cast<unsigned long long*>(d_misc.as<int>() + 6);
+    }
 };
 
 namespace {
@@ -742,6 +746,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     DBuf d_ctx(P, sizeof(RowCtx) * std::max<long long>(crow, 1));
     DBuf d_work(P, sizeof(int) * std::max<long long>(crow, 1));
     DBuf d_woff(P, sizeof(long long) * (crow + 1));
+    DBuf d_partial(P, sizeof(unsigned long long) * ((crow + GS_BLOCK - 1) / GS_BLOCK + 1));
     const long long tmp_elems = gsk::scan_tmp_elems(crow);
     DBuf d_tmp(P, sizeof(long long) * tmp_elems);
     DBuf d_kc(P, sizeof(int) * K);
@@ -795,12 +800,15 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     }
 
     long long W_total = 0;
+    unsigned long long W_full = 0;
     std::vector<long long> Wc_host(nchunks, 0);
-    auto prep_chunk = [&](int ci) -> long long {
+    HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->work_full64()), 0, 2, c->stream));
+    auto prep_chunk = [&](int ci, bool first_pass) -> long long {
         const long long b0 = (long long)ci * chunk;
         const long long Rc = std::min(chunk, R - b0);
         HIPCHECK(gsk::prep(c->dg, d_pats.as<PatDev>(), B, b0, Rc, L, d_ctx.as<RowCtx>(),
-                           d_work.as<int>(), c->stream));
+                           d_work.as<int>(), d_partial.as<unsigned long long>(),
+                           first_pass ? c->work_full64() : nullptr, c->stream));
         HIPCHECK(gsk::scan_exclusive(d_work.as<int>(), d_woff.as<long long>(), Rc,
                                      d_tmp.as<long long>(), tmp_elems, c->stream));
         return Rc;
@@ -888,7 +896,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             std::vector<long long> Wc_p(nchunks, 0);
             for (int ci = 0; ci < nchunks; ++ci) {
                 const long long b0 = (long long)ci * chunk;
-                const long long Rc = prep_chunk(ci);
+                const long long Rc = prep_chunk(ci, true);
                 d2h(c, &Wc_p[ci], d_woff.as<long long>() + Rc, 1);
                 const gsk::ItemsLaunch lc = shape(true, false, Rc);
                 time_begin();
@@ -909,6 +917,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                 (int)K, d_flags.as<int>(), c->err_flag(), c->stream));
             int na_err_ov[3] = {0, 0, 0};
             d2h(c, na_err_ov, c->n_active(), 3);  // n_active, err, overflow
+            d2h(c, &W_full, c->work_full64(), 1);
             sync(c);
             resolve_timing(c);
             for (long long w : Wc_p) Wp += w;
@@ -969,7 +978,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                     }
                 }
                 {
-                    double bytes = 16.0 * (double)Wp + 8.0 * (double)emitted;
+                    double bytes = 16.0 * (double)W_full + 8.0 * (double)emitted;
                     for (int i = 0; i < B; ++i)
                         bytes += (double)pats[i].row_count *
                                  (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
@@ -985,13 +994,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             c->st.fallbacks++;
             out.children.clear();
             HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
+            HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->work_full64()), 0, 2, c->stream));
         }
     }
 
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
     for (int ci = 0; ci < nchunks; ++ci) {
         const long long b0 = (long long)ci * chunk;
-        const long long Rc = prep_chunk(ci);
+        const long long Rc = prep_chunk(ci, true);
         d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
         const gsk::ItemsLaunch lc = shape(single, single, Rc);
         time_begin();
@@ -1037,6 +1047,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                         d_flags.as<int>(), c->err_flag(), c->stream));
     int na_err[2] = {0, 0};
     d2h(c, na_err, c->n_active(), 2);
+    d2h(c, &W_full, c->work_full64(), 1);
     sync(c);
     resolve_timing(c);
     for (long long w : Wc_host) W_total += w;
@@ -1150,7 +1161,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
         for (int ci = 0; ci < nchunks; ++ci) {
             const long long b0 = (long long)ci * chunk;
-            const long long Rc = nchunks > 1 ? prep_chunk(ci) : std::min(chunk, R - b0);
+            const long long Rc = nchunks > 1 ? prep_chunk(ci, false) : std::min(chunk, R - b0);
             const gsk::ItemsLaunch lc = shape(true, false, Rc);
             time_begin();
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
@@ -1166,7 +1177,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
     {
-        double bytes = 16.0 * (double)W_total;
+        double bytes = 16.0 * (double)W_full;
         for (int i = 0; i < B; ++i)
             bytes += (double)pats[i].row_count *
                      (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
@@ -1532,18 +1543,49 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
             adj[i].x = (unsigned)e[1];
             adj[i].y = ((unsigned)emap[e[2]] << 16) | (unsigned)vr[e[1]];
         }
+        // each vertex's entries ordered by neighbour label, with their CSR indices
+        std::vector<uint2> adjl(nnz);
+        std::vector<int> eidl(nnz);
+        for (int v = 0; v < n; ++v) {
+            const int o0 = offsets[v], o1 = offsets[v + 1];
+            for (int i = o0; i < o1; ++i) eidl[i] = i;
+            std::stable_sort(eidl.begin() + o0, eidl.begin() + o1, [&](int a, int b) {
+                return (adj[a].y & 0xffffu) < (adj[b].y & 0xffffu);
+            });
+            for (int i = o0; i < o1; ++i) adjl[i] = adj[eidl[i]];
+        }
+        // label start table, when it is at most 1 GiB
+        const long long lstride = (long long)x->VL + 1;
+        std::vector<int> lstart;
+        if ((long long)n * lstride * 4 <= (1LL << 30)) {
+            lstart.resize((size_t)n * lstride);
+            for (int v = 0; v < n; ++v) {
+                int at = offsets[v];
+                const int o1 = offsets[v + 1];
+                for (int l = 0; l <= x->VL; ++l) {
+                    while (at < o1 && (int)(adjl[at].y & 0xffffu) < l) ++at;
+                    lstart[(size_t)v * lstride + l] = at;
+                }
+            }
+        }
         DevPool* P = &x->pool;
         x->d_offsets = DBuf(P, sizeof(int) * (n + 1));
         const int zero = 0;
         if (n == 0) offsets = &zero;
         x->d_src = DBuf(P, sizeof(int) * std::max(nnz, 1));
         x->d_adj = DBuf(P, sizeof(uint2) * std::max(nnz, 1));
+        x->d_adjl = DBuf(P, sizeof(uint2) * std::max(nnz, 1));
+        x->d_eidl = DBuf(P, sizeof(int) * std::max(nnz, 1));
+        if (!lstart.empty()) x->d_lstart = DBuf(P, sizeof(int) * lstart.size());
         x->d_vlab = DBuf(P, sizeof(int) * std::max(n, 1));
         x->d_misc = DBuf(P, 64);
         x->d_tiles = DBuf(P, sizeof(int) * GS_XCDS * GS_XCD_STRIDE);
         h2d(x, x->d_offsets.as<int>(), offsets, n + 1);
         h2d(x, x->d_src.as<int>(), src.data(), nnz);
         h2d(x, x->d_adj.as<uint2>(), adj.data(), nnz);
+        h2d(x, x->d_adjl.as<uint2>(), adjl.data(), nnz);
+        h2d(x, x->d_eidl.as<int>(), eidl.data(), nnz);
+        if (!lstart.empty()) h2d(x, x->d_lstart.as<int>(), lstart.data(), lstart.size());
         h2d(x, x->d_vlab.as<int>(), vr.data(), n);
         init_label_stats(x, vr, src, adj);
         mlp_init(x);
@@ -1553,6 +1595,10 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->dg.offsets = x->d_offsets.as<int>();
         x->dg.src = x->d_src.as<int>();
         x->dg.adj = x->d_adj.as<uint2>();
+        x->dg.adjl = x->d_adjl.as<uint2>();
+        x->dg.eidl = x->d_eidl.as<int>();
+        x->dg.lstart = lstart.empty() ? nullptr : x->d_lstart.as<int>();
+        x->dg.lstride = (int)lstride;
         x->dg.vlab = x->d_vlab.as<int>();
     });
     if (rc != GS_OK) {
@@ -1937,6 +1983,9 @@ int gs_destroy(gs_ctx* c) {
     c->d_offsets.reset();
     c->d_src.reset();
     c->d_adj.reset();
+    c->d_adjl.reset();
+    c->d_eidl.reset();
+    c->d_lstart.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();
