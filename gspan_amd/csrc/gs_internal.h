@@ -167,10 +167,8 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
 // resident workgroups per CU of the count (fill = false) or fill kernel at a tile size
 int items_blocks_per_cu(bool fill, int tile);
 
-hipError_t compact_active(const int* key_count, long long K, int* active_keys,
-                          int* active_counts, int* n_active, hipStream_t s);
-// The same in ascending key order (flags / pos: K and K+1 elements; scan_tmp as for
-// scan_exclusive over K); *n_active is written, not added to.
+// Keys with a non-zero count, in ascending key order, with their counts (flags / pos: K
+// and K+1 elements; scan_tmp as for scan_exclusive over K); *n_active is written.
 hipError_t compact_active_ordered(const int* key_count, long long K, int* active_keys,
                                   int* active_counts, int* n_active, int* flags, long long* pos,
                                   long long* scan_tmp, long long scan_tmp_elems, hipStream_t s);

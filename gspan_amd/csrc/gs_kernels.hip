@@ -272,10 +272,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
         }
         __syncthreads();
     }
-    if (!FILL) {
+    if (!FILL) {  // one atomic per workgroup on the total
+        __shared__ long long s_tot[GS_BLOCK / 64];
         for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
-        if (lane_id() == 0 && local)
-            atomicAdd((unsigned long long*)total, (unsigned long long)local);
+        if (lane_id() == 0) s_tot[threadIdx.x / 64] = local;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long t = 0;
+            for (int w = 0; w < GS_BLOCK / 64; ++w) t += s_tot[w];
+            if (t) atomicAdd((unsigned long long*)total, (unsigned long long)t);
+        }
     }
 }
 
@@ -550,19 +556,30 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     // first drains the b % 8-th contiguous eighth of the tiles (whose rows, patterns and
     // child sets then stay in that XCD's L2), then helps the other eighths.
     const int x0 = blockIdx.x % GS_XCDS;
-    int xi = 0;  // queues this workgroup has found empty (thread 0 only)
+    // Tiles are claimed in batches of TB per atomic when there are many per workgroup:
+    // the queue counters are hot addresses and device atomics serialise at the memory side.
+    const int TB = ntiles >= 64LL * gridDim.x ? 4 : 1;
+    int xi = 0;                        // queues this workgroup has found empty (thread 0 only)
+    long long next = 0, claim_end = 0;  // claimed, not yet processed tiles (thread 0 only)
     for (;;) {
         if (threadIdx.x == 0) {
             long long tile = -1;
-            while (xi < GS_XCDS) {
-                const int x = (x0 + xi) % GS_XCDS;
-                const long long beg = ntiles * x / GS_XCDS, end = ntiles * (x + 1) / GS_XCDS;
-                const long long t = beg + atomicAdd(&tile_counters[x * GS_XCD_STRIDE], 1);
-                if (t < end) {
-                    tile = t;
-                    break;
+            if (next < claim_end) {
+                tile = next++;
+            } else {
+                while (xi < GS_XCDS) {
+                    const int x = (x0 + xi) % GS_XCDS;
+                    const long long beg = ntiles * x / GS_XCDS, end = ntiles * (x + 1) / GS_XCDS;
+                    const long long t =
+                        beg + (long long)atomicAdd(&tile_counters[x * GS_XCD_STRIDE], TB);
+                    if (t < end) {
+                        tile = t;
+                        next = t + 1;
+                        claim_end = min(t + TB, end);
+                        break;
+                    }
+                    ++xi;
                 }
-                ++xi;
             }
             s_tile = tile;
         }
@@ -718,29 +735,6 @@ __global__ __launch_bounds__(GS_BLOCK) void k_active_scatter(const int* key_coun
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *n_active = (int)pos[K];
-}
-
-__global__ __launch_bounds__(GS_BLOCK) void k_compact_active(const int* key_count, long long K,
-                                                             int* active, int* active_counts,
-                                                             int* n_active) {
-    long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x; base < K; base += stride) {
-        long long i = base + threadIdx.x;
-        bool a = i < K && key_count[i] > 0;
-        unsigned long long m = __ballot(a);
-        int wbase = 0;
-        int leader = 0;
-        if (m) {
-            leader = __ffsll((long long)m) - 1;
-            if (lane_id() == leader) wbase = atomicAdd(n_active, __popcll(m));
-            wbase = __shfl(wbase, leader);
-        }
-        if (a) {
-            const int at = wbase + (int)mbcnt64(m);
-            active[at] = (int)i;
-            active_counts[at] = key_count[i];
-        }
-    }
 }
 
 __global__ void k_scatter_slots(int* key_slot, const int* keys, int nkeys) {
@@ -1291,13 +1285,6 @@ int items_blocks_per_cu(bool fill, int tile) {
 #undef GS_OCC
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     return per_cu;
-}
-
-hipError_t compact_active(const int* key_count, long long K, int* active_keys,
-                          int* active_counts, int* n_active, hipStream_t s) {
-    hipLaunchKernelGGL(k_compact_active, dim3(grid_for(K, GS_BLOCK, 4096)), dim3(GS_BLOCK), 0, s,
-                       key_count, K, active_keys, active_counts, n_active);
-    return hipGetLastError();
 }
 
 hipError_t compact_active_ordered(const int* key_count, long long K, int* active_keys,
