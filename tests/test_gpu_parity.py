@@ -271,3 +271,26 @@ def test_prune_graph_matches_reference_golden():
             np.testing.assert_array_equal(got.offsets, want.offsets)
             np.testing.assert_array_equal(got.vlabels, want.vlabels)
             np.testing.assert_array_equal(got.edges, want.edges)
+
+
+def test_mining_edge_cases(oracle_mod):
+    """k = 1, thresholds above every support, no edges, isolated vertices, the empty graph,
+    and a bitmap/bytemap budget of one byte: every case gives the oracle's report."""
+    cases = []
+    g = synthetic("er", n=400, m=900, n_vlabels=3, n_elabels=2, seed=31)
+    cases += [("k1", g, 1, 5), ("ms_above_all", g, 3, 10**6), ("ms1_k2", g, 2, 1)]
+    iso = csr_from_undirected(12, np.array([0, 1, 2, 3]), np.array([1, 2, 3, 0]),
+                              np.array([0, 1, 0, 1]), np.array([0, 1] * 6))
+    cases += [("isolated", iso, 3, 1)]
+    noedge = CSRGraph(np.zeros(6, np.int32), np.array([0, 1, 2, 0, 1], np.int32),
+                      np.zeros((0, 3), np.int32))
+    cases += [("no_edges", noedge, 3, 1)]
+    empty = CSRGraph(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros((0, 3), np.int32))
+    cases += [("empty", empty, 2, 1)]
+    for name, gg, k, ms in cases:
+        _announce(f"mine edge case {name} k={k} ms={ms}")
+        want, st = oracle_mod.fsm(gg, k, ms)
+        for mode in ("batched", "plain"):
+            with Miner(gg, mode=mode) as mn:
+                assert mn.mine(k, ms) == want, (name, mode)
+                assert mn.stats()["extended"] == st["extended"], (name, mode)
