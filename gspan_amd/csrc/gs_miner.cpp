@@ -619,6 +619,46 @@ void resolve_timing(gs_ctx* c) {
     c->ev_used = 0;
 }
 
+// The active keys' arrays (keys, counts and optionally flags, n of them) to the host.
+// When the key space is small they are copied whole in the same synchronisation as the
+// active count (one host round trip instead of two); otherwise after it.
+constexpr long long kSpeculativeKeys = 1 << 16;
+struct ActiveHost {
+    std::vector<int> keys, cnt, flag;
+};
+void fetch_active(gs_ctx* c, long long K, const int* d_active, const int* d_acnt,
+                  const int* d_flags, int* head, int nhead, ActiveHost& h) {
+    const bool spec = K <= kSpeculativeKeys;
+    d2h(c, head, c->n_active(), (size_t)nhead);
+    if (spec) {
+        h.keys.resize(K);
+        h.cnt.resize(K);
+        d2h(c, h.keys.data(), d_active, (size_t)K);
+        d2h(c, h.cnt.data(), d_acnt, (size_t)K);
+        if (d_flags) {
+            h.flag.resize(K);
+            d2h(c, h.flag.data(), d_flags, (size_t)K);
+        }
+    }
+    sync(c);
+    const int A = head[0];
+    if (spec) {
+        h.keys.resize(A);
+        h.cnt.resize(A);
+        if (d_flags) h.flag.resize(A);
+        return;
+    }
+    h.keys.resize(A);
+    h.cnt.resize(A);
+    d2h(c, h.keys.data(), d_active, (size_t)A);
+    d2h(c, h.cnt.data(), d_acnt, (size_t)A);
+    if (d_flags) {
+        h.flag.resize(A);
+        d2h(c, h.flag.data(), d_flags, (size_t)A);
+    }
+    sync(c);
+}
+
 // Keys with a non-zero count, in ascending key order, into active / acnt; *n_active.
 void compact_ordered(gs_ctx* c, const int* key_count, long long K, int* active, int* acnt) {
     DevPool* P = &c->pool;
@@ -915,10 +955,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
             HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(),
                                 (int)K, d_flags.as<int>(), c->err_flag(), c->stream));
-            int na_err_ov[3] = {0, 0, 0};
-            d2h(c, na_err_ov, c->n_active(), 3);  // n_active, err, overflow
+            int na_err_ov[3] = {0, 0, 0};  // n_active, err, overflow
             d2h(c, &W_full, c->work_full64(), 1);
-            sync(c);
+            ActiveHost ah;
+            fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(),
+                         na_err_ov, 3, ah);
             resolve_timing(c);
             for (long long w : Wc_p) Wp += w;
             c->st.items += Wp;
@@ -929,11 +970,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
             if (na_err_ov[2] == 0) {
                 const int A = na_err_ov[0];
-                std::vector<int> akeys(A), acnt(A), aflag(A);
-                d2h(c, akeys.data(), d_active.as<int>(), A);
-                d2h(c, acnt.data(), d_acnt.as<int>(), A);
-                d2h(c, aflag.data(), d_flags.as<int>(), A);
-                sync(c);
+                const std::vector<int>& akeys = ah.keys;
+                const std::vector<int>& acnt = ah.cnt;
+                const std::vector<int>& aflag = ah.flag;
                 std::vector<int> order(A);  // keys arrive in key order
                 for (int i = 0; i < A; ++i) order[i] = i;
                 std::vector<SetDesc> live_sets;
@@ -1046,9 +1085,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
                         d_flags.as<int>(), c->err_flag(), c->stream));
     int na_err[2] = {0, 0};
-    d2h(c, na_err, c->n_active(), 2);
     d2h(c, &W_full, c->work_full64(), 1);
-    sync(c);
+    ActiveHost ah;
+    fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(), na_err, 2, ah);
     resolve_timing(c);
     for (long long w : Wc_host) W_total += w;
     c->st.items += W_total;
@@ -1056,11 +1095,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
     if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
     const int A = na_err[0];
-    std::vector<int> akeys(A), acnt(A), aflag(A);
-    d2h(c, akeys.data(), d_active.as<int>(), A);
-    d2h(c, acnt.data(), d_acnt.as<int>(), A);
-    d2h(c, aflag.data(), d_flags.as<int>(), A);
-    sync(c);
+    const std::vector<int>& akeys = ah.keys;
+    const std::vector<int>& acnt = ah.cnt;
+    const std::vector<int>& aflag = ah.flag;
     // keys arrive in key order = (parent, slot) order
     std::vector<int> order(A);
     for (int i = 0; i < A; ++i) order[i] = i;
@@ -1273,12 +1310,10 @@ void mine_seeds(gs_ctx* c) {
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K);
     compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
     int A = 0;
-    d2h(c, &A, c->n_active(), 1);
-    sync(c);
-    std::vector<int> akeys(A), acnt(A);
-    d2h(c, akeys.data(), d_active.as<int>(), A);
-    d2h(c, acnt.data(), d_acnt.as<int>(), A);
-    sync(c);
+    ActiveHost ah;
+    fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), nullptr, &A, 1, ah);
+    const std::vector<int>& akeys = ah.keys;
+    const std::vector<int>& acnt = ah.cnt;
     std::vector<int> order(A);  // keys arrive in key order
     for (int i = 0; i < A; ++i) order[i] = i;
     const bool rows = c->k > 1;
