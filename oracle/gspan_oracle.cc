@@ -9,13 +9,18 @@
 // reference's CPU path (per-thread std::map groupings, bitmaps with atomic OR) so
 // that, timed as the CPU baseline, it costs what the reference's OpenMP path costs.
 //
-// Parity pins (tests/test_oracle_*.py):
-//   * loaders / CSR build      -> reference convert binary (oracle/_ref, container only),
-//                                 hand graphs, binary-format round trips
+// Parity pins (tests/test_oracle_*.py, tests/test_golden.py, tests/test_prune.py):
+//   * loaders / CSR build      -> the reference's own convert (oracle/_ref, container only,
+//                                 and its outputs committed in tests/golden), hand graphs,
+//                                 binary-format round trips
+//   * pattern graph            -> the reference's CSRGraph(const DFSCode&) (oracle/_ref and
+//                                 tests/golden/pattern_graphs.json)
 //   * rules / extension        -> brute-force monomorphism enumeration on small graphs
 //   * MNI (both modes)         -> closed forms (K_n, stars) + brute force
 //   * canonicality             -> exhaustive minimum-DFS-code enumeration
-//   * output order / format    -> restated from gspan.cu:67-77, 1142-1168
+//   * output order / format    -> parity unpinned: restated from gspan.cu:67-77,
+//                                 1142-1168 (the reference's mining code does not build
+//                                 here, so no run of it can check the line order)
 //
 // Build: oracle/Makefile (g++ -O3 -fopenmp -shared).
 

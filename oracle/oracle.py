@@ -3,6 +3,10 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` as the checker / timed CPU baseline, never by the
 product package.
+
+``prune_graph`` is pinned against the reference's own prune_graph (oracle/_ref, and its
+outputs in tests/golden) and a loop restatement; ``format_report`` order and format are
+parity unpinned (restated from gspan.cu:67-77, 1142-1168).
 """
 from __future__ import annotations
 
