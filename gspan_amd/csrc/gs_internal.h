@@ -155,7 +155,10 @@ struct ItemsLaunch {
 // tile_counters: GS_XCDS * GS_XCD_STRIDE ints, reset by the launchers.
 hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                       KeySpace ks, int* key_count, int* tile_counters, hipStream_t s);
+                       KeySpace ks, int* key_count, int* tile_counters, bool saturate,
+                       hipStream_t s);
+// saturate: the launches adding to key_count may process more than INT_MAX items in all;
+// a count that passes INT_MAX then reads INT_MIN (otherwise counts are exact 32-bit sums).
 // key_count (nullable): also count children per key; key_slot == nullptr: slot = key.
 // cap_end (nullable): rows of slot s may only be written below cap_end[s]; a child that
 // does not fit, or a hash set that fills up, sets *overflow instead (predicted sizes).
@@ -163,7 +166,8 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
                       KeySpace ks, int* key_count, const int* key_slot, int* cursor,
                       const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                      int* overflow, int* tile_counters, int nlive, hipStream_t s);
+                      int* overflow, int* tile_counters, int nlive, bool saturate,
+                      hipStream_t s);
 // resident workgroups per CU of the count (fill = false) or fill kernel at a tile size
 int items_blocks_per_cu(bool fill, int tile);
 

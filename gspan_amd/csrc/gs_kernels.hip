@@ -15,6 +15,8 @@
 // work; persistent workgroups steal fixed tiles of items from an atomic counter and
 // locate their rows by a search over the scanned offsets staged in LDS.  Output slots
 // come from one atomic per (tile, distinct key): keys are aggregated per tile in LDS.
+#include <climits>
+
 #include "gs_internal.h"
 
 namespace {
@@ -506,14 +508,33 @@ __device__ __forceinline__ int lds_slot_fill(int* s_key, int key, int* fill) {
     }
 }
 
+// Tiles after which an item kernel flushes its persistent count table (a smaller value can be
+// given at build time to exercise the flush).
+#ifndef GS_FLUSH_TILES
+#define GS_FLUSH_TILES (1 << 14)
+#endif
+
+// Adds c > 0 to a child count (used when the launches adding to the counts may process
+// more than INT_MAX items; otherwise no count can pass INT_MAX and a plain add is used, as
+// the returned value costs a wait).  A count that would pass INT_MAX becomes INT_MIN and stays
+// there: every add that sees a negative count (or crosses INT_MAX) stores INT_MIN after
+// it, so the last write to the word is such a store whenever the true sum passes INT_MAX
+// (a child count is the number of child embeddings; the host reads a negative count as
+// "more than INT_MAX").  Below that the count is exact.
+__device__ __forceinline__ void count_add(int* count, int c) {
+    const int old = atomicAdd(count, c);
+    if (old < 0 || old > INT_MAX - c) atomicExch(count, INT_MIN);
+}
+
 // TILE items per stolen tile (256, 512 or 1024); LDS use grows with TILE, so the tile
 // size trades the number of steals against workgroups resident per CU.
 //
 // Child counts (key_count) are summed per workgroup in a second LDS table that lives
-// across the tiles the workgroup steals, and added to global memory when it is half full
-// and when the workgroup ends: the counters of a level's hot keys then see one atomic
-// per workgroup instead of one per tile (device-scope atomics are performed at the
-// memory side, so atomics on one hot address serialise there).
+// across the tiles the workgroup steals, and added to global memory when it is half full,
+// after 2^14 tiles (so no LDS count passes 2^24) and when the workgroup ends: the
+// counters of a level's hot keys then see one atomic per workgroup instead of one per
+// tile (device-scope atomics are performed at the memory side, so atomics on one hot
+// address serialise there).
 template <bool FILL, int TILE>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
@@ -524,7 +545,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     const int* __restrict__ cap_end,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
                                                     uint32_t* pool, int* overflow,
-                                                    int* tile_counters, int dedup) {
+                                                    int* tile_counters, int dedup,
+                                                    int saturate) {
     constexpr int LDS_SLOTS = 2 * TILE;
     constexpr int ITEMS_PER_THREAD = TILE / GS_BLOCK;
     static_assert(TILE % GS_BLOCK == 0 && (TILE & (TILE - 1)) == 0, "tile: power of two");
@@ -539,8 +561,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     // tile (at most TILE new keys) always finds room and probing ends
     __shared__ int s_pkey[LDS_SLOTS];
     __shared__ int s_pcnt[LDS_SLOTS];
+    // s_pfill: used slots of the persistent table (low 16 bits, at most LDS_SLOTS) and
+    // tiles since its last flush (high bits)
     __shared__ int s_pfill;
     __shared__ long long s_tile, s_rlo;
+    // every tile adds at most TILE to a count, so no LDS count passes 2^14 * TILE <= 2^24
+    constexpr int FLUSH_TILES = GS_FLUSH_TILES;
+    static_assert(FLUSH_TILES >= 1 && FLUSH_TILES <= (1 << 14), "tile bits of s_pfill");
     if (R <= 0) return;
     for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
         s_key[h] = -1;
@@ -582,6 +609,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 }
             }
             s_tile = tile;
+            // all reads of the last tile ended before its last barrier; unsigned, as the
+            // tile bits only reset on a flush (they are read only when counting)
+            s_pfill = (int)((unsigned)s_pfill + (1u << 16));
         }
         __syncthreads();
         const long long tile = s_tile;
@@ -662,10 +692,15 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             }
         }
         __syncthreads();
-        if (key_count && s_pfill > TILE) {  // uniform: read after the barrier
+        // Flush the persistent table once more than TILE slots are used (a tile adds at
+        // most TILE keys, so the next tile still finds room) or after FLUSH_TILES tiles.
+        // Uniform: s_pfill was last written before the barrier above, and thread 0 writes
+        // it next only after the barrier that ends phase 3.
+        if (key_count && ((s_pfill & 0xffff) > TILE || (s_pfill >> 16) >= FLUSH_TILES)) {
             for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
                 if (s_pkey[h] >= 0) {
-                    atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
+                    if (saturate) count_add(&key_count[s_pkey[h]], s_pcnt[h]);
+                    else atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
                     s_pkey[h] = -1;
                     s_pcnt[h] = 0;
                 }
@@ -707,9 +742,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         }
     }
     // the loop ends after a barrier; the persistent table was last written before it
-    if (key_count)
-        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK)
-            if (s_pkey[h] >= 0) atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
+    if (key_count) {
+        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+            if (s_pkey[h] < 0) continue;
+            if (saturate) count_add(&key_count[s_pkey[h]], s_pcnt[h]);
+            else atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ keys
@@ -719,7 +758,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_active_flags(const int* key_count,
                                                            int* flags) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += stride)
-        flags[i] = key_count[i] > 0 ? 1 : 0;
+        flags[i] = key_count[i] != 0 ? 1 : 0;  // negative: more than INT_MAX (count_add)
 }
 
 __global__ __launch_bounds__(GS_BLOCK) void k_active_scatter(const int* key_count,
@@ -729,7 +768,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_active_scatter(const int* key_coun
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += stride) {
         const int c = key_count[i];
-        if (c > 0) {
+        if (c != 0) {
             active[pos[i]] = (int)i;
             active_counts[pos[i]] = c;
         }
@@ -1231,13 +1270,14 @@ hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* 
                         const RowCtx* ctx, const long long* woff, long long b0, long long R,
                         KeySpace ks, int* key_count, const int* key_slot, int* cursor,
                         const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                        int* overflow, int* tile_counters, int dedup, hipStream_t s) {
+                        int* overflow, int* tile_counters, int dedup, int saturate,
+                        hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
                        (long long)GS_XCDS * GS_XCD_STRIDE);
 #define GS_LAUNCH_ITEMS(T)                                                                     \
     hipLaunchKernelGGL((k_items<FILL, T>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, \
                        woff, b0, R, ks, key_count, key_slot, cursor, cap_end, rows_out, sets,  \
-                       pool, overflow, tile_counters, dedup)
+                       pool, overflow, tile_counters, dedup, saturate)
     switch (lc.tile) {
         case 256: GS_LAUNCH_ITEMS(256); break;
         case 512: GS_LAUNCH_ITEMS(512); break;
@@ -1250,22 +1290,26 @@ hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* 
 
 hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                       KeySpace ks, int* key_count, int* tile_counters, hipStream_t s) {
+                       KeySpace ks, int* key_count, int* tile_counters, bool saturate,
+                       hipStream_t s) {
     if (R <= 0) return hipSuccess;
     return launch_items<false>(lc, g, pats, ctx, woff, b0, R, ks, key_count, nullptr, nullptr,
-                               nullptr, nullptr, nullptr, nullptr, nullptr, tile_counters, 0, s);
+                               nullptr, nullptr, nullptr, nullptr, nullptr, tile_counters, 0,
+                               saturate ? 1 : 0, s);
 }
 
 hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
                       KeySpace ks, int* key_count, const int* key_slot, int* cursor,
                       const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                      int* overflow, int* tile_counters, int nlive, hipStream_t s) {
+                      int* overflow, int* tile_counters, int nlive, bool saturate,
+                      hipStream_t s) {
     if (R <= 0) return hipSuccess;
     // pair keys slot * TILE + row must stay below 2^31
     const int dedup = (long long)nlive * lc.tile < (1LL << 31) ? 1 : 0;
     return launch_items<true>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot, cursor,
-                              cap_end, rows_out, sets, pool, overflow, tile_counters, dedup, s);
+                              cap_end, rows_out, sets, pool, overflow, tile_counters, dedup,
+                              saturate ? 1 : 0, s);
 }
 
 int items_blocks_per_cu(bool fill, int tile) {

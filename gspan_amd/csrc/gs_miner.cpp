@@ -270,6 +270,7 @@ struct gs_ctx {
     hipStream_t stream = nullptr;
     int cus = 0;
     int n = 0, nnz = 0;
+    int max_deg = 0;
     std::vector<int> vl_vals, el_vals;  // rank -> original label
     int VL = 0, EL = 0;
     DevPool pool;
@@ -670,6 +671,12 @@ void compact_ordered(gs_ctx* c, const int* key_count, long long K, int* active, 
                                          tmp.as<long long>(), te, c->stream));
 }
 
+// Child count as read back from the device: a negative count means more than INT_MAX
+// embeddings (the kernels saturate it); 2^31 stands for it, so a level that would have to
+// store those rows fails its 2^31-row check, while a level that stores no rows is exact
+// (supports come from the image sets, not the counts).
+inline long long child_count(int c) { return c < 0 ? (long long)INT_MAX + 1 : c; }
+
 struct ChildInfo {
     HPat pat;
     long long count;
@@ -791,6 +798,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     DBuf d_tmp(P, sizeof(long long) * tmp_elems);
     DBuf d_kc(P, sizeof(int) * K);
     HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
+    // Child counts saturate (a returning atomic per flush) only when this step's items --
+    // at most 1 + (r - 1) backward probes + (r + 1) forward segments of at most max degree
+    // entries per parent row -- may pass INT_MAX.
+    double items_bound = 0;
+    for (int i = 0; i < B; ++i)
+        items_bound += (double)pats[i].row_count *
+                       (1.0 + (pd[i].r - 1) + (double)(pd[i].r + 1) * c->max_deg);
+    const bool saturate = items_bound > (double)INT_MAX;
 
     // Single pass at the last level: when a bitmap for every potential child key fits in a
     // quarter of the free budget, the fill pass counts children and fills the MNI sets in
@@ -945,7 +960,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                          nullptr, d_cur.as<int>(), d_end.as<int>(),
                                          rows.as<int2>(), d_psets.as<SetDesc>(),
                                          d_ppool.as<uint32_t>(), c->overflow_flag(),
-                                         c->tile_counters(), (int)K, c->stream));
+                                         c->tile_counters(), (int)K, saturate, c->stream));
                 time_end(1, &Wc_p[ci]);
                 c->st.fill_launches++;
             }
@@ -988,11 +1003,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                     ChildInfo ci;
                     if (aflag[a] || all_live)  // other children only count as candidates
                         ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
-                    ci.count = acnt[a];
+                    ci.count = child_count(acnt[a]);
                     ci.canonical = aflag[a];
                     ci.support = 0;
                     ci.row_off = row_off[key];
-                    emitted += acnt[a];
+                    emitted += ci.count;
                     if (ci.canonical) live_sets.push_back(psets[key]);
                     out.children.push_back(ci);
                 }
@@ -1049,13 +1064,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
                                      nullptr, nullptr, nullptr, d_all_sets.as<SetDesc>(),
                                      d_all_pool.as<uint32_t>(), c->overflow_flag(),
-                                     c->tile_counters(), (int)K, c->stream));
+                                     c->tile_counters(), (int)K, saturate, c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
         } else {
             HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                       d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
-                                      c->tile_counters(), c->stream));
+                                      c->tile_counters(), saturate, c->stream));
             time_end(0, &Wc_host[ci]);
             c->st.count_launches++;
         }
@@ -1117,11 +1132,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         ChildInfo ci;
         if (aflag[a] || all_live)  // other children only count as candidates
             ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
-        ci.count = acnt[a];
+        ci.count = child_count(acnt[a]);
         ci.canonical = aflag[a];
         ci.support = 0;
         ci.row_off = -1;
-        emitted += acnt[a];
+        emitted += ci.count;
         if (ci.canonical || all_live) {
             live_child.push_back((int)out.children.size());
             live_keys.push_back(key);
@@ -1206,7 +1221,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      d_slot.as<int>(), d_cursor.as<int>(), nullptr,
                                      write_rows ? out.rows.ptr() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
-                                     c->overflow_flag(), c->tile_counters(), NL, c->stream));
+                                     c->overflow_flag(), c->tile_counters(), NL, false,
+                                     c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
         }
@@ -1536,6 +1552,7 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         }
         x->n = n;
         x->nnz = nnz;
+        for (int v = 0; v < n; ++v) x->max_deg = std::max(x->max_deg, offsets[v + 1] - offsets[v]);
         x->vl_vals.assign(vlabels, vlabels + n);
         std::sort(x->vl_vals.begin(), x->vl_vals.end());
         x->vl_vals.erase(std::unique(x->vl_vals.begin(), x->vl_vals.end()), x->vl_vals.end());

@@ -169,7 +169,7 @@ __global__ void k_features(const PatDev* __restrict__ pats, int npats, KeySpace 
 // target = log1p(count) for logged samples; predicted count = expm1(y)
 __global__ void k_log_targets(const int* __restrict__ counts, long long K, float* __restrict__ t) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < K) t[i] = log1pf((float)counts[i]);
+    if (i < K) t[i] = log1pf(counts[i] < 0 ? 2147483648.0f : (float)counts[i]);  // < 0: > INT_MAX
 }
 
 inline unsigned blocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }

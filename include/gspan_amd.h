@@ -74,7 +74,8 @@ typedef struct gs_opts {
 #define GS_SETS_BYTES 2
 
 typedef struct gs_stats {
-    long long extended;     /* child rows emitted by extension (gspan.cu:708,717,730)    */
+    long long extended;     /* child rows emitted by extension (gspan.cu:708,717,730);   */
+                            /* a child with more than 2^31-1 rows adds 2^31 (lower bound) */
     long long candidates;   /* candidate codes examined (mine() calls)                   */
     long long canonical;    /* candidates passing the canonicality check                 */
     long long reported;     /* frequent patterns reported                                */

@@ -8,6 +8,7 @@ import pytest
 
 import brute
 from gspan_amd.graph import CSRGraph, synthetic, csr_from_undirected
+from gspan_amd import _lib as L
 from gspan_amd.miner import Miner, format_report
 
 pytestmark = pytest.mark.gpu
@@ -294,3 +295,44 @@ def test_mining_edge_cases(oracle_mod):
             with Miner(gg, mode=mode) as mn:
                 assert mn.mine(k, ms) == want, (name, mode)
                 assert mn.stats()["extended"] == st["extended"], (name, mode)
+
+
+def _star(d):
+    """Hub 0 joined to leaves 1..d; one vertex label, one edge label."""
+    leaves = np.arange(1, d + 1)
+    return csr_from_undirected(d + 1, np.zeros(d, np.int64), leaves, np.zeros(d, np.int64),
+                               np.zeros(d + 1, np.int64))
+
+
+def test_child_count_past_int_max(oracle_mod):
+    """More than 2^31 - 1 embeddings of one child.  In a star whose hub has degree d, the
+    3-edge star has d(d-1)(d-2) embeddings: 2,191,922,600 for d = 1300, past a 32-bit count.
+    With one label, mining to k = 3 reports the edge (support d + 1: every vertex can image
+    either end), the 2-path and the 3-star (support 1: only the hub can be their centre).
+    The oracle fixes the codes and order on a star of degree 40; the large star must give
+    the same codes with those closed-form supports in every mode.  Extending the 2-path
+    on the large star must store its child rows, more than 2^31 of them: that is an error."""
+    _announce("star d=40 vs oracle")
+    small = _star(40)
+    want_small, _ = oracle_mod.fsm(small, 3, 1)
+    assert [s for _, s in want_small] == [41, 1, 1]
+    d = 1300
+    assert d * (d - 1) * (d - 2) > 2**31 - 1
+    want = [(code, s) for (code, _), s in zip(want_small, [d + 1, 1, 1])]
+    big = _star(d)
+    for mode in ("batched", "plain"):
+        _announce(f"star d={d} k=3 mode={mode}")
+        with Miner(big, mode=mode) as mn:
+            assert mn.mine(3, 1) == want, mode
+            assert mn.stats()["extended"] >= 2**31, mode  # a saturated count adds 2^31
+    _announce(f"star d={d} extend 2-path")
+    path = [(0, 1, 0, 0, 0), (1, 2, -1, 0, 0)]
+    (e5, l0), = oracle_mod.seed(big)
+    assert e5 == path[0] and len(l0) == 2 * d
+    with Miner(big) as mn:
+        ch = mn.extend(path[:1], [l0])
+        rows = next(r for e5, r, _, _ in ch if e5 == path[1])
+        assert len(rows) == d * (d - 1)
+        with pytest.raises(L.GsError, match="2\\^31") as ei:
+            mn.extend(path, [l0, rows])
+        assert ei.value.code == L.GS_ERR_BUDGET
