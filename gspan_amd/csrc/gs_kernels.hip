@@ -508,6 +508,11 @@ __device__ __forceinline__ int lds_slot_fill(int* s_key, int key, int* fill) {
     }
 }
 
+// Key spaces up to this size are compacted by one workgroup (k_compact_small).
+#ifndef GS_COMPACT_SMALL
+#define GS_COMPACT_SMALL (1 << 17)
+#endif
+
 // Tiles after which an item kernel flushes its persistent count table (a smaller value can be
 // given at build time to exercise the flush).
 #ifndef GS_FLUSH_TILES
@@ -774,6 +779,39 @@ __global__ __launch_bounds__(GS_BLOCK) void k_active_scatter(const int* key_coun
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *n_active = (int)pos[K];
+}
+
+// The same compaction in one workgroup, for key spaces of at most GS_COMPACT_SMALL keys:
+// one launch instead of five (flags, three scan stages, scatter), whose launch gaps cost
+// more than the work at these sizes.  Each pass takes GS_BLOCK * 16 consecutive keys, 16
+// per thread, and places them after the keys of the earlier passes.
+__global__ __launch_bounds__(GS_BLOCK) void k_compact_small(const int* key_count, int K,
+                                                            int* active, int* active_counts,
+                                                            int* n_active) {
+    constexpr int PER = 16;
+    long long placed = 0;  // uniform: every thread adds the same pass totals
+    for (int base = 0; base < K; base += GS_BLOCK * PER) {
+        const int i0 = base + threadIdx.x * PER;
+        int c[PER];
+        int nz = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            c[q] = i0 + q < K ? key_count[i0 + q] : 0;
+            nz += c[q] != 0;  // negative: more than INT_MAX (count_add)
+        }
+        long long total = 0;
+        long long at = placed + block_exclusive_scan(nz, &total);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            if (c[q] != 0) {
+                active[at] = i0 + q;
+                active_counts[at] = c[q];
+                ++at;
+            }
+        }
+        placed += total;
+    }
+    if (threadIdx.x == 0) *n_active = (int)placed;
 }
 
 __global__ void k_scatter_slots(int* key_slot, const int* keys, int nkeys) {
@@ -1336,6 +1374,11 @@ hipError_t compact_active_ordered(const int* key_count, long long K, int* active
                                   long long* scan_tmp, long long scan_tmp_elems, hipStream_t s) {
     if (K <= 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_active, 0, 1LL);
+        return hipGetLastError();
+    }
+    if (K <= GS_COMPACT_SMALL) {
+        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(GS_BLOCK), 0, s, key_count, (int)K,
+                           active_keys, active_counts, n_active);
         return hipGetLastError();
     }
     const unsigned nb = (unsigned)grid_for(K, GS_BLOCK, 1 << 20);
