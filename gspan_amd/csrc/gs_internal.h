@@ -152,10 +152,18 @@ struct ItemsLaunch {
     int grid;
 };
 
-// tile_counters: GS_XCDS * GS_XCD_STRIDE ints, reset by the launchers.
+// Tile queues of the item kernels: two sets of GS_TILE_QUEUE ints (one counter per XCD,
+// GS_XCD_STRIDE apart) used in turn.  A launch takes set `cur` and zeroes the other one for
+// the next launch (launches on one stream do not overlap), so no separate reset launch is
+// needed; the launchers flip `cur`.  Both sets must be zero before the first launch.
+#define GS_TILE_QUEUE (GS_XCDS * GS_XCD_STRIDE)
+struct TileQueues {
+    int* base;  // 2 * GS_TILE_QUEUE ints
+    int cur;
+};
 hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                       KeySpace ks, int* key_count, int* tile_counters, bool saturate,
+                       KeySpace ks, int* key_count, TileQueues& tq, bool saturate,
                        hipStream_t s);
 // saturate: the launches adding to key_count may process more than INT_MAX items in all;
 // a count that passes INT_MAX then reads INT_MIN (otherwise counts are exact 32-bit sums).
@@ -166,16 +174,18 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
                       KeySpace ks, int* key_count, const int* key_slot, int* cursor,
                       const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                      int* overflow, int* tile_counters, int nlive, bool saturate,
+                      int* overflow, TileQueues& tq, int nlive, bool saturate,
                       hipStream_t s);
 // resident workgroups per CU of the count (fill = false) or fill kernel at a tile size
 int items_blocks_per_cu(bool fill, int tile);
 
 // Keys with a non-zero count, in ascending key order, with their counts (flags / pos: K
-// and K+1 elements; scan_tmp as for scan_exclusive over K); *n_active is written.
+// and K+1 elements; scan_tmp as for scan_exclusive over K); *n_active is written and
+// *clear_word (nullable) zeroed.
 hipError_t compact_active_ordered(const int* key_count, long long K, int* active_keys,
-                                  int* active_counts, int* n_active, int* flags, long long* pos,
-                                  long long* scan_tmp, long long scan_tmp_elems, hipStream_t s);
+                                  int* active_counts, int* n_active, int* clear_word, int* flags,
+                                  long long* pos, long long* scan_tmp, long long scan_tmp_elems,
+                                  hipStream_t s);
 hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_keys,
                  const int* n_active, int max_active, int* flags, int* err, hipStream_t s);
 hipError_t canon_codes(const int* codes, int ncodes, int* flags, int* err, hipStream_t s);

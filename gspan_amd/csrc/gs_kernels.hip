@@ -550,11 +550,14 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     const int* __restrict__ cap_end,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
                                                     uint32_t* pool, int* overflow,
-                                                    int* tile_counters, int dedup,
-                                                    int saturate) {
+                                                    int* tile_counters, int* next_counters,
+                                                    int dedup, int saturate) {
     constexpr int LDS_SLOTS = 2 * TILE;
     constexpr int ITEMS_PER_THREAD = TILE / GS_BLOCK;
     static_assert(TILE % GS_BLOCK == 0 && (TILE & (TILE - 1)) == 0, "tile: power of two");
+    static_assert(GS_TILE_QUEUE <= GS_BLOCK, "one thread per queue counter");
+    // the queues of the next launch (the previous launch used them and has ended)
+    if (blockIdx.x == 0 && threadIdx.x < GS_TILE_QUEUE) next_counters[threadIdx.x] = 0;
     __shared__ long long s_off[TILE + 1];
     __shared__ int s_key[LDS_SLOTS];
     __shared__ int s_cnt[LDS_SLOTS];
@@ -787,7 +790,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_active_scatter(const int* key_coun
 // per thread, and places them after the keys of the earlier passes.
 __global__ __launch_bounds__(GS_BLOCK) void k_compact_small(const int* key_count, int K,
                                                             int* active, int* active_counts,
-                                                            int* n_active) {
+                                                            int* n_active, int* clear_word) {
     constexpr int PER = 16;
     long long placed = 0;  // uniform: every thread adds the same pass totals
     for (int base = 0; base < K; base += GS_BLOCK * PER) {
@@ -811,7 +814,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_compact_small(const int* key_count
         }
         placed += total;
     }
-    if (threadIdx.x == 0) *n_active = (int)placed;
+    if (threadIdx.x == 0) {
+        *n_active = (int)placed;
+        if (clear_word) *clear_word = 0;
+    }
 }
 
 __global__ void k_scatter_slots(int* key_slot, const int* keys, int nkeys) {
@@ -1308,14 +1314,14 @@ hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* 
                         const RowCtx* ctx, const long long* woff, long long b0, long long R,
                         KeySpace ks, int* key_count, const int* key_slot, int* cursor,
                         const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                        int* overflow, int* tile_counters, int dedup, int saturate,
+                        int* overflow, TileQueues& tq, int dedup, int saturate,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, tile_counters, 0,
-                       (long long)GS_XCDS * GS_XCD_STRIDE);
+    int* tile_counters = tq.base + tq.cur * GS_TILE_QUEUE;
+    int* next_counters = tq.base + (1 - tq.cur) * GS_TILE_QUEUE;
 #define GS_LAUNCH_ITEMS(T)                                                                     \
     hipLaunchKernelGGL((k_items<FILL, T>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, \
                        woff, b0, R, ks, key_count, key_slot, cursor, cap_end, rows_out, sets,  \
-                       pool, overflow, tile_counters, dedup, saturate)
+                       pool, overflow, tile_counters, next_counters, dedup, saturate)
     switch (lc.tile) {
         case 256: GS_LAUNCH_ITEMS(256); break;
         case 512: GS_LAUNCH_ITEMS(512); break;
@@ -1323,16 +1329,18 @@ hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* 
         default: return hipErrorInvalidValue;
     }
 #undef GS_LAUNCH_ITEMS
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) tq.cur = 1 - tq.cur;
+    return e;
 }
 
 hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                       KeySpace ks, int* key_count, int* tile_counters, bool saturate,
+                       KeySpace ks, int* key_count, TileQueues& tq, bool saturate,
                        hipStream_t s) {
     if (R <= 0) return hipSuccess;
     return launch_items<false>(lc, g, pats, ctx, woff, b0, R, ks, key_count, nullptr, nullptr,
-                               nullptr, nullptr, nullptr, nullptr, nullptr, tile_counters, 0,
+                               nullptr, nullptr, nullptr, nullptr, nullptr, tq, 0,
                                saturate ? 1 : 0, s);
 }
 
@@ -1340,13 +1348,13 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
                       KeySpace ks, int* key_count, const int* key_slot, int* cursor,
                       const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                      int* overflow, int* tile_counters, int nlive, bool saturate,
+                      int* overflow, TileQueues& tq, int nlive, bool saturate,
                       hipStream_t s) {
     if (R <= 0) return hipSuccess;
     // pair keys slot * TILE + row must stay below 2^31
     const int dedup = (long long)nlive * lc.tile < (1LL << 31) ? 1 : 0;
     return launch_items<true>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot, cursor,
-                              cap_end, rows_out, sets, pool, overflow, tile_counters, dedup,
+                              cap_end, rows_out, sets, pool, overflow, tq, dedup,
                               saturate ? 1 : 0, s);
 }
 
@@ -1370,17 +1378,16 @@ int items_blocks_per_cu(bool fill, int tile) {
 }
 
 hipError_t compact_active_ordered(const int* key_count, long long K, int* active_keys,
-                                  int* active_counts, int* n_active, int* flags, long long* pos,
-                                  long long* scan_tmp, long long scan_tmp_elems, hipStream_t s) {
-    if (K <= 0) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, n_active, 0, 1LL);
+                                  int* active_counts, int* n_active, int* clear_word, int* flags,
+                                  long long* pos, long long* scan_tmp, long long scan_tmp_elems,
+                                  hipStream_t s) {
+    if (K <= GS_COMPACT_SMALL) {  // also K == 0: n_active = 0
+        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(GS_BLOCK), 0, s, key_count,
+                           (int)(K > 0 ? K : 0), active_keys, active_counts, n_active,
+                           clear_word);
         return hipGetLastError();
     }
-    if (K <= GS_COMPACT_SMALL) {
-        hipLaunchKernelGGL(k_compact_small, dim3(1), dim3(GS_BLOCK), 0, s, key_count, (int)K,
-                           active_keys, active_counts, n_active);
-        return hipGetLastError();
-    }
+    if (clear_word) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, clear_word, 0, 1LL);
     const unsigned nb = (unsigned)grid_for(K, GS_BLOCK, 1 << 20);
     hipLaunchKernelGGL(k_active_flags, dim3(nb), dim3(GS_BLOCK), 0, s, key_count, K, flags);
     hipError_t e = scan_exclusive(flags, pos, K, scan_tmp, scan_tmp_elems, s);

@@ -310,7 +310,12 @@ struct gs_ctx {
     long long log_cap = 0, n_logged = 0;
     DBuf d_logX, d_logT;
 
-    int* tile_counters() const { return d_tiles.as<int>(); }
+    gsk::TileQueues tq{};  // over d_tiles
+    // zeroes both tile-queue sets (at the start of every entry point that extends)
+    void reset_tile_queues() {
+        tq = gsk::TileQueues{d_tiles.as<int>(), 0};
+        HIPCHECK(gsk::fill_i32(tq.base, 0, 2 * GS_TILE_QUEUE, stream));
+    }
     int* n_active() const { return d_misc.as<int>() + 1; }
     int* err_flag() const { return d_misc.as<int>() + 2; }
     int* overflow_flag() const { return d_misc.as<int>() + 3; }
@@ -661,13 +666,14 @@ void fetch_active(gs_ctx* c, long long K, const int* d_active, const int* d_acnt
 }
 
 // Keys with a non-zero count, in ascending key order, into active / acnt; *n_active.
+// The error word of the canonicality check that follows is zeroed on the way.
 void compact_ordered(gs_ctx* c, const int* key_count, long long K, int* active, int* acnt) {
     DevPool* P = &c->pool;
     const long long te = gsk::scan_tmp_elems(K);
     DBuf flags(P, sizeof(int) * std::max(K, 1LL)), pos(P, sizeof(long long) * (K + 1)),
         tmp(P, sizeof(long long) * te);
     HIPCHECK(gsk::compact_active_ordered(key_count, K, active, acnt, c->n_active(),
-                                         flags.as<int>(), pos.as<long long>(),
+                                         c->err_flag(), flags.as<int>(), pos.as<long long>(),
                                          tmp.as<long long>(), te, c->stream));
 }
 
@@ -960,13 +966,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                          nullptr, d_cur.as<int>(), d_end.as<int>(),
                                          rows.as<int2>(), d_psets.as<SetDesc>(),
                                          d_ppool.as<uint32_t>(), c->overflow_flag(),
-                                         c->tile_counters(), (int)K, saturate, c->stream));
+                                         c->tq, (int)K, saturate, c->stream));
                 time_end(1, &Wc_p[ci]);
                 c->st.fill_launches++;
             }
             DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K),
                 d_flags(P, sizeof(int) * K);
-            HIPCHECK(gsk::fill_i32(c->n_active(), 0, 2, c->stream));
             compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
             HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(),
                                 (int)K, d_flags.as<int>(), c->err_flag(), c->stream));
@@ -1064,13 +1069,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
                                      nullptr, nullptr, nullptr, d_all_sets.as<SetDesc>(),
                                      d_all_pool.as<uint32_t>(), c->overflow_flag(),
-                                     c->tile_counters(), (int)K, saturate, c->stream));
+                                     c->tq, (int)K, saturate, c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
         } else {
             HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                       d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
-                                      c->tile_counters(), saturate, c->stream));
+                                      c->tq, saturate, c->stream));
             time_end(0, &Wc_host[ci]);
             c->st.count_launches++;
         }
@@ -1095,7 +1100,6 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
 
     // ---- candidates and their canonicality
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
-    HIPCHECK(gsk::fill_i32(c->n_active(), 0, 2, c->stream));  // n_active and err
     compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
     HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
                         d_flags.as<int>(), c->err_flag(), c->stream));
@@ -1221,7 +1225,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      d_slot.as<int>(), d_cursor.as<int>(), nullptr,
                                      write_rows ? out.rows.ptr() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
-                                     c->overflow_flag(), c->tile_counters(), NL, false,
+                                     c->overflow_flag(), c->tq, NL, false,
                                      c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
@@ -1320,7 +1324,6 @@ void mine_seeds(gs_ctx* c) {
     if (K > (1LL << 30)) throw GsError(GS_ERR_LIMIT, "too many labels for the seed key space");
     DBuf d_kc(P, sizeof(int) * K);
     HIPCHECK(gsk::fill_i32(d_kc.as<int>(), 0, K, c->stream));
-    HIPCHECK(gsk::fill_i32(c->n_active(), 0, 1, c->stream));
     HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->total64()), 0, 2, c->stream));
     HIPCHECK(gsk::seed_count(c->dg, c->EL, c->VL, d_kc.as<int>(), c->total64(), c->stream));
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K);
@@ -1631,7 +1634,7 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         if (!lstart.empty()) x->d_lstart = DBuf(P, sizeof(int) * lstart.size());
         x->d_vlab = DBuf(P, sizeof(int) * std::max(n, 1));
         x->d_misc = DBuf(P, 64);
-        x->d_tiles = DBuf(P, sizeof(int) * GS_XCDS * GS_XCD_STRIDE);
+        x->d_tiles = DBuf(P, sizeof(int) * 2 * GS_TILE_QUEUE);
         h2d(x, x->d_offsets.as<int>(), offsets, n + 1);
         h2d(x, x->d_src.as<int>(), src.data(), nnz);
         h2d(x, x->d_adj.as<uint2>(), adj.data(), nnz);
@@ -1675,6 +1678,7 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->min_sup = min_support;
         c->mni_mode = c->opts.mni_mode;
         c->reports.clear();
+        c->reset_tile_queues();
         mine_seeds(c);
         std::stable_sort(c->reports.begin(), c->reports.end(),
                          [](const Report& a, const Report& b) { return a.order < b.order; });
@@ -1733,6 +1737,7 @@ int gs_extend(gs_ctx* c, const int* code5, int m, const int* const* layers,
         std::vector<const int2*> ptrs;
         std::vector<DBuf> bufs = upload_layers(c, m, layers, counts, ptrs);
         c->mni_mode = mni_mode;
+        c->reset_tile_queues();
         std::vector<HPat> one{h};
         StepOut so;
         if (run_step(c, one, ptrs, true, true, so) != STEP_OK)
