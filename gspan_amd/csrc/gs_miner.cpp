@@ -190,6 +190,14 @@ struct HPat {
     }
 };
 
+struct ChildInfo {
+    HPat pat;
+    long long count;
+    int canonical;
+    int support;
+    long long row_off;  // in the child row array (-1 when not written)
+};
+
 // Position of a pattern in the reference's report order (DFS pre-order of mine()): a
 // sequence of four numbers per code edge, compared lexicographically, a prefix first.
 struct OrderKey {
@@ -311,6 +319,9 @@ struct gs_ctx {
     DBuf d_logX, d_logT;
 
     gsk::TileQueues tq{};  // over d_tiles
+    // child arrays of finished steps, reused by later ones: a level's array is ~170 bytes
+    // per candidate, and fresh memory for it every step costs page faults
+    std::vector<std::vector<ChildInfo>> child_bufs;
     // zeroes both tile-queue sets (at the start of every entry point that extends)
     void reset_tile_queues() {
         tq = gsk::TileQueues{d_tiles.as<int>(), 0};
@@ -417,8 +428,8 @@ PatDev to_dev(const HPat& h) {
 }
 
 // Child pattern of parent p for a slot of its key space (see KeySpace in gs_internal.h).
-HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
-    HPat c = p;
+void decode_child_into(const HPat& p, const PatDev& d, int slot, int EL, int VL, HPat& c) {
+    c = p;
     c.m = p.m + 1;
     const int j = p.m;
     const int nb = (d.r - 1) * EL;
@@ -436,6 +447,11 @@ HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
         c.el[j] = (s / VL) % EL;
         c.tl[j] = s % VL;
     }
+}
+
+HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
+    HPat c;
+    decode_child_into(p, d, slot, EL, VL, c);
     return c;
 }
 
@@ -683,14 +699,6 @@ void compact_ordered(gs_ctx* c, const int* key_count, long long K, int* active, 
 // (supports come from the image sets, not the counts).
 inline long long child_count(int c) { return c < 0 ? (long long)INT_MAX + 1 : c; }
 
-struct ChildInfo {
-    HPat pat;
-    long long count;
-    int canonical;
-    int support;
-    long long row_off;  // in the child row array (-1 when not written)
-};
-
 // Rows of one DFS level: in device memory, or -- when they do not fit the device
 // budget -- spilled to pinned host memory that the kernels reach through its device
 // mapping (slower, over the host link, but the level can be mined).
@@ -920,10 +928,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         std::vector<int> row_off(K), cap_end(K);
         std::vector<SetDesc> psets(K);
         long long rows_total = 0, pool_words = 0;
+        int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
         for (long long key = 0; key < K; ++key) {
-            int p = int(std::upper_bound(pd.begin(), pd.end(), (int)key,
-                                         [](int kk, const PatDev& d) { return kk < d.key_base; }) -
-                        pd.begin()) - 1;
+            while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
             const int slot = (int)(key - pd[p].key_base);
             const int nv_c = pd[p].nv + (slot < (pd[p].r - 1) * c->EL ? 0 : 1);
             long long cap = 0;
@@ -997,14 +1004,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 for (int i = 0; i < A; ++i) order[i] = i;
                 std::vector<SetDesc> live_sets;
                 long long emitted = 0;
+                int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
                 for (int oi = 0; oi < A; ++oi) {
                     const int a = order[oi];
                     const int key = akeys[a];
-                    int p = int(std::upper_bound(pd.begin(), pd.end(), key,
-                                                 [](int kk, const PatDev& d) {
-                                                     return kk < d.key_base;
-                                                 }) -
-                                pd.begin()) - 1;
+                    while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
                     ChildInfo ci;
                     if (aflag[a] || all_live)  // other children only count as candidates
                         ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
@@ -1125,24 +1129,29 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     std::vector<int> live_keys, live_child;
     std::vector<int> cursor;
     std::vector<SetDesc> sets;
+    if (!c->child_bufs.empty()) {
+        out.children = std::move(c->child_bufs.back());
+        c->child_bufs.pop_back();
+        out.children.clear();
+    }
     out.children.reserve(A);
     long long rows_total = 0, pool_words = 0, emitted = 0;
+    int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
     for (int oi = 0; oi < A; ++oi) {
         const int a = order[oi];
         const int key = akeys[a];
-        int p = int(std::upper_bound(pd.begin(), pd.end(), key,
-                                     [](int k, const PatDev& d) { return k < d.key_base; }) -
-                    pd.begin()) - 1;
-        ChildInfo ci;
+        while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
+        out.children.emplace_back();  // built in place: a ChildInfo is ~170 bytes
+        ChildInfo& ci = out.children.back();
         if (aflag[a] || all_live)  // other children only count as candidates
-            ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
+            decode_child_into(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL, ci.pat);
         ci.count = child_count(acnt[a]);
         ci.canonical = aflag[a];
         ci.support = 0;
         ci.row_off = -1;
         emitted += ci.count;
         if (ci.canonical || all_live) {
-            live_child.push_back((int)out.children.size());
+            live_child.push_back((int)out.children.size() - 1);
             live_keys.push_back(key);
             if (write_rows) {
                 ci.row_off = rows_total;
@@ -1152,7 +1161,6 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 cursor.push_back(0);
             }
         }
-        out.children.push_back(ci);
     }
     out.emitted = emitted;
     const int NL = (int)live_keys.size();
@@ -1296,6 +1304,7 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers
         mine_level(c, next, layers);
         layers.pop_back();
     }
+    if (c->child_bufs.size() < 16) c->child_bufs.push_back(std::move(so.children));
 }
 
 void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers) {
