@@ -198,15 +198,17 @@ struct ChildInfo {
     long long row_off;  // in the child row array (-1 when not written)
 };
 
-// Position of a pattern in the reference's report order (DFS pre-order of mine()): a
-// sequence of four numbers per code edge, compared lexicographically, a prefix first.
+// Position of a pattern in the reference's report order (DFS pre-order of mine()): four
+// numbers per code edge, compared lexicographically, a prefix first.  The four numbers of
+// an edge are packed into one 64-bit word, 16 bits each, by order-preserving maps (label
+// ranks are below 2^16, vertex ids within +-8), so a comparison reads one word per edge.
 struct OrderKey {
-    int n = 0;
-    int v[4 * GS_MAXE];
+    int n = 0;  // edges
+    uint64_t w[GS_MAXE];
     bool operator<(const OrderKey& o) const {
         const int k = std::min(n, o.n);
         for (int i = 0; i < k; ++i)
-            if (v[i] != o.v[i]) return v[i] < o.v[i];
+            if (w[i] != o.w[i]) return w[i] < o.w[i];
         return n < o.n;
     }
 };
@@ -460,18 +462,30 @@ HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
 // a pattern precedes its descendants (gspan.cu:1142-1168, 1185).
 OrderKey order_key(const HPat& h) {
     OrderKey k;
+    // a, b, c, d each mapped into [0, 2^16) keeping their order
     auto put = [&](int a, int b, int c, int d) {
-        k.v[k.n++] = a;
-        k.v[k.n++] = b;
-        k.v[k.n++] = c;
-        k.v[k.n++] = d;
+        k.w[k.n++] = (uint64_t)a << 48 | (uint64_t)b << 32 | (uint64_t)c << 16 | (uint64_t)d;
     };
-    put(h.fl0, h.el[0], h.tl[0], 0);
+    put(h.fl0, h.el[0], h.tl[0], 0);  // label ranks
     for (int j = 1; j < h.m; ++j) {
-        if (h.f[j] > h.t[j]) put(0, h.t[j], h.el[j], 0);
-        else put(1, -h.f[j], h.el[j], h.tl[j]);
+        // backward: (0, to, el, 0); forward: (1, -from, el, tl); vertex ids shifted by 8
+        if (h.f[j] > h.t[j]) put(0, h.t[j] + 8, h.el[j], 0);
+        else put(1, 8 - h.f[j], h.el[j], h.tl[j]);
     }
     return k;
+}
+
+// Reports into the reference's order: indices are sorted (a Report is ~200 bytes), then the
+// reports moved once.  Distinct patterns have distinct keys, so the order is total.
+void sort_reports(gs_ctx* c) {
+    std::vector<Report>& r = c->reports;
+    std::vector<int> idx(r.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return r[a].order < r[b].order; });
+    std::vector<Report> sorted;
+    sorted.reserve(r.size());
+    for (int i : idx) sorted.push_back(std::move(r[i]));
+    r.swap(sorted);
 }
 
 void add_report(gs_ctx* c, const HPat& h) {
@@ -1689,8 +1703,7 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->reports.clear();
         c->reset_tile_queues();
         mine_seeds(c);
-        std::stable_sort(c->reports.begin(), c->reports.end(),
-                         [](const Report& a, const Report& b) { return a.order < b.order; });
+        sort_reports(c);
         auto t1 = std::chrono::steady_clock::now();
         c->st.t_mine_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         c->st.peak_device_bytes = c->pool.peak();
