@@ -37,7 +37,7 @@ WORKLOADS = {
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per fill launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this bench,
 # scaled by a 1 GiB copy's readings (profiles/k_items_er10k_k4.md); None where not measured
-FILL_TRAFFIC_BYTES = {"er10k": 434_338_583}
+FILL_TRAFFIC_BYTES = {"er10k": 431_515_558}
 
 
 def log(msg: str) -> None:
