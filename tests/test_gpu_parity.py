@@ -336,3 +336,19 @@ def test_child_count_past_int_max(oracle_mod):
         with pytest.raises(L.GsError, match="2\\^31") as ei:
             mn.extend(path, [l0, rows])
         assert ei.value.code == L.GS_ERR_BUDGET
+
+
+def test_large_key_space(oracle_mod):
+    """A level whose child key space is far above what one workgroup compacts (128k keys):
+    60 vertex labels and 4 edge labels at min_sup 1 make ~7,000 frequent edges with 480
+    forward keys each (3.2M keys in the batched step), so the ordered compaction takes its
+    multi-launch path; plain steps stay on the one-workgroup path.  Both modes give the
+    oracle's report."""
+    g = synthetic("er", n=3000, m=20000, n_vlabels=60, n_elabels=4, seed=5)
+    want, st = oracle_mod.fsm(g, 2, 1)
+    assert len(want) > 200_000
+    for mode in ("batched", "plain"):
+        _announce(f"large key space mode={mode}")
+        with Miner(g, mode=mode) as mn:
+            assert mn.mine(2, 1) == want, mode
+            assert mn.stats()["extended"] == st["extended"], mode
