@@ -1318,7 +1318,10 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers
         mine_level(c, next, layers);
         layers.pop_back();
     }
-    if (c->child_bufs.size() < 16) c->child_bufs.push_back(std::move(so.children));
+    // kept for reuse: one array per DFS level (at most 7 are alive at once), each of at
+    // most 2^18 candidates (~45 MB); larger ones are freed
+    if (c->child_bufs.size() < 8 && so.children.capacity() <= (1u << 18))
+        c->child_bufs.push_back(std::move(so.children));
 }
 
 void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers) {
