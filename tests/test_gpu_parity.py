@@ -352,3 +352,29 @@ def test_large_key_space(oracle_mod):
         with Miner(g, mode=mode) as mn:
             assert mn.mine(2, 1) == want, mode
             assert mn.stats()["extended"] == st["extended"], mode
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_graphs_all_modes(oracle_mod, seed):
+    """Seeded ER and small R-MAT graphs with random label counts, k, threshold and MNI
+    semantics: batched, plain and bitmap-set runs all give the oracle's report and
+    `extended` total."""
+    rng = np.random.default_rng(7000 + seed)
+    if seed % 3:
+        nv = int(rng.integers(30, 400))
+        m = int(rng.integers(nv, 4 * nv))
+        g = synthetic("er", n=nv, m=m, n_vlabels=int(rng.integers(1, 6)),
+                      n_elabels=int(rng.integers(1, 4)), seed=seed)
+    else:
+        g = synthetic("rmat", scale=int(rng.integers(6, 10)), edge_factor=4,
+                      n_vlabels=int(rng.integers(1, 6)), vlabel_zipf=1.1,
+                      n_elabels=int(rng.integers(1, 3)), seed=seed)
+    k = int(rng.integers(2, 5))
+    ms = int(rng.integers(2, 12))
+    mni = int(rng.integers(0, 2))
+    _announce(f"random graph seed={seed} k={k} ms={ms} mni={mni}")
+    want, st = oracle_mod.fsm(g, k, ms, mni_mode=mni)
+    for mode, sets in (("batched", "auto"), ("plain", "auto"), ("batched", "bits")):
+        with Miner(g, mode=mode, mni=("gpu", "omp-literal")[mni], sets=sets) as mn:
+            assert mn.mine(k, ms) == want, (mode, sets)
+            assert mn.stats()["extended"] == st["extended"], (mode, sets)
