@@ -20,8 +20,13 @@
 #define GS_XCDS 8
 #define GS_XCD_STRIDE 16
 
-// Canonicality kernel: one wave per candidate, projections double-buffered in LDS.
+// Canonicality kernel: one wave per candidate, projections double-buffered in LDS.  The
+// walk of a code with m edges carries at most as many projections as a connected subgraph
+// of the pattern has embeddings in it; the most is 7!/(7-i)! for a single-label 7-edge star
+// (5040), 720 for 6 edges.  Codes of up to 6 edges use the small buffers, 7-edge codes the
+// large ones.
 #define GS_CANON_CAP 2048
+#define GS_CANON_CAP7 5120
 
 // Child-count predictor: MLP GS_FEATURES -> GS_HIDDEN -> GS_HIDDEN -> 1.
 #define GS_FEATURES 32
@@ -168,14 +173,15 @@ hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* p
 // saturate: the launches adding to key_count may process more than INT_MAX items in all;
 // a count that passes INT_MAX then reads INT_MIN (otherwise counts are exact 32-bit sums).
 // key_count (nullable): also count children per key; key_slot == nullptr: slot = key.
-// cap_end (nullable): rows of slot s may only be written below cap_end[s]; a child that
-// does not fit, or a hash set that fills up, sets *overflow instead (predicted sizes).
+// cursor[s]: next row of slot s (64-bit, so an overrun cannot wrap).  cap_end (nullable):
+// rows of slot s may only be written below cap_end[s]; a child that does not fit, or a hash
+// set that fills up, sets *overflow instead and writes nothing (predicted sizes).
 hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                      KeySpace ks, int* key_count, const int* key_slot, int* cursor,
-                      const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                      int* overflow, TileQueues& tq, int nlive, bool saturate,
-                      hipStream_t s);
+                      KeySpace ks, int* key_count, const int* key_slot,
+                      unsigned long long* cursor, const int* cap_end, int2* rows_out,
+                      const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
+                      int nlive, bool saturate, hipStream_t s);
 // resident workgroups per CU of the count (fill = false) or fill kernel at a tile size
 int items_blocks_per_cu(bool fill, int tile);
 
@@ -186,9 +192,12 @@ hipError_t compact_active_ordered(const int* key_count, long long K, int* active
                                   int* active_counts, int* n_active, int* clear_word, int* flags,
                                   long long* pos, long long* scan_tmp, long long scan_tmp_elems,
                                   hipStream_t s);
+// child_m: edges of the candidate codes (parent size + 1); max_m: longest code given
 hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_keys,
-                 const int* n_active, int max_active, int* flags, int* err, hipStream_t s);
-hipError_t canon_codes(const int* codes, int ncodes, int* flags, int* err, hipStream_t s);
+                 const int* n_active, int max_active, int child_m, int* flags, int* err,
+                 hipStream_t s);
+hipError_t canon_codes(const int* codes, int ncodes, int max_m, int* flags, int* err,
+                       hipStream_t s);
 
 hipError_t fill_i32(int* p, int v, long long n, hipStream_t s);
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);

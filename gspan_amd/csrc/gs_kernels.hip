@@ -546,7 +546,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     const long long* __restrict__ woff,
                                                     long long b0, long long R, KeySpace ks,
                                                     int* key_count,
-                                                    const int* __restrict__ key_slot, int* cursor,
+                                                    const int* __restrict__ key_slot,
+                                                    unsigned long long* cursor,
                                                     const int* __restrict__ cap_end,
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
                                                     uint32_t* pool, int* overflow,
@@ -693,7 +694,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
             const int key = s_key[h];
             if (key < 0) continue;
-            if (FILL && rows_out) s_base[h] = atomicAdd(&cursor[key], s_cnt[h]);
+            if (FILL && rows_out) {
+                // 64-bit cursors cannot wrap however far a key overruns a predicted
+                // capacity; a reservation that does not fit below cap_end writes nothing
+                const unsigned long long at =
+                    atomicAdd(&cursor[key], (unsigned long long)s_cnt[h]);
+                const bool fits = !cap_end || at + (unsigned long long)s_cnt[h] <=
+                                                  (unsigned long long)cap_end[key];
+                s_base[h] = fits ? (int)at : -1;
+                if (!fits) atomicOr(overflow, 1);
+            }
             if (key_count) {
                 const int ph = lds_slot_fill<LDS_SLOTS>(s_pkey, key, &s_pfill);
                 atomicAdd(&s_pcnt[ph], s_cnt[h]);
@@ -724,11 +734,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 const int slot = s_key[hs[q]];
                 const RowCtx& c = ctx[rlo + row[q]];
                 const PatDev& P = pats[pat[q]];
-                if (rows_out) {
+                if (rows_out && s_base[hs[q]] >= 0) {
                     const long long prow = P.row_begin + (b0 + rlo + row[q] - P.brow_begin);
-                    const int at = s_base[hs[q]] + rank[q];
-                    if (!cap_end || at < cap_end[slot]) rows_out[at] = make_int2((int)prow, o[q].eid);
-                    else atomicOr(overflow, 1);
+                    rows_out[s_base[hs[q]] + rank[q]] = make_int2((int)prow, o[q].eid);
                 }
                 const SetDesc d = sets[slot];
                 const int nv = P.nv;
@@ -962,7 +970,9 @@ __device__ __forceinline__ int canon_backward(const CanonCode& C, uint32_t p, in
     return -1;
 }
 
-// Returns 1 canonical, 0 not; err bit 0: projection buffer overflow, bit 1: walk overran.
+// Returns 1 canonical, 0 not; err bit 0: projection buffer overflow (more than CAP
+// projections), bit 1: walk overran.
+template <int CAP>
 __device__ int canon_wave(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, int* mf, int* mt,
                           int* mel, int* err) {
     const int lane = lane_id();
@@ -1049,11 +1059,11 @@ __device__ int canon_wave(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, in
                 unsigned long long km = __ballot(keep);
                 if (keep) {
                     int at = np + (int)mbcnt64(km);
-                    if (at < GS_CANON_CAP) nxt[at] = pr;
+                    if (at < CAP) nxt[at] = pr;
                 }
                 np += __popcll(km);
             }
-            if (np > GS_CANON_CAP) {
+            if (np > CAP) {
                 *err |= 1;
                 return 1;
             }
@@ -1143,10 +1153,10 @@ __device__ int canon_wave(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, in
             int tot;
             int at = np + wave_excl_sum(cnt, &tot);
             for (int q = 0; q < cnt; ++q)
-                if (at + q < GS_CANON_CAP) nxt[at + q] = outs[q];
+                if (at + q < CAP) nxt[at + q] = outs[q];
             np += tot;
         }
-        if (np > GS_CANON_CAP) {
+        if (np > CAP) {
             *err |= 1;
             return 1;
         }
@@ -1193,10 +1203,11 @@ __device__ void decode_child(const PatDev& Pp, int slot, KeySpace ks, CanonCode&
     }
 }
 
+template <int CAP>
 __global__ __launch_bounds__(64) void k_canon(const PatDev* __restrict__ pats, int npats,
                                               KeySpace ks, const int* active_keys,
                                               const int* n_active, int* flags, int* err) {
-    __shared__ uint32_t bufA[GS_CANON_CAP], bufB[GS_CANON_CAP];
+    __shared__ uint32_t bufA[CAP], bufB[CAP];
     __shared__ CanonCode C;
     __shared__ int mf[GS_MAXE], mt[GS_MAXE], mel[GS_MAXE];
     const int na = *n_active;
@@ -1213,7 +1224,7 @@ __global__ __launch_bounds__(64) void k_canon(const PatDev* __restrict__ pats, i
         }
         __syncthreads();
         int e = 0;
-        int f = canon_wave(C, bufA, bufB, mf, mt, mel, &e);
+        int f = canon_wave<CAP>(C, bufA, bufB, mf, mt, mel, &e);
         if (threadIdx.x == 0) {
             flags[a] = f;
             if (e) atomicOr(err, e);
@@ -1224,9 +1235,10 @@ __global__ __launch_bounds__(64) void k_canon(const PatDev* __restrict__ pats, i
 
 // Codes given explicitly: codes[i*(1+5*GS_MAXE)] = m, then (from, to, from_label, edge_label,
 // to_label) per edge in label-rank space, -1 where the reference stores no label.
+template <int CAP>
 __global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes, int* flags,
                                                     int* err) {
-    __shared__ uint32_t bufA[GS_CANON_CAP], bufB[GS_CANON_CAP];
+    __shared__ uint32_t bufA[CAP], bufB[CAP];
     __shared__ CanonCode C;
     __shared__ int mf[GS_MAXE], mt[GS_MAXE], mel[GS_MAXE];
     for (int a = blockIdx.x; a < ncodes; a += gridDim.x) {
@@ -1246,7 +1258,7 @@ __global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes
         }
         __syncthreads();
         int e = 0;
-        int f = canon_wave(C, bufA, bufB, mf, mt, mel, &e);
+        int f = canon_wave<CAP>(C, bufA, bufB, mf, mt, mel, &e);
         if (threadIdx.x == 0) {
             flags[a] = f;
             if (e) atomicOr(err, e);
@@ -1312,9 +1324,10 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
 template <bool FILL>
 hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                         const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                        KeySpace ks, int* key_count, const int* key_slot, int* cursor,
-                        const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                        int* overflow, TileQueues& tq, int dedup, int saturate,
+                        KeySpace ks, int* key_count, const int* key_slot,
+                        unsigned long long* cursor, const int* cap_end, int2* rows_out,
+                        const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
+                        int dedup, int saturate,
                         hipStream_t s) {
     int* tile_counters = tq.base + tq.cur * GS_TILE_QUEUE;
     int* next_counters = tq.base + (1 - tq.cur) * GS_TILE_QUEUE;
@@ -1346,10 +1359,10 @@ hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* p
 
 hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
-                      KeySpace ks, int* key_count, const int* key_slot, int* cursor,
-                      const int* cap_end, int2* rows_out, const SetDesc* sets, uint32_t* pool,
-                      int* overflow, TileQueues& tq, int nlive, bool saturate,
-                      hipStream_t s) {
+                      KeySpace ks, int* key_count, const int* key_slot,
+                      unsigned long long* cursor, const int* cap_end, int2* rows_out,
+                      const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
+                      int nlive, bool saturate, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     // pair keys slot * TILE + row must stay below 2^31
     const int dedup = (long long)nlive * lc.tile < (1LL << 31) ? 1 : 0;
@@ -1398,18 +1411,29 @@ hipError_t compact_active_ordered(const int* key_count, long long K, int* active
 }
 
 hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_keys,
-                 const int* n_active, int max_active, int* flags, int* err, hipStream_t s) {
+                 const int* n_active, int max_active, int child_m, int* flags, int* err,
+                 hipStream_t s) {
     if (max_active <= 0) return hipSuccess;
     int grid = max_active < 16384 ? max_active : 16384;
-    hipLaunchKernelGGL(k_canon, dim3(grid), dim3(64), 0, s, pats, npats, ks, active_keys, n_active,
-                       flags, err);
+    if (child_m > 6)
+        hipLaunchKernelGGL(k_canon<GS_CANON_CAP7>, dim3(grid), dim3(64), 0, s, pats, npats, ks,
+                           active_keys, n_active, flags, err);
+    else
+        hipLaunchKernelGGL(k_canon<GS_CANON_CAP>, dim3(grid), dim3(64), 0, s, pats, npats, ks,
+                           active_keys, n_active, flags, err);
     return hipGetLastError();
 }
 
-hipError_t canon_codes(const int* codes, int ncodes, int* flags, int* err, hipStream_t s) {
+hipError_t canon_codes(const int* codes, int ncodes, int max_m, int* flags, int* err,
+                       hipStream_t s) {
     if (ncodes <= 0) return hipSuccess;
     int grid = ncodes < 16384 ? ncodes : 16384;
-    hipLaunchKernelGGL(k_canon_codes, dim3(grid), dim3(64), 0, s, codes, ncodes, flags, err);
+    if (max_m > 6)
+        hipLaunchKernelGGL(k_canon_codes<GS_CANON_CAP7>, dim3(grid), dim3(64), 0, s, codes, ncodes,
+                           flags, err);
+    else
+        hipLaunchKernelGGL(k_canon_codes<GS_CANON_CAP>, dim3(grid), dim3(64), 0, s, codes, ncodes,
+                           flags, err);
     return hipGetLastError();
 }
 

@@ -783,8 +783,24 @@ struct StepOut {
     long long emitted = 0;
 };
 
+// error of a canonicality walk that outgrew its projection buffers (never expected: the
+// buffers hold the most projections any code of up to 7 edges can have)
+const char* const kCanonCapMsg = "canonicality check: more projections than the walk buffers hold";
+
 constexpr int STEP_OK = 0;
 constexpr int STEP_SPLIT = 1;
+
+// Row capacity of a child key from the predictor's output y = log1p(predicted count):
+// 1.25x the prediction plus 16, at most 2^30.  The clamp is applied in floating point
+// before the conversion, so every output -- negative, huge, infinite or NaN -- maps to a
+// defined capacity in [16, 2^30].
+long long predicted_capacity(float y) {
+    constexpr double kMax = (double)(1 << 30);
+    const double yd = (double)y;
+    if (!(yd == yd)) return (long long)kMax;  // NaN
+    const double pred = yd > 40.0 ? kMax : std::max(0.0, std::expm1(yd));
+    return (long long)std::min(std::ceil(1.25 * pred) + 16.0, kMax);
+}
 
 // One extension step of a batch of size-m patterns.  write_rows: keep child rows (needed
 // below the last level); all_live: write rows and sets of non-canonical children too.
@@ -938,8 +954,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             d2h(c, poss.data(), d_poss.as<unsigned char>(), K);
             sync(c);
         }
-        // capacities: 1.25x the prediction plus 16 rows; keys the rules exclude get none
-        std::vector<int> row_off(K), cap_end(K);
+        // capacities: 1.25x the prediction plus 16 rows, at most 2^30 (clamped in floating
+        // point before any conversion, so a diverged network or foreign weights -- huge,
+        // infinite or NaN outputs -- give large but well-defined capacities, and the step
+        // then takes the exact path below); keys the rules exclude get none
+        std::vector<unsigned long long> row_off(K);
+        std::vector<int> cap_end(K);
         std::vector<SetDesc> psets(K);
         long long rows_total = 0, pool_words = 0;
         int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
@@ -948,11 +968,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             const int slot = (int)(key - pd[p].key_base);
             const int nv_c = pd[p].nv + (slot < (pd[p].r - 1) * c->EL ? 0 : 1);
             long long cap = 0;
-            if (poss[key]) {
-                const double pred = std::max(0.0, std::expm1((double)y[key]));
-                cap = std::min<long long>(1LL << 30, (long long)std::ceil(1.25 * pred) + 16);
-            }
-            row_off[key] = (int)std::min<long long>(rows_total, INT_MAX);
+            if (poss[key]) cap = predicted_capacity(y[key]);
+            row_off[key] = (unsigned long long)std::min<long long>(rows_total, INT_MAX);
             rows_total += cap;
             cap_end[key] = (int)std::min<long long>(rows_total, INT_MAX);
             SetDesc& d = psets[key];
@@ -963,13 +980,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             }
         }
         const size_t need = (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
-                            (size_t)K * (sizeof(SetDesc) + 12);
+                            (size_t)K * (sizeof(SetDesc) + 16);
         if (rows_total <= INT_MAX && c->pool.in_use() + need <= c->budget) {
             DBuf rows(P, sizeof(int2) * std::max<long long>(rows_total, 1));
-            DBuf d_cur(P, sizeof(int) * K), d_end(P, sizeof(int) * K);
+            DBuf d_cur(P, sizeof(unsigned long long) * K), d_end(P, sizeof(int) * K);
             DBuf d_psets(P, sizeof(SetDesc) * K);
             DBuf d_ppool(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
-            h2d(c, d_cur.as<int>(), row_off.data(), K);
+            h2d(c, d_cur.as<unsigned long long>(), row_off.data(), K);
             h2d(c, d_end.as<int>(), cap_end.data(), K);
             h2d(c, d_psets.as<SetDesc>(), psets.data(), K);
             HIPCHECK(gsk::fill_u32(d_ppool.as<uint32_t>(), 0u, pool_words, c->stream));
@@ -984,7 +1001,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 time_begin();
                 HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                          d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
-                                         nullptr, d_cur.as<int>(), d_end.as<int>(),
+                                         nullptr, d_cur.as<unsigned long long>(), d_end.as<int>(),
                                          rows.as<int2>(), d_psets.as<SetDesc>(),
                                          d_ppool.as<uint32_t>(), c->overflow_flag(),
                                          c->tq, (int)K, saturate, c->stream));
@@ -995,7 +1012,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 d_flags(P, sizeof(int) * K);
             compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
             HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(),
-                                (int)K, d_flags.as<int>(), c->err_flag(), c->stream));
+                                (int)K, m + 1, d_flags.as<int>(), c->err_flag(), c->stream));
             int na_err_ov[3] = {0, 0, 0};  // n_active, err, overflow
             d2h(c, &W_full, c->work_full64(), 1);
             ActiveHost ah;
@@ -1006,7 +1023,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             c->st.items += Wp;
             c->st.parent_rows += R;
             if (na_err_ov[1] & 1)
-                throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
+                throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
             if (na_err_ov[1] & 2)
                 throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
             if (na_err_ov[2] == 0) {
@@ -1029,7 +1046,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                     ci.count = child_count(acnt[a]);
                     ci.canonical = aflag[a];
                     ci.support = 0;
-                    ci.row_off = row_off[key];
+                    ci.row_off = (long long)row_off[key];
                     emitted += ci.count;
                     if (ci.canonical) live_sets.push_back(psets[key]);
                     out.children.push_back(ci);
@@ -1120,7 +1137,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
     compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
     HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
-                        d_flags.as<int>(), c->err_flag(), c->stream));
+                        m + 1, d_flags.as<int>(), c->err_flag(), c->stream));
     int na_err[2] = {0, 0};
     d2h(c, &W_full, c->work_full64(), 1);
     ActiveHost ah;
@@ -1129,7 +1146,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     for (long long w : Wc_host) W_total += w;
     c->st.items += W_total;
     if (na_err[1] & 1)
-        throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
+        throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
     if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
     const int A = na_err[0];
     const std::vector<int>& akeys = ah.keys;
@@ -1141,7 +1158,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
 
     // ---- plan child rows and sets of the live keys
     std::vector<int> live_keys, live_child;
-    std::vector<int> cursor;
+    std::vector<unsigned long long> cursor;
     std::vector<SetDesc> sets;
     if (!c->child_bufs.empty()) {
         out.children = std::move(c->child_bufs.back());
@@ -1169,7 +1186,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             live_keys.push_back(key);
             if (write_rows) {
                 ci.row_off = rows_total;
-                cursor.push_back((int)rows_total);
+                cursor.push_back((unsigned long long)rows_total);
                 rows_total += ci.count;
             } else {
                 cursor.push_back(0);
@@ -1222,7 +1239,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     h2d(c, d_sets.as<SetDesc>(), sets.data(), NL);
     if (!single) {
         DBuf d_slot(P, sizeof(int) * K);
-        DBuf d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(int) * NL);
+        DBuf d_live(P, sizeof(int) * NL), d_cursor(P, sizeof(unsigned long long) * NL);
         d_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(pool_words, 4));
         if (write_rows) {
             if (spill) {
@@ -1234,7 +1251,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         }
         HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
         h2d(c, d_live.as<int>(), live_keys.data(), NL);
-        h2d(c, d_cursor.as<int>(), cursor.data(), NL);
+        h2d(c, d_cursor.as<unsigned long long>(), cursor.data(), NL);
         HIPCHECK(gsk::scatter_slots(d_slot.as<int>(), d_live.as<int>(), NL, c->stream));
         HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
         for (int ci = 0; ci < nchunks; ++ci) {
@@ -1244,7 +1261,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             time_begin();
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, nullptr,
-                                     d_slot.as<int>(), d_cursor.as<int>(), nullptr,
+                                     d_slot.as<int>(), d_cursor.as<unsigned long long>(), nullptr,
                                      write_rows ? out.rows.ptr() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
                                      c->overflow_flag(), c->tq, NL, false,
@@ -1449,6 +1466,9 @@ void drop_pending(gs_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->pending_d2h.clear();
     c->stage.reset();
+    // recorded launch timings point at item counts of the failed call's frames
+    c->ev_pending.clear();
+    c->ev_used = 0;
 }
 
 int guard(gs_ctx* c, const std::function<void()>& fn) {
@@ -1840,13 +1860,15 @@ int gs_is_canonical_batch(gs_ctx* c, const int* codes5, const int* lens, int nco
         DBuf d_flags(&c->pool, sizeof(int) * std::max(ncodes, 1));
         h2d(c, d_codes.as<int>(), packed.data(), packed.size());
         HIPCHECK(gsk::fill_i32(c->err_flag(), 0, 1, c->stream));
-        HIPCHECK(gsk::canon_codes(d_codes.as<int>(), ncodes, d_flags.as<int>(), c->err_flag(),
-                                  c->stream));
+        int max_m = 0;
+        for (int i = 0; i < ncodes; ++i) max_m = std::max(max_m, lens[i]);
+        HIPCHECK(gsk::canon_codes(d_codes.as<int>(), ncodes, max_m, d_flags.as<int>(),
+                                  c->err_flag(), c->stream));
         int e = 0;
         d2h(c, flags, d_flags.as<int>(), ncodes);
         d2h(c, &e, c->err_flag(), 1);
         sync(c);
-        if (e & 1) throw GsError(GS_ERR_LIMIT, "canonicality check: more than 2048 projections");
+        if (e & 1) throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
         if (e & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
     });
 }

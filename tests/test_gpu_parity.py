@@ -56,6 +56,27 @@ def test_canonical_star_projection_count(oracle_mod):
     assert oracle_mod.is_canonical(leaf) == 1 and oracle_mod.is_canonical(centre) == 0
 
 
+def test_canonical_seven_star(oracle_mod):
+    """K_{1,7} with one label: the walk of its minimum code (rooted at a leaf) carries
+    7! = 5040 projections, the most any code of up to 7 edges has.  Mining the star graph
+    K_{1,7} itself to k = 7 checks that 7-edge candidate in both modes."""
+    _announce("canonical 7-star")
+    g = synthetic("er", n=50, m=100, n_vlabels=1, seed=1)
+    leaf = ((0, 1, 0, 0, 0),) + tuple((1, j, -1, 0, 0) for j in range(2, 8))
+    centre = ((0, 1, 0, 0, 0),) + tuple((0, j, -1, 0, 0) for j in range(2, 8))
+    with Miner(g) as mn:
+        assert mn.is_canonical([leaf, centre, leaf[:6]]) == [True, False, True]
+    assert oracle_mod.is_canonical(leaf) == 1 and oracle_mod.is_canonical(centre) == 0
+    star = _star(7)
+    want, st = oracle_mod.fsm(star, 7, 1)
+    assert want[-1] == (leaf, 1)
+    for mode in ("batched", "plain"):
+        _announce(f"mine K_1,7 k=7 mode={mode}")
+        with Miner(star, mode=mode) as mn:
+            assert mn.mine(7, 1) == want, mode
+            assert mn.stats()["extended"] == st["extended"], mode
+
+
 @pytest.mark.parametrize("mni", ["gpu", "omp-literal"])
 def test_support_matches_oracle(oracle_mod, mni):
     for s, rng, g in small_graphs(6):

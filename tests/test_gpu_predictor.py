@@ -86,3 +86,25 @@ def test_training_and_predicted_mining(small_graph, oracle_mod):
     assert s["predicted_steps"] > 0
     # the trained sizes let most steps skip the count pass
     assert s["count_launches"] < base["count_launches"]
+
+
+@pytest.mark.parametrize("b3", [1e4, float("inf"), float("nan"), 30.0, 3.0])
+def test_extreme_predictions_stay_exact(small_graph, oracle_mod, b3):
+    """Weights whose network outputs the constant b3 for every key (all other parameters
+    zero).  Huge, infinite and NaN outputs clamp to 2^30 rows per key, which no step can
+    hold, so every step takes the exact two-pass path; a small output (b3 = 3: about 25
+    rows per key) under-sizes the large children, whose steps are redone exactly."""
+    g = small_graph
+    want, st = oracle_mod.fsm(g, 4, 20)
+    w = np.zeros(M.N_PARAMS, np.float32)
+    w[-1] = b3
+    _announce(f"predicted mining with constant output {b3}")
+    with Miner(g, mode="predicted") as mn:
+        mn.set_predictor_weights(w)
+        assert mn.mine(4, 20) == want
+        s = mn.stats()
+    assert s["extended"] == st["extended"]
+    if b3 == 3.0:
+        assert s["fallbacks"] > 0
+    else:
+        assert s["predicted_steps"] == 0 and s["fallbacks"] == 0
