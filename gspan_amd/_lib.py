@@ -33,6 +33,7 @@ GS_SETS_AUTO = 0
 GS_SETS_BITS = 1
 GS_SETS_BYTES = 2
 GS_MAX_PATTERN_EDGES = 7
+GS_NOALLOC_MAX_CODES = 8192
 
 
 class GsOpts(ctypes.Structure):
@@ -118,6 +119,7 @@ def lib():
     L.gs_extend.argtypes = [vp, P(c_int), c_int, P(P(c_int)), P(c_ll), c_int, CHILD_CB, vp,
                             P(c_ll)]
     L.gs_is_canonical_batch.argtypes = [vp, P(c_int), P(c_int), c_int, P(c_int)]
+    L.gs_is_canonical_batch_noalloc.argtypes = [vp, P(c_int), P(c_int), c_int, P(c_int)]
     c_f = ctypes.c_float
     L.gs_predictor_log.argtypes = [vp, c_int, c_ll]
     L.gs_predictor_samples.argtypes = [vp, P(c_ll)]
@@ -140,7 +142,7 @@ def lib():
     L.gs_last_error.restype = ctypes.c_char_p
     L.gs_destroy.argtypes = [vp]
     for name in ("gs_default_opts", "gs_create", "gs_mine", "gs_support", "gs_extend",
-                 "gs_is_canonical_batch", "gs_result_count", "gs_result_copy", "gs_get_stats",
+                 "gs_is_canonical_batch", "gs_is_canonical_batch_noalloc", "gs_result_count", "gs_result_copy", "gs_get_stats",
                  "gs_reset_stats", "gs_destroy", "gs_predictor_log", "gs_predictor_samples",
                  "gs_predictor_train", "gs_predictor_get_weights", "gs_predictor_set_weights",
                  "gs_mlp_forward", "gs_mlp_gradients", "gs_prune_graph", "gs_tuner_report",

@@ -199,6 +199,15 @@ hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_k
 hipError_t canon_codes(const int* codes, int ncodes, int max_m, int* flags, int* err,
                        hipStream_t s);
 
+// Device scratch of gs_is_canonical_batch_noalloc (gs_noalloc.hip): looks up the address
+// of its static device array on `device` (current) once; called by gs_create.
+#define GS_NOALLOC_SCRATCH_BYTES (5 << 19)
+hipError_t noalloc_init(int device);
+// context fields for gs_noalloc.hip (gs_ctx is defined in gs_miner.cpp)
+int ctx_device(const gs_ctx* c);
+hipStream_t ctx_stream(const gs_ctx* c);
+void ctx_set_error(gs_ctx* c, const char* msg);
+
 hipError_t fill_i32(int* p, int v, long long n, hipStream_t s);
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s);

@@ -1556,6 +1556,12 @@ std::vector<DBuf> upload_layers(gs_ctx* c, int m, const int* const* layers,
 
 }  // namespace
 
+namespace gsk {
+int ctx_device(const gs_ctx* c) { return c->device; }
+hipStream_t ctx_stream(const gs_ctx* c) { return c->stream; }
+void ctx_set_error(gs_ctx* c, const char* msg) { c->err = msg; }
+}  // namespace gsk
+
 // ================================================================== C-ABI
 
 extern "C" {
@@ -1618,6 +1624,7 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
 
         HIPCHECK(hipSetDevice(x->opts.device));
         x->device = x->opts.device;
+        HIPCHECK(gsk::noalloc_init(x->device));
         HIPCHECK(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&x->ev0));
         HIPCHECK(hipEventCreate(&x->ev1));

@@ -130,6 +130,18 @@ int gs_extend(gs_ctx* ctx, const int* code5, int m, const int* const* layers,
 int gs_is_canonical_batch(gs_ctx* ctx, const int* codes5, const int* lens, int ncodes,
                           int* flags);
 
+/* gs_is_canonical_batch without any allocation: no hipMalloc, hipFree or pinned-host
+ * allocation in the call.  No counterpart in the reference; for callers that cannot
+ * allocate (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Inputs and outputs are the caller's host arrays, as for
+ * gs_is_canonical_batch; the device staging lives in a static device array of 2.5 MiB that
+ * each GPU the library runs on holds until the process ends.  At most
+ * GS_NOALLOC_MAX_CODES codes per call (GS_ERR_LIMIT beyond).  Calls on one GPU share that
+ * array: two calls that overlap on one GPU (from two threads, or two contexts) race. */
+#define GS_NOALLOC_MAX_CODES 8192
+int gs_is_canonical_batch_noalloc(gs_ctx* ctx, const int* codes5, const int* lens, int ncodes,
+                                  int* flags);
+
 /* Child-count predictor (MLP 32 -> 128 -> 128 -> 1, ReLU, bf16 weights, fp32 accumulation;
  * no counterpart in the reference, whose count pass is gspan.cu:737-755).
  * gs_predictor_log: while on, every exact count pass appends (features, log1p(count)) of its
