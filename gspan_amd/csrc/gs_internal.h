@@ -84,6 +84,26 @@ static_assert(sizeof(RowCtx) == 96, "RowCtx is six 16-byte loads");
 
 struct LayerPtrs {
     const int2* layer[GS_MAXE];  // layer[l] = rows of code edge l: {prev, eid}
+    long long rows[GS_MAXE];     // rows in layer[l] (index guard)
+};
+
+// Index guards of the kernels.  Every index a kernel derives from data (child keys, slots,
+// row reservations, set words, vertex ids, CSR entries, layer rows) is compared with the
+// extent of the buffer it addresses; a violation sets a bit of the context's guard word and
+// the access is skipped, so an inconsistent launch is reported as GS_ERR_INTERNAL instead
+// of touching memory outside its buffers.
+#define GS_GUARD_KEY 1      // child key outside the key space
+#define GS_GUARD_SLOT 2     // live slot outside the set / cursor arrays
+#define GS_GUARD_ROW 4      // row reservation past the row array
+#define GS_GUARD_POOL 8     // set words outside the set pool
+#define GS_GUARD_GRAPH 16   // vertex id or CSR entry outside the graph
+#define GS_GUARD_LAYER 32   // layer row outside its layer
+struct ItemGuard {
+    long long keys;        // child key space (key_count, and key_slot / cap_end when no slots)
+    long long slots;       // entries of sets, cursor and cap_end
+    long long rows_cap;    // entries of rows_out
+    long long pool_words;  // u32 words of the set pool
+    int* err;              // guard word (bits GS_GUARD_*)
 };
 
 // MNI set descriptor of one child key that survived canonicality.
@@ -138,14 +158,15 @@ namespace gsk {
 hipError_t seed_count(const DevGraph& g, int EL, int VL, int* key_count, long long* total,
                       hipStream_t s);
 hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int* cursor,
-                     int2* rows_out, const SetDesc* sets, uint32_t* pool, hipStream_t s);
+                     int2* rows_out, const SetDesc* sets, uint32_t* pool, ItemGuard gd,
+                     hipStream_t s);
 
 // work[row] >= 1 items per row; *work_full (nullable) += the rows' items without the
 // label filter, (r - 1) + sum of segment degrees (the reference scans every neighbour);
 // partial: ceil(R / GS_BLOCK) elements of scratch for that sum
 hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
                 LayerPtrs L, RowCtx* ctx, int* work, unsigned long long* partial,
-                unsigned long long* work_full, hipStream_t s);
+                unsigned long long* work_full, int* guard_err, hipStream_t s);
 hipError_t scan_exclusive(const int* in, long long* out, long long n, long long* tmp,
                           long long tmp_elems, hipStream_t s);
 long long scan_tmp_elems(long long n);
@@ -156,6 +177,14 @@ struct ItemsLaunch {
     int tile;
     int grid;
 };
+
+// Modes of the extension kernel: count children per key; fill the MNI image sets of the
+// children (and count them, in the single-pass last level) without writing rows; fill
+// child rows and sets.
+#define GS_ITEMS_COUNT 0
+#define GS_ITEMS_SETS 1
+#define GS_ITEMS_ROWS 2
+#define GS_ITEMS_MODES 3
 
 // Tile queues of the item kernels: two sets of GS_TILE_QUEUE ints (one counter per XCD,
 // GS_XCD_STRIDE apart) used in turn.  A launch takes set `cur` and zeroes the other one for
@@ -169,21 +198,22 @@ struct TileQueues {
 hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
                        KeySpace ks, int* key_count, TileQueues& tq, bool saturate,
-                       hipStream_t s);
+                       const ItemGuard& gd, hipStream_t s);
 // saturate: the launches adding to key_count may process more than INT_MAX items in all;
 // a count that passes INT_MAX then reads INT_MIN (otherwise counts are exact 32-bit sums).
 // key_count (nullable): also count children per key; key_slot == nullptr: slot = key.
 // cursor[s]: next row of slot s (64-bit, so an overrun cannot wrap).  cap_end (nullable):
 // rows of slot s may only be written below cap_end[s]; a child that does not fit, or a hash
 // set that fills up, sets *overflow instead and writes nothing (predicted sizes).
+// rows_out == nullptr: sets only (GS_ITEMS_SETS), else rows and sets (GS_ITEMS_ROWS).
 hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
                       KeySpace ks, int* key_count, const int* key_slot,
                       unsigned long long* cursor, const int* cap_end, int2* rows_out,
                       const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
-                      int nlive, bool saturate, hipStream_t s);
-// resident workgroups per CU of the count (fill = false) or fill kernel at a tile size
-int items_blocks_per_cu(bool fill, int tile);
+                      int nlive, bool saturate, const ItemGuard& gd, hipStream_t s);
+// resident workgroups per CU of the extension kernel in a mode (GS_ITEMS_*) at a tile size
+int items_blocks_per_cu(int mode, int tile);
 
 // Keys with a non-zero count, in ascending key order, with their counts (flags / pos: K
 // and K+1 elements; scan_tmp as for scan_exclusive over K); *n_active is written and
@@ -211,10 +241,11 @@ void ctx_set_error(gs_ctx* c, const char* msg);
 hipError_t fill_i32(int* p, int v, long long n, hipStream_t s);
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s);
-hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool, int* counts,
-                      hipStream_t s);
+hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
+                      long long pool_words, int* counts, int* guard_err, hipStream_t s);
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
-                        int mni_mode, const SetDesc* set, uint32_t* pool, hipStream_t s);
+                        int mni_mode, const SetDesc* set, uint32_t* pool,
+                        long long pool_words, int* guard_err, hipStream_t s);
 
 // graph.cc:216-266 on the device graph; newid/newoff are exclusive scans (n+1 entries)
 hipError_t prune_graph(const DevGraph& g, int VL, int min_sup, const int* vl_vals,

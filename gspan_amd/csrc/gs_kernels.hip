@@ -129,16 +129,23 @@ struct ItemOut {
     int key;
     int eid;
     int newv;  // image of the new vertex (forward) or -1 (backward)
+    int nv;    // vertices of the parent pattern
+    long long pbase;  // parent row in its level = pbase + (row in the batch)
 };
+
+__device__ __forceinline__ bool in_range(long long x, long long n) {
+    return (unsigned long long)x < (unsigned long long)n;
+}
 
 // One work item of a parent row: item q < r-1 is the backward probe of rmpath index
 // q+1 (gspan.cu:428-445); the rest walk the neighbours of u_n (first forward,
 // gspan.cu:447-462) and of every rightmost-path source (other forward, gspan.cu:464-483).
-// c is the row's context in registers.
+// c is the row's context in registers.  Vertex ids and CSR entries are checked against
+// the graph before they are used as indices (GS_GUARD_GRAPH).
 template <bool UNI>
 __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowRegs& c,
-                                             int q, KeySpace ks) {
-    ItemOut o{false, 0, 0, -1};
+                                             int q, KeySpace ks, int* gerr) {
+    ItemOut o{false, 0, 0, -1, P.nv, P.row_begin - P.brow_begin};
     const int r = P.r;
     if (q < r - 1) {
         const int i = q + 1;
@@ -146,6 +153,10 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
         const int target = pick(c.phi, pget<UNI>(P.bw_v, i));
         const int e_el = pget<UNI>(P.bw_el, i);
         const bool le = pget<UNI>(P.bw_le, i) != 0;
+        if (!in_range(u_n, g.n)) {
+            atomicOr(gerr, GS_GUARD_GRAPH);
+            return o;
+        }
         int lo = g.offsets[u_n];  // all neighbours of u_n, ordered by (to, label)
         const int end = g.offsets[u_n + 1];
         int hi = end;
@@ -174,6 +185,10 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
 #pragma unroll
     for (int s = 0; s < GS_MAXE - 1; ++s) j += (s < r && f >= c.fcum[s]) ? 1 : 0;
     const int el_idx = pick(c.base, j) + f;  // in adjl: neighbours by label from min_label
+    if (!in_range(el_idx, g.nnz)) {
+        atomicOr(gerr, GS_GUARD_GRAPH);
+        return o;
+    }
     const uint2 a = g.adjl[el_idx];
     const int e = g.eidl[el_idx];  // the entry's CSR index
     const int to = (int)a.x, el = (int)(a.y >> 16), tl = (int)(a.y & 0xffffu);
@@ -211,7 +226,8 @@ template <bool FILL>
 __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, int* key_count,
                                                    long long* total, const int* key_slot,
                                                    int* cursor, int2* rows_out,
-                                                   const SetDesc* sets, uint32_t* pool) {
+                                                   const SetDesc* sets, uint32_t* pool,
+                                                   ItemGuard gd) {
     constexpr int IPT = GS_TILE / GS_BLOCK;
     __shared__ int s_key[SEED_SLOTS];
     __shared__ int s_cnt[SEED_SLOTS];
@@ -238,6 +254,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
             const int key = (fl * EL + el) * VL + tl;
             const int agg = FILL ? key_slot[key] : key;
             if (agg < 0) continue;
+            if (FILL && !in_range(agg, gd.slots)) {
+                atomicOr(gd.err, GS_GUARD_SLOT);
+                continue;
+            }
             ++local;
             hs[q] = seed_lds_slot(s_key, agg);
             rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
@@ -259,8 +279,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
                 if (hs[q] < 0) continue;
                 const long long i = t0 + (long long)q * GS_BLOCK + threadIdx.x;
                 const int slot = s_key[hs[q]];
-                if (rows_out) rows_out[s_base[hs[q]] + rank[q]] = make_int2(-1, (int)i);
+                if (rows_out) {
+                    const long long at = (long long)s_base[hs[q]] + rank[q];
+                    if (in_range(at, gd.rows_cap)) rows_out[at] = make_int2(-1, (int)i);
+                    else atomicOr(gd.err, GS_GUARD_ROW);
+                }
                 const SetDesc d = sets[slot];
+                if (d.off < 0 || d.off + 2LL * d.size > gd.pool_words) {
+                    atomicOr(gd.err, GS_GUARD_POOL);
+                    continue;
+                }
                 set_insert(d, pool, 0, (uint32_t)g.src[i]);
                 set_insert(d, pool, 1, g.adj[i].x);
             }
@@ -293,9 +321,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
 // the row's items without the label filter.
 __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
                                                        const PatDev* __restrict__ pats, int npats,
-                                                       long long b, LayerPtrs L,
+                                                       long long b, const LayerPtrs& L,
                                                        RowCtx* __restrict__ out_ctx,
-                                                       int* __restrict__ out_work) {
+                                                       int* __restrict__ out_work, int* gerr) {
     int lo = 0, hi = npats - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -305,10 +333,21 @@ __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
     const PatDev& P = pats[lo];
     int8v phi = int8v{-1, -1, -1, -1, -1, -1, -1, -1};
     long long idx = P.row_begin + (b - P.brow_begin);
+    bool bad = false;
 #pragma unroll
     for (int l = GS_MAXE - 1; l >= 0; --l) {
-        if (l >= P.m) continue;
+        if (l >= P.m || bad) continue;
+        if (!in_range(idx, L.rows[l])) {
+            atomicOr(gerr, GS_GUARD_LAYER);
+            bad = true;
+            continue;
+        }
         const int2 rr = L.layer[l][idx];
+        if (!in_range(rr.y, g.nnz)) {
+            atomicOr(gerr, GS_GUARD_GRAPH);
+            bad = true;
+            continue;
+        }
         if (P.cfrom[l] < P.cto[l]) {
             const int to = (int)g.adj[rr.y].x, v = P.cto[l];
 #pragma unroll
@@ -317,13 +356,20 @@ __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
         if (l == 0) phi[0] = g.src[rr.y];
         idx = rr.x;
     }
+    // a row that failed a guard gets images of vertex 0: its items stay inside the graph
+    // and the guard word fails the call
+    if (bad) phi = int8v{0, 0, 0, 0, 0, 0, 0, 0};
     int8v fcum = int8v{0, 0, 0, 0, 0, 0, 0, 0}, base = fcum;
     int fc = 0, deg = 0;
 #pragma unroll
     for (int j = 0; j < GS_MAXE; ++j) {
         int bj = 0;
         if (j <= P.r) {
-            const int u = pick(phi, P.seg_v[j]);
+            int u = pick(phi, P.seg_v[j]);
+            if (!in_range(u, g.n)) {
+                atomicOr(gerr, GS_GUARD_GRAPH);
+                u = 0;
+            }
             const int o0 = g.offsets[u], o1 = g.offsets[u + 1];
             int first;  // first neighbour (by label) with label >= min_label
             if (g.lstart) {
@@ -362,11 +408,12 @@ __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
 __global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __restrict__ pats,
                                                    int npats, long long b0, long long R, LayerPtrs L,
                                                    RowCtx* __restrict__ ctx, int* __restrict__ work,
-                                                   unsigned long long* __restrict__ partial) {
+                                                   unsigned long long* __restrict__ partial,
+                                                   int* gerr) {
     __shared__ unsigned long long s_w[GS_BLOCK / 64];
     const long long lb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long full = 0;
-    if (lb < R) full = prep_row(g, pats, npats, b0 + lb, L, ctx + lb, work + lb);
+    if (lb < R) full = prep_row(g, pats, npats, b0 + lb, L, ctx + lb, work + lb, gerr);
     if (partial) {
         for (int off = 32; off > 0; off >>= 1) full += __shfl_xor(full, off);
         if (lane_id() == 0) s_w[threadIdx.x / 64] = full;
@@ -531,16 +578,61 @@ __device__ __forceinline__ void count_add(int* count, int c) {
     if (old < 0 || old > INT_MAX - c) atomicExch(count, INT_MIN);
 }
 
-// TILE items per stolen tile (256, 512 or 1024); LDS use grows with TILE, so the tile
-// size trades the number of steals against workgroups resident per CU.
+// First row whose scanned offset is <= item, found by wave 0 with 64-way probes of woff
+// (four rounds of one load per lane for 16M rows).  woff is non-decreasing and
+// woff[0] = 0 <= item, so the probes that pass form a prefix of the lanes.
+__device__ __forceinline__ long long wave_find_row(const long long* __restrict__ woff,
+                                                   long long R, long long item) {
+    const int lane = lane_id();
+    long long lo = 0, hi = R - 1;
+    while (lo < hi) {  // uniform over the wave
+        const long long step = (hi - lo + 64) / 64;
+        const long long p = lo + lane * step;
+        const bool ok = p <= hi && woff[p] <= item;
+        const unsigned long long m = __ballot(ok);
+        const int last = 63 - __clzll((long long)m);  // lane 0 always passes
+        lo += last * step;
+        hi = min(hi, lo + step - 1);
+    }
+    return lo;
+}
+
+// The same over a staged tile's offsets in LDS: last i in [0, n] with off[i] <= item.
+__device__ __forceinline__ int wave_find_lds(const int* off, int n, int item) {
+    const int lane = lane_id();
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int step = (hi - lo + 64) / 64;
+        const int p = lo + lane * step;
+        const bool ok = p <= hi && off[p] <= item;
+        const unsigned long long m = __ballot(ok);
+        const int last = 63 - __clzll((long long)m);
+        lo += last * step;
+        hi = min(hi, lo + step - 1);
+    }
+    return lo;
+}
+
+// Persistent extension kernel, in one of three modes (GS_ITEMS_COUNT / _SETS / _ROWS).
+// Items of all parent rows are flattened by the scan of per-row work (woff); workgroups
+// take TILE-item tiles from per-XCD queues.  Per tile:
+//   phase 1  every thread evaluates its items (row by a search over the tile's staged row
+//            offsets, row context, adjacency entry), aggregates their child keys in an LDS
+//            table and -- in the fill modes -- inserts the children's vertex images into
+//            their MNI sets right away (the set of a key is known once the key is);
+//            meanwhile wave 0 claims the next tile and finds its first row
+//   phase 2  one output reservation per distinct key (rows mode); child counts go to the
+//            workgroup's persistent count table; the next tile's row offsets are loaded
+//   phase 3  (rows mode) child rows at their reserved positions
+// So a tile costs three or four barriers, and the claim, row search and offset loads of
+// the next tile overlap the work of this one.
 //
 // Child counts (key_count) are summed per workgroup in a second LDS table that lives
-// across the tiles the workgroup steals, and added to global memory when it is half full,
-// after 2^14 tiles (so no LDS count passes 2^24) and when the workgroup ends: the
-// counters of a level's hot keys then see one atomic per workgroup instead of one per
-// tile (device-scope atomics are performed at the memory side, so atomics on one hot
-// address serialise there).
-template <bool FILL, int TILE>
+// across the tiles the workgroup takes, and added to global memory when it is more than
+// half full, after 2^14 tiles (so no LDS count passes 2^24) and when the workgroup ends:
+// the counters of a level's hot keys see one atomic per workgroup instead of one per tile
+// (device-scope atomics are performed at the memory side and serialise on one address).
+template <int MODE, int TILE>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
                                                     const long long* __restrict__ woff,
@@ -552,17 +644,22 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     int2* rows_out, const SetDesc* __restrict__ sets,
                                                     uint32_t* pool, int* overflow,
                                                     int* tile_counters, int* next_counters,
-                                                    int dedup, int saturate) {
+                                                    ItemGuard gd, int dedup, int saturate) {
+    constexpr bool FILL = MODE != GS_ITEMS_COUNT;
+    constexpr bool ROWS = MODE == GS_ITEMS_ROWS;
     constexpr int LDS_SLOTS = 2 * TILE;
-    constexpr int ITEMS_PER_THREAD = TILE / GS_BLOCK;
+    constexpr int IPT = TILE / GS_BLOCK;
+    constexpr int STAGE = (TILE + 1 + GS_BLOCK - 1) / GS_BLOCK;  // offsets per thread
     static_assert(TILE % GS_BLOCK == 0 && (TILE & (TILE - 1)) == 0, "tile: power of two");
     static_assert(GS_TILE_QUEUE <= GS_BLOCK, "one thread per queue counter");
     // the queues of the next launch (the previous launch used them and has ended)
     if (blockIdx.x == 0 && threadIdx.x < GS_TILE_QUEUE) next_counters[threadIdx.x] = 0;
-    __shared__ long long s_off[TILE + 1];
+    // scanned offsets of the staged tiles' rows relative to the tile's first item, two
+    // buffers: the current tile and the next (row 0 may start before the tile: <= 0)
+    __shared__ int s_off[2][TILE + 1];
     __shared__ int s_key[LDS_SLOTS];
     __shared__ int s_cnt[LDS_SLOTS];
-    __shared__ int s_base[FILL ? LDS_SLOTS : 1];
+    __shared__ int s_base[ROWS ? LDS_SLOTS : 1];
     // (live slot, row in tile) pairs already seen: only their first item inserts the
     // parent's vertex images, which are the same for every item of the pair
     __shared__ int s_pair[FILL ? LDS_SLOTS : 1];
@@ -573,7 +670,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     // s_pfill: used slots of the persistent table (low 16 bits, at most LDS_SLOTS) and
     // tiles since its last flush (high bits)
     __shared__ int s_pfill;
-    __shared__ long long s_tile, s_rlo;
+    __shared__ long long s_tile[2], s_rlo[2];
+    __shared__ int s_nrows[2];
     // every tile adds at most TILE to a count, so no LDS count passes 2^14 * TILE <= 2^24
     constexpr int FLUSH_TILES = GS_FLUSH_TILES;
     static_assert(FLUSH_TILES >= 1 && FLUSH_TILES <= (1 << 14), "tile bits of s_pfill");
@@ -590,119 +688,190 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     const long long ntiles = (W + TILE - 1) / TILE;
     // XCD-aware stealing: workgroups are dealt to the 8 XCDs round-robin, so workgroup b
     // first drains the b % 8-th contiguous eighth of the tiles (whose rows, patterns and
-    // child sets then stay in that XCD's L2), then helps the other eighths.
+    // child sets then stay in that XCD's L2), then helps the other eighths.  Tiles are
+    // claimed in runs of TB consecutive tiles, about eight claims per workgroup: the queue
+    // counters are hot addresses, and the first row of a tile that follows the previous one
+    // comes from that tile's staged offsets instead of a search over woff.
     const int x0 = blockIdx.x % GS_XCDS;
-    // Tiles are claimed in batches of TB per atomic when there are many per workgroup:
-    // the queue counters are hot addresses and device atomics serialise at the memory side.
-    const int TB = ntiles >= 64LL * gridDim.x ? 4 : 1;
-    int xi = 0;                        // queues this workgroup has found empty (thread 0 only)
-    long long next = 0, claim_end = 0;  // claimed, not yet processed tiles (thread 0 only)
-    for (;;) {
+    const int TB = (int)max(1LL, min(16LL, ntiles / (8LL * gridDim.x)));
+    const bool w0 = threadIdx.x < 64;  // wave 0 claims tiles and finds their first rows
+    int xi = 0;                        // queues found empty (lane 0 of wave 0 only)
+    long long nxt = 0, claim_end = 0;  // claimed, not yet handed out (lane 0 of wave 0)
+    auto claim = [&]() -> long long {  // lane 0 of wave 0
+        if (nxt < claim_end) return nxt++;
+        while (xi < GS_XCDS) {
+            const int x = (x0 + xi) % GS_XCDS;
+            const long long beg = ntiles * x / GS_XCDS, end = ntiles * (x + 1) / GS_XCDS;
+            const long long t = beg + (long long)atomicAdd(&tile_counters[x * GS_XCD_STRIDE], TB);
+            if (t < end) {
+                nxt = t + 1;
+                claim_end = min(t + TB, end);
+                return t;
+            }
+            ++xi;
+        }
+        return -1;
+    };
+    // wave 0: hands tile `tl` (or -1) to buffer b with its first row and row count
+    auto publish = [&](int b, long long tl, long long rlo) {
         if (threadIdx.x == 0) {
-            long long tile = -1;
-            if (next < claim_end) {
-                tile = next++;
-            } else {
-                while (xi < GS_XCDS) {
-                    const int x = (x0 + xi) % GS_XCDS;
-                    const long long beg = ntiles * x / GS_XCDS, end = ntiles * (x + 1) / GS_XCDS;
-                    const long long t =
-                        beg + (long long)atomicAdd(&tile_counters[x * GS_XCD_STRIDE], TB);
-                    if (t < end) {
-                        tile = t;
-                        next = t + 1;
-                        claim_end = min(t + TB, end);
-                        break;
-                    }
-                    ++xi;
-                }
-            }
-            s_tile = tile;
-            // all reads of the last tile ended before its last barrier; unsigned, as the
-            // tile bits only reset on a flush (they are read only when counting)
-            s_pfill = (int)((unsigned)s_pfill + (1u << 16));
+            s_tile[b] = tl;
+            s_rlo[b] = rlo;
+            s_nrows[b] = tl < 0 ? 0 : (int)min((long long)TILE, R - rlo);
         }
-        __syncthreads();
-        const long long tile = s_tile;
-        if (tile < 0) break;
-        const long long t0 = tile * TILE;
-        const long long t1 = min(t0 + TILE, W);
-        if (threadIdx.x == 0) {  // row holding item t0
-            long long lo = 0, hi = R - 1;
-            while (lo < hi) {
-                long long mid = (lo + hi + 1) >> 1;
-                if (woff[mid] <= t0) lo = mid;
-                else hi = mid - 1;
-            }
-            s_rlo = lo;
-        }
-        __syncthreads();
-        const long long rlo = s_rlo;
-        // every row has >= 1 item (u_n always has a neighbour), so a tile spans at most
-        // TILE rows
-        const int nrows = (int)min((long long)TILE, R - rlo);
-        for (int i = threadIdx.x; i <= nrows; i += blockDim.x) s_off[i] = woff[rlo + i];
-        __syncthreads();
+    };
+    // offsets of the rows of buffer b's tile, relative to its first item, clamped so that
+    // they fit an int (only comparisons with item offsets below TILE use the large ones)
+    auto rel = [&](long long v, long long t0) -> int {
+        const long long d = v - t0;
+        return (int)(d > (long long)INT_MAX ? (long long)INT_MAX : d);
+    };
 
-        // phase 1: evaluate this thread's items, count them per key in LDS
-        ItemOut o[ITEMS_PER_THREAD];
-        int row[ITEMS_PER_THREAD], pat[ITEMS_PER_THREAD], hs[ITEMS_PER_THREAD],
-            rank[ITEMS_PER_THREAD];
-        bool lead[ITEMS_PER_THREAD];
+    // prologue: the first tile
+    if (w0) {
+        long long t = -1;
+        if (threadIdx.x == 0) t = claim();
+        t = __shfl(t, 0);
+        publish(0, t, t < 0 ? 0 : wave_find_row(woff, R, t * TILE));
+    }
+    __syncthreads();
+    {
+        const long long t = s_tile[0];
+        if (t >= 0) {
+            const long long rlo = s_rlo[0];
+            for (int i = threadIdx.x; i <= s_nrows[0]; i += GS_BLOCK)
+                s_off[0][i] = rel(woff[rlo + i], t * TILE);
+        }
+    }
+    __syncthreads();
+
+    int buf = 0;
+    for (;;) {
+        const long long tile = s_tile[buf];
+        if (tile < 0) break;
+        const int nb = buf ^ 1;
+        const long long rlo = s_rlo[buf];
+        const int nrows = s_nrows[buf];
+        const int* off = s_off[buf];
+        const long long t0 = tile * TILE;
+        const int tn = (int)min((long long)TILE, W - t0);  // items in this tile
+        // wave 0 claims the next tile now; the claim's atomic overlaps the item loads
+        long long ntile = -1;
+        if (threadIdx.x == 0) ntile = claim();
+
+        // phase 1: evaluate this thread's items, aggregate them per key in LDS, insert the
+        // children's vertex images into their sets
+        int hs[IPT], rank[IPT], eid[ROWS ? IPT : 1], prow[ROWS ? IPT : 1];
 #pragma unroll
-        for (int q = 0; q < ITEMS_PER_THREAD; ++q) {
-            const long long t = t0 + (long long)q * GS_BLOCK + threadIdx.x;
-            o[q] = ItemOut{false, 0, 0, -1};
-            row[q] = 0;
+        for (int q = 0; q < IPT; ++q) {
+            const int it = q * GS_BLOCK + threadIdx.x;  // item offset in the tile
             hs[q] = -1;
             rank[q] = 0;
-            lead[q] = true;
+            ItemOut o{false, 0, 0, -1, 0, 0};
             RowRegs c;
-            int p = -1, f = 0;
-            if (t < t1) {
+            int p = -1, f = 0, row = 0;
+            if (it < tn) {
                 int lo = 0, hi = nrows - 1;
                 while (lo < hi) {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (s_off[mid] <= t) lo = mid;
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (off[mid] <= it) lo = mid;
                     else hi = mid - 1;
                 }
-                row[q] = lo;
+                row = lo;
                 c = load_row(ctx + rlo + lo);
                 p = c.p;
-                f = (int)(t - s_off[lo]);
+                f = it - off[lo];
             }
-            pat[q] = p;
             // rows of one pattern are contiguous, so a wave's items mostly share it: then
             // the pattern's tables are read once per wave with scalar loads
             const int p0 = __builtin_amdgcn_readfirstlane(p);
             if (__all(p == p0 || p < 0)) {
-                if (t < t1) o[q] = eval_item<true>(g, pats[p0], c, f, ks);
-            } else if (t < t1) {
-                o[q] = eval_item<false>(g, pats[p], c, f, ks);
+                if (it < tn) o = eval_item<true>(g, pats[p0], c, f, ks, gd.err);
+            } else if (it < tn) {
+                o = eval_item<false>(g, pats[p], c, f, ks, gd.err);
             }
-            int agg = -1;  // aggregation key: child key (count) / live slot (fill)
-            if (o[q].valid) agg = (FILL && key_slot) ? key_slot[o[q].key] : o[q].key;
-            if (agg >= 0) {
+            if (!o.valid) continue;
+            if (!in_range(o.key, gd.keys)) {
+                atomicOr(gd.err, GS_GUARD_KEY);
+                continue;
+            }
+            // aggregation key: child key (count, single pass) or live slot (fill)
+            const int agg = (FILL && key_slot) ? key_slot[o.key] : o.key;
+            if (agg < 0) continue;  // a child that is not written (not canonical)
+            if (FILL && !in_range(agg, gd.slots)) {
+                atomicOr(gd.err, GS_GUARD_SLOT);
+                continue;
+            }
+            if (FILL) {
+                const SetDesc d = sets[agg];
                 hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
-                if (FILL && dedup) lead[q] = lds_first<LDS_SLOTS>(s_pair, agg * TILE + row[q]);
+                const bool lead = !dedup || lds_first<LDS_SLOTS>(s_pair, agg * TILE + row);
+                if (ROWS) {
+                    eid[q] = o.eid;
+                    prow[q] = (int)(o.pbase + b0 + rlo + row);
+                }
+                // the child's sets: one per child vertex (parent's nv, plus the new vertex
+                // of a forward child)
+                const int nv = o.nv;
+                if (d.off < 0 || d.nv < nv + (o.newv >= 0 ? 1 : 0) ||
+                    d.off + (long long)d.nv * d.size > gd.pool_words) {
+                    atomicOr(gd.err, GS_GUARD_POOL);
+                    continue;
+                }
+                if (lead) {
+#pragma unroll
+                    for (int v = 0; v < GS_MAXV; ++v) {
+                        if (v >= nv) continue;
+                        const int x = c.phi[v];
+                        if (in_range(x, g.n)) set_insert(d, pool, v, (uint32_t)x, overflow);
+                        else atomicOr(gd.err, GS_GUARD_GRAPH);
+                    }
+                }
+                if (o.newv >= 0) {
+                    if (in_range(o.newv, g.n)) set_insert(d, pool, nv, (uint32_t)o.newv, overflow);
+                    else atomicOr(gd.err, GS_GUARD_GRAPH);
+                }
+            } else {
+                hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
+                rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
             }
         }
+        // wave 0: the next tile's first row -- from this tile's offsets when it follows
+        // this tile (its first item is t0 + TILE), else by a search over woff
+        if (w0) {
+            ntile = __shfl(ntile, 0);
+            long long nrlo = 0;
+            if (ntile == tile + 1) nrlo = rlo + wave_find_lds(off, nrows, TILE);
+            else if (ntile >= 0) nrlo = wave_find_row(woff, R, ntile * TILE);
+            publish(nb, ntile, nrlo);
+        }
         __syncthreads();
+
         // phase 2: one output reservation per distinct key of the tile; counts go to the
-        // workgroup's persistent table
+        // workgroup's persistent table; the next tile's row offsets are loaded
+        long long stage[STAGE];
+        const long long ntl = s_tile[nb], nrlo = s_rlo[nb];
+        const int nnr = s_nrows[nb];
+#pragma unroll
+        for (int k = 0; k < STAGE; ++k) {
+            const int i = threadIdx.x + k * GS_BLOCK;
+            stage[k] = (ntl >= 0 && i <= nnr) ? woff[nrlo + i] : 0;
+        }
+        if (key_count && threadIdx.x == 0) atomicAdd(&s_pfill, 1 << 16);  // a tile counted
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
             const int key = s_key[h];
             if (key < 0) continue;
-            if (FILL && rows_out) {
+            if (ROWS) {
                 // 64-bit cursors cannot wrap however far a key overruns a predicted
-                // capacity; a reservation that does not fit below cap_end writes nothing
-                const unsigned long long at =
-                    atomicAdd(&cursor[key], (unsigned long long)s_cnt[h]);
-                const bool fits = !cap_end || at + (unsigned long long)s_cnt[h] <=
-                                                  (unsigned long long)cap_end[key];
+                // capacity; a reservation that does not fit writes nothing
+                const unsigned long long cnt = (unsigned long long)s_cnt[h];
+                const unsigned long long at = atomicAdd(&cursor[key], cnt);
+                const unsigned long long cap = cap_end ? (unsigned long long)cap_end[key]
+                                                       : (unsigned long long)gd.rows_cap;
+                const bool fits = at + cnt <= cap && at + cnt <= (unsigned long long)gd.rows_cap;
                 s_base[h] = fits ? (int)at : -1;
-                if (!fits) atomicOr(overflow, 1);
+                if (!fits) atomicOr(cap_end ? overflow : gd.err, cap_end ? 1 : GS_GUARD_ROW);
             }
             if (key_count) {
                 const int ph = lds_slot_fill<LDS_SLOTS>(s_pkey, key, &s_pfill);
@@ -712,8 +881,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         __syncthreads();
         // Flush the persistent table once more than TILE slots are used (a tile adds at
         // most TILE keys, so the next tile still finds room) or after FLUSH_TILES tiles.
-        // Uniform: s_pfill was last written before the barrier above, and thread 0 writes
-        // it next only after the barrier that ends phase 3.
+        // Uniform: s_pfill was last written before the barrier above.
         if (key_count && ((s_pfill & 0xffff) > TILE || (s_pfill >> 16) >= FLUSH_TILES)) {
             for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
                 if (s_pkey[h] >= 0) {
@@ -726,29 +894,16 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             __syncthreads();
             if (threadIdx.x == 0) s_pfill = 0;  // read again only after later barriers
         }
-        // phase 3 (fill): write child rows at their reserved positions, insert images
-        if (FILL) {
+        // phase 3 (rows): child rows at their reserved positions
+        if (ROWS) {
 #pragma unroll
-            for (int q = 0; q < ITEMS_PER_THREAD; ++q) {
+            for (int q = 0; q < IPT; ++q) {
                 if (hs[q] < 0) continue;
-                const int slot = s_key[hs[q]];
-                const RowCtx& c = ctx[rlo + row[q]];
-                const PatDev& P = pats[pat[q]];
-                if (rows_out && s_base[hs[q]] >= 0) {
-                    const long long prow = P.row_begin + (b0 + rlo + row[q] - P.brow_begin);
-                    rows_out[s_base[hs[q]] + rank[q]] = make_int2((int)prow, o[q].eid);
-                }
-                const SetDesc d = sets[slot];
-                const int nv = P.nv;
-                if (lead[q]) {
-#pragma unroll
-                    for (int v = 0; v < GS_MAXV; ++v)
-                        if (v < nv) set_insert(d, pool, v, (uint32_t)c.phi[v], overflow);
-                }
-                if (o[q].newv >= 0) set_insert(d, pool, nv, (uint32_t)o[q].newv, overflow);
+                const int base = s_base[hs[q]];
+                if (base >= 0) rows_out[base + rank[q]] = make_int2(prow[q], eid[q]);
             }
+            __syncthreads();
         }
-        __syncthreads();
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
             if (s_key[h] >= 0) {
                 s_key[h] = -1;
@@ -756,6 +911,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             }
             if (FILL) s_pair[h] = -1;
         }
+#pragma unroll
+        for (int k = 0; k < STAGE; ++k) {
+            const int i = threadIdx.x + k * GS_BLOCK;
+            if (ntl >= 0 && i <= nnr) s_off[nb][i] = rel(stage[k], ntl * TILE);
+        }
+        __syncthreads();
+        buf = nb;
     }
     // the loop ends after a barrier; the persistent table was last written before it
     if (key_count) {
@@ -850,12 +1012,16 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
 
 // distinct images per (set, vertex): popcount of a bitmap / non-empty hash slots
 __global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets,
-                                                         const uint32_t* pool, int* counts) {
+                                                         const uint32_t* pool,
+                                                         long long pool_words, int* counts,
+                                                         int* gerr) {
     const int s = blockIdx.x / GS_MAXV, v = blockIdx.x % GS_MAXV;
     const SetDesc d = sets[s];
     __shared__ int s_red[GS_BLOCK / 64];
     int c = 0;
-    if (v < d.nv) {
+    const bool ok = d.off >= 0 && d.off + (long long)d.nv * d.size <= pool_words;
+    if (!ok && v == 0 && threadIdx.x == 0) atomicOr(gerr, GS_GUARD_POOL);
+    if (ok && v < d.nv) {
         const uint32_t* base = pool + d.off + (long long)v * d.size;
         if (d.kind != GS_SET_HASH) {  // bitmap bits, or bytemap bytes that are 0 or 1
             for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += __popc(base[i]);
@@ -877,7 +1043,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets,
 // every pattern vertex go into the pattern's sets.
 __global__ __launch_bounds__(GS_BLOCK) void k_support_rows(DevGraph g, const PatDev* pat,
                                                            long long N, LayerPtrs L,
-                                                           const SetDesc* set, uint32_t* pool) {
+                                                           const SetDesc* set, uint32_t* pool,
+                                                           long long pool_words, int* gerr) {
     long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= N) return;
     const PatDev& P = *pat;
@@ -886,13 +1053,28 @@ __global__ __launch_bounds__(GS_BLOCK) void k_support_rows(DevGraph g, const Pat
     for (int v = 0; v < GS_MAXV; ++v) phi[v] = -1;
     long long idx = b;
     for (int l = P.m - 1; l >= 0; --l) {
+        if (!in_range(idx, L.rows[l])) {
+            atomicOr(gerr, GS_GUARD_LAYER);
+            return;
+        }
         int2 rr = L.layer[l][idx];
+        if (!in_range(rr.y, g.nnz)) {
+            atomicOr(gerr, GS_GUARD_GRAPH);
+            return;
+        }
         if (P.cfrom[l] < P.cto[l]) phi[P.cto[l]] = (int)g.adj[rr.y].x;
         if (l == 0) phi[0] = g.src[rr.y];
         idx = rr.x;
     }
     const SetDesc d = *set;
-    for (int v = 0; v < P.nv; ++v) set_insert(d, pool, v, (uint32_t)phi[v]);
+    if (d.off < 0 || d.off + (long long)P.nv * d.size > pool_words) {
+        atomicOr(gerr, GS_GUARD_POOL);
+        return;
+    }
+    for (int v = 0; v < P.nv; ++v) {
+        if (in_range(phi[v], g.n)) set_insert(d, pool, v, (uint32_t)phi[v]);
+        else atomicOr(gerr, GS_GUARD_GRAPH);
+    }
 }
 
 // ------------------------------------------------------------------ canonicality
@@ -1280,25 +1462,28 @@ namespace gsk {
 
 hipError_t seed_count(const DevGraph& g, int EL, int VL, int* key_count, long long* total,
                       hipStream_t s) {
+    const ItemGuard none{0, 0, 0, 0, nullptr};
     hipLaunchKernelGGL(k_seed<false>, dim3(grid_for(g.nnz, GS_TILE, 4096)), dim3(GS_BLOCK), 0, s,
-                       g, EL, VL, key_count, total, nullptr, nullptr, nullptr, nullptr, nullptr);
+                       g, EL, VL, key_count, total, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       none);
     return hipGetLastError();
 }
 
 hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int* cursor,
-                     int2* rows_out, const SetDesc* sets, uint32_t* pool, hipStream_t s) {
+                     int2* rows_out, const SetDesc* sets, uint32_t* pool, ItemGuard gd,
+                     hipStream_t s) {
     hipLaunchKernelGGL(k_seed<true>, dim3(grid_for(g.nnz, GS_TILE, 4096)), dim3(GS_BLOCK), 0, s,
-                       g, EL, VL, nullptr, nullptr, key_slot, cursor, rows_out, sets, pool);
+                       g, EL, VL, nullptr, nullptr, key_slot, cursor, rows_out, sets, pool, gd);
     return hipGetLastError();
 }
 
 hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
                 LayerPtrs L, RowCtx* ctx, int* work, unsigned long long* partial,
-                unsigned long long* work_full, hipStream_t s) {
+                unsigned long long* work_full, int* guard_err, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     long long grid = (R + GS_BLOCK - 1) / GS_BLOCK;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pats, npats, b0, R,
-                       L, ctx, work, work_full ? partial : nullptr);
+                       L, ctx, work, work_full ? partial : nullptr, guard_err);
     if (work_full)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(GS_BLOCK), 0, s, partial, grid,
                            work_full);
@@ -1321,20 +1506,19 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
     return hipGetLastError();
 }
 
-template <bool FILL>
+template <int MODE>
 hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                         const RowCtx* ctx, const long long* woff, long long b0, long long R,
                         KeySpace ks, int* key_count, const int* key_slot,
                         unsigned long long* cursor, const int* cap_end, int2* rows_out,
                         const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
-                        int dedup, int saturate,
-                        hipStream_t s) {
+                        const ItemGuard& gd, int dedup, int saturate, hipStream_t s) {
     int* tile_counters = tq.base + tq.cur * GS_TILE_QUEUE;
     int* next_counters = tq.base + (1 - tq.cur) * GS_TILE_QUEUE;
 #define GS_LAUNCH_ITEMS(T)                                                                     \
-    hipLaunchKernelGGL((k_items<FILL, T>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, \
+    hipLaunchKernelGGL((k_items<MODE, T>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, \
                        woff, b0, R, ks, key_count, key_slot, cursor, cap_end, rows_out, sets,  \
-                       pool, overflow, tile_counters, next_counters, dedup, saturate)
+                       pool, overflow, tile_counters, next_counters, gd, dedup, saturate)
     switch (lc.tile) {
         case 256: GS_LAUNCH_ITEMS(256); break;
         case 512: GS_LAUNCH_ITEMS(512); break;
@@ -1350,11 +1534,11 @@ hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* 
 hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                        const RowCtx* ctx, const long long* woff, long long b0, long long R,
                        KeySpace ks, int* key_count, TileQueues& tq, bool saturate,
-                       hipStream_t s) {
+                       const ItemGuard& gd, hipStream_t s) {
     if (R <= 0) return hipSuccess;
-    return launch_items<false>(lc, g, pats, ctx, woff, b0, R, ks, key_count, nullptr, nullptr,
-                               nullptr, nullptr, nullptr, nullptr, nullptr, tq, 0,
-                               saturate ? 1 : 0, s);
+    return launch_items<GS_ITEMS_COUNT>(lc, g, pats, ctx, woff, b0, R, ks, key_count, nullptr,
+                                        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tq,
+                                        gd, 0, saturate ? 1 : 0, s);
 }
 
 hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
@@ -1362,29 +1546,36 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       KeySpace ks, int* key_count, const int* key_slot,
                       unsigned long long* cursor, const int* cap_end, int2* rows_out,
                       const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
-                      int nlive, bool saturate, hipStream_t s) {
+                      int nlive, bool saturate, const ItemGuard& gd, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     // pair keys slot * TILE + row must stay below 2^31
     const int dedup = (long long)nlive * lc.tile < (1LL << 31) ? 1 : 0;
-    return launch_items<true>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot, cursor,
-                              cap_end, rows_out, sets, pool, overflow, tq, dedup,
-                              saturate ? 1 : 0, s);
+    if (rows_out)
+        return launch_items<GS_ITEMS_ROWS>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot,
+                                           cursor, cap_end, rows_out, sets, pool, overflow, tq,
+                                           gd, dedup, saturate ? 1 : 0, s);
+    return launch_items<GS_ITEMS_SETS>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot,
+                                       cursor, cap_end, nullptr, sets, pool, overflow, tq, gd,
+                                       dedup, saturate ? 1 : 0, s);
 }
 
-int items_blocks_per_cu(bool fill, int tile) {
+int items_blocks_per_cu(int mode, int tile) {
     int per_cu = 0;
     hipError_t e = hipErrorInvalidValue;
-#define GS_OCC(F, T) \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_items<F, T>, GS_BLOCK, 0)
-    if (fill) {
-        if (tile == 256) GS_OCC(true, 256);
-        else if (tile == 512) GS_OCC(true, 512);
-        else if (tile == 1024) GS_OCC(true, 1024);
+#define GS_OCC(M, T) \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_items<M, T>, GS_BLOCK, 0)
+#define GS_OCC_TILES(M)                          \
+    if (tile == 256) GS_OCC(M, 256);             \
+    else if (tile == 512) GS_OCC(M, 512);        \
+    else if (tile == 1024) GS_OCC(M, 1024)
+    if (mode == GS_ITEMS_COUNT) {
+        GS_OCC_TILES(GS_ITEMS_COUNT);
+    } else if (mode == GS_ITEMS_SETS) {
+        GS_OCC_TILES(GS_ITEMS_SETS);
     } else {
-        if (tile == 256) GS_OCC(false, 256);
-        else if (tile == 512) GS_OCC(false, 512);
-        else if (tile == 1024) GS_OCC(false, 1024);
+        GS_OCC_TILES(GS_ITEMS_ROWS);
     }
+#undef GS_OCC_TILES
 #undef GS_OCC
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     return per_cu;
@@ -1456,21 +1647,22 @@ hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool, int* counts,
-                      hipStream_t s) {
+hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
+                      long long pool_words, int* counts, int* guard_err, hipStream_t s) {
     if (nsets <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_set_counts, dim3(nsets * GS_MAXV), dim3(GS_BLOCK), 0, s, sets, pool,
-                       counts);
+                       pool_words, counts, guard_err);
     return hipGetLastError();
 }
 
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
-                        int mni_mode, const SetDesc* set, uint32_t* pool, hipStream_t s) {
+                        int mni_mode, const SetDesc* set, uint32_t* pool, long long pool_words,
+                        int* guard_err, hipStream_t s) {
     (void)mni_mode;
     if (N <= 0) return hipSuccess;
     long long grid = (N + GS_BLOCK - 1) / GS_BLOCK;
     hipLaunchKernelGGL(k_support_rows, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pat, N, L,
-                       set, pool);
+                       set, pool, pool_words, guard_err);
     return hipGetLastError();
 }
 

@@ -288,7 +288,7 @@ struct gs_ctx {
     std::vector<PendingD2H> pending_d2h;
     DBuf d_offsets, d_src, d_adj, d_vlab, d_adjl, d_eidl, d_lstart;
     DevGraph dg{};
-    DBuf d_misc;  // [0] unused, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
+    DBuf d_misc;  // [0] guard word, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
     DBuf d_tiles; // per-XCD tile queues of the item kernels
     LaunchTuner tuner;  // launch shape of the item kernels
     std::string err;
@@ -329,6 +329,9 @@ struct gs_ctx {
         tq = gsk::TileQueues{d_tiles.as<int>(), 0};
         HIPCHECK(gsk::fill_i32(tq.base, 0, 2 * GS_TILE_QUEUE, stream));
     }
+    // index guards of the kernels (GS_GUARD_* bits); read back at every sync
+    int* guard_word() const { return d_misc.as<int>(); }
+    int guard_host = 0;
     int* n_active() const { return d_misc.as<int>() + 1; }
     int* err_flag() const { return d_misc.as<int>() + 2; }
     int* overflow_flag() const { return d_misc.as<int>() + 3; }
@@ -341,13 +344,44 @@ struct gs_ctx {
 
 namespace {
 
+// Names of the guard bits a kernel raised.
+std::string guard_names(int bits) {
+    static const char* const names[] = {"child key", "live slot", "row reservation",
+                                        "set pool", "graph index", "layer row"};
+    std::string s;
+    for (int b = 0; b < 6; ++b)
+        if (bits & (1 << b)) s += std::string(s.empty() ? "" : ", ") + names[b];
+    return s;
+}
+
 // Waits for the stream; device-to-host copies issued since the last sync are then in
-// their destinations.
+// their destinations.  The kernels' guard word is read in the same wait: a kernel that
+// met an index outside its buffers (and skipped the access) fails the call here.
 void sync(gs_ctx* c) {
+    const bool guarded = c->d_misc.as<int>() != nullptr;
+    if (guarded) {
+        void* p = c->stage.take(sizeof(int));
+        HIPCHECK(hipMemcpyAsync(p, c->guard_word(), sizeof(int), hipMemcpyDeviceToHost,
+                                c->stream));
+        c->pending_d2h.push_back(PendingD2H{&c->guard_host, p, sizeof(int)});
+    }
     HIPCHECK(hipStreamSynchronize(c->stream));
     for (const PendingD2H& p : c->pending_d2h) std::memcpy(p.dst, p.staged, p.bytes);
     c->pending_d2h.clear();
     c->stage.reset();
+    if (guarded && c->guard_host != 0) {
+        const int bits = c->guard_host;
+        c->guard_host = 0;
+        HIPCHECK(gsk::fill_i32(c->guard_word(), 0, 1, c->stream));  // for later calls
+        throw GsError(GS_ERR_INTERNAL,
+                      "kernel index guard: " + guard_names(bits) + " outside its buffer");
+    }
+}
+
+// Clears the guard word (at the start of every call that launches kernels).
+void clear_guard(gs_ctx* c) {
+    HIPCHECK(gsk::fill_i32(c->guard_word(), 0, 1, c->stream));
+    c->guard_host = 0;
 }
 
 // Host-to-device copy: src may be released as soon as this returns.
@@ -777,9 +811,16 @@ class LevelRows {
     int2* ptr_ = nullptr;
 };
 
+// One DFS level's rows as the kernels read them: {prev, eid} per row.
+struct Layer {
+    const int2* p = nullptr;
+    long long rows = 0;
+};
+
 struct StepOut {
     std::vector<ChildInfo> children;  // every candidate child (active key), key order
     LevelRows rows;                   // child rows of the written keys
+    long long rows_total = 0;         // rows in `rows`
     long long emitted = 0;
 };
 
@@ -804,7 +845,7 @@ long long predicted_capacity(float y) {
 
 // One extension step of a batch of size-m patterns.  write_rows: keep child rows (needed
 // below the last level); all_live: write rows and sets of non-canonical children too.
-int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const int2*>& layers,
+int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>& layers,
              bool write_rows, bool all_live, StepOut& out) {
     const int B = (int)pats.size();
     if (B == 0) return STEP_OK;
@@ -828,7 +869,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     DBuf d_pats(P, sizeof(PatDev) * B);
     h2d(c, d_pats.as<PatDev>(), pd.data(), B);
     LayerPtrs L{};
-    for (int l = 0; l < m; ++l) L.layer[l] = layers[l];
+    for (int l = 0; l < m; ++l) {
+        L.layer[l] = layers[l].p;
+        L.rows[l] = layers[l].rows;
+    }
     const KeySpace ks{c->EL, c->VL};
 
     const long long chunk = c->opts.chunk_rows > 0 ? c->opts.chunk_rows : (16LL << 20);
@@ -907,15 +951,20 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         const long long Rc = std::min(chunk, R - b0);
         HIPCHECK(gsk::prep(c->dg, d_pats.as<PatDev>(), B, b0, Rc, L, d_ctx.as<RowCtx>(),
                            d_work.as<int>(), d_partial.as<unsigned long long>(),
-                           first_pass ? c->work_full64() : nullptr, c->stream));
+                           first_pass ? c->work_full64() : nullptr, c->guard_word(),
+                           c->stream));
         HIPCHECK(gsk::scan_exclusive(d_work.as<int>(), d_woff.as<long long>(), Rc,
                                      d_tmp.as<long long>(), tmp_elems, c->stream));
         return Rc;
     };
     // launch shape from the tuner, and events around the launch when they are wanted
     int t_bucket = -1, t_cfg = 0;
-    auto shape = [&](bool fill, bool single_pass, long long Rc) {
-        return c->tuner.choose(fill, single_pass, m, Rc, &t_bucket, &t_cfg);
+    auto shape = [&](int mode, bool single_pass, long long Rc) {
+        return c->tuner.choose(mode, single_pass, m, Rc, &t_bucket, &t_cfg);
+    };
+    // extents of the buffers a launch addresses, for the kernels' index guards
+    auto guard_of = [&](long long slots, long long rows_cap, long long pool_words) {
+        return ItemGuard{K, slots, rows_cap, pool_words, c->guard_word()};
     };
     auto time_begin = [&]() {
         if (!c->opts.timing && t_bucket < 0) return;
@@ -997,14 +1046,17 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 const long long b0 = (long long)ci * chunk;
                 const long long Rc = prep_chunk(ci, true);
                 d2h(c, &Wc_p[ci], d_woff.as<long long>() + Rc, 1);
-                const gsk::ItemsLaunch lc = shape(true, false, Rc);
+                const gsk::ItemsLaunch lc = shape(GS_ITEMS_ROWS, false, Rc);
                 time_begin();
                 HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                          d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
                                          nullptr, d_cur.as<unsigned long long>(), d_end.as<int>(),
                                          rows.as<int2>(), d_psets.as<SetDesc>(),
                                          d_ppool.as<uint32_t>(), c->overflow_flag(),
-                                         c->tq, (int)K, saturate, c->stream));
+                                         c->tq, (int)K, saturate,
+                                         guard_of(K, std::max<long long>(rows_total, 1),
+                                                  std::max<long long>(pool_words, 4)),
+                                         c->stream));
                 time_end(1, &Wc_p[ci]);
                 c->st.fill_launches++;
             }
@@ -1057,7 +1109,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                     DBuf d_ls(P, sizeof(SetDesc) * NL), d_counts(P, sizeof(int) * NL * GS_MAXV);
                     h2d(c, d_ls.as<SetDesc>(), live_sets.data(), NL);
                     HIPCHECK(gsk::set_counts(d_ls.as<SetDesc>(), NL, d_ppool.as<uint32_t>(),
-                                             d_counts.as<int>(), c->stream));
+                                             std::max<long long>(pool_words, 4),
+                                             d_counts.as<int>(), c->guard_word(), c->stream));
                     std::vector<int> counts((size_t)NL * GS_MAXV);
                     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
                     sync(c);
@@ -1080,6 +1133,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                 }
                 out.rows = LevelRows();
                 out.rows.adopt(std::move(rows));
+                out.rows_total = rows_total;
                 c->st.predicted_steps++;
                 c->st.batches++;
                 return STEP_OK;
@@ -1097,20 +1151,22 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         const long long b0 = (long long)ci * chunk;
         const long long Rc = prep_chunk(ci, true);
         d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
-        const gsk::ItemsLaunch lc = shape(single, single, Rc);
+        const gsk::ItemsLaunch lc = shape(single ? GS_ITEMS_SETS : GS_ITEMS_COUNT, single, Rc);
         time_begin();
         if (single) {
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
                                      nullptr, nullptr, nullptr, d_all_sets.as<SetDesc>(),
                                      d_all_pool.as<uint32_t>(), c->overflow_flag(),
-                                     c->tq, (int)K, saturate, c->stream));
+                                     c->tq, (int)K, saturate,
+                                     guard_of(K, K, std::max<long long>(all_words, 4)),
+                                     c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
         } else {
             HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                       d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
-                                      c->tq, saturate, c->stream));
+                                      c->tq, saturate, guard_of(0, 0, 0), c->stream));
             time_end(0, &Wc_host[ci]);
             c->st.count_launches++;
         }
@@ -1248,6 +1304,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
             } else {
                 out.rows.on_device(P, rows_total);
             }
+            out.rows_total = rows_total;
         }
         HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
         h2d(c, d_live.as<int>(), live_keys.data(), NL);
@@ -1257,7 +1314,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
         for (int ci = 0; ci < nchunks; ++ci) {
             const long long b0 = (long long)ci * chunk;
             const long long Rc = nchunks > 1 ? prep_chunk(ci, false) : std::min(chunk, R - b0);
-            const gsk::ItemsLaunch lc = shape(true, false, Rc);
+            const gsk::ItemsLaunch lc =
+                shape(write_rows ? GS_ITEMS_ROWS : GS_ITEMS_SETS, false, Rc);
             time_begin();
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, nullptr,
@@ -1265,6 +1323,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
                                      write_rows ? out.rows.ptr() : nullptr,
                                      d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
                                      c->overflow_flag(), c->tq, NL, false,
+                                     guard_of(NL, write_rows ? rows_total : 0,
+                                              std::max<long long>(pool_words, 4)),
                                      c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
@@ -1282,7 +1342,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     }
     HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
                              single ? d_all_pool.as<uint32_t>() : d_pool.as<uint32_t>(),
-                             d_counts.as<int>(), c->stream));
+                             std::max<long long>(single ? all_words : pool_words, 4),
+                             d_counts.as<int>(), c->guard_word(), c->stream));
     std::vector<int> counts((size_t)NL * GS_MAXV);
     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
     sync(c);
@@ -1300,10 +1361,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<const i
     return STEP_OK;
 }
 
-void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers);
+void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers);
 
 // Extends a batch, reports its frequent canonical children and descends into them.
-void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers) {
+void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) {
     const int m = pats[0].m;
     const bool last = m + 1 >= c->k;
     StepOut so;
@@ -1331,7 +1392,7 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers
         if (!last) next.push_back(ch.pat);
     }
     if (!last && !next.empty()) {
-        layers.push_back(so.rows.ptr());
+        layers.push_back(Layer{so.rows.ptr(), so.rows_total});
         mine_level(c, next, layers);
         layers.pop_back();
     }
@@ -1341,7 +1402,7 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers
         c->child_bufs.push_back(std::move(so.children));
 }
 
-void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<const int2*>& layers) {
+void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) {
     if (pats.empty() || pats[0].m >= c->k) return;
     if (c->opts.mode == GS_MODE_PLAIN) {
         // one parent at a time, in the order the reference visits them
@@ -1435,9 +1496,13 @@ void mine_seeds(gs_ctx* c) {
     HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
     HIPCHECK(gsk::seed_fill(c->dg, c->EL, c->VL, d_slot.as<int>(), d_cursor.as<int>(),
                             rows ? d_rows.ptr() : nullptr, d_sets.as<SetDesc>(),
-                            d_pool.as<uint32_t>(), c->stream));
-    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(), d_counts.as<int>(),
-                             c->stream));
+                            d_pool.as<uint32_t>(),
+                            ItemGuard{K, NL, rows_total, std::max<long long>(pool_words, 4),
+                                      c->guard_word()},
+                            c->stream));
+    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL, d_pool.as<uint32_t>(),
+                             std::max<long long>(pool_words, 4), d_counts.as<int>(),
+                             c->guard_word(), c->stream));
     std::vector<int> counts((size_t)NL * GS_MAXV);
     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
     sync(c);
@@ -1450,7 +1515,7 @@ void mine_seeds(gs_ctx* c) {
         freq.push_back(seeds[i]);
     }
     if (rows && !freq.empty()) {
-        std::vector<const int2*> layers{d_rows.ptr()};
+        std::vector<Layer> layers{Layer{d_rows.ptr(), rows_total}};
         mine_level(c, freq, layers);
     }
 }
@@ -1533,7 +1598,7 @@ HPat code_to_hpat(gs_ctx* c, const int* code5, int m) {
 }
 
 std::vector<DBuf> upload_layers(gs_ctx* c, int m, const int* const* layers,
-                                const long long* counts, std::vector<const int2*>& ptrs) {
+                                const long long* counts, std::vector<Layer>& ptrs) {
     std::vector<DBuf> bufs;
     ptrs.clear();
     for (int l = 0; l < m; ++l) {
@@ -1549,7 +1614,7 @@ std::vector<DBuf> upload_layers(gs_ctx* c, int m, const int* const* layers,
         }
         bufs.emplace_back(&c->pool, sizeof(int2) * std::max<long long>(counts[l], 1));
         h2d(c, bufs.back().as<int2>(), reinterpret_cast<const int2*>(layers[l]), counts[l]);
-        ptrs.push_back(bufs.back().as<int2>());
+        ptrs.push_back(Layer{bufs.back().as<int2>(), counts[l]});
     }
     return bufs;
 }
@@ -1687,6 +1752,7 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         if (!lstart.empty()) x->d_lstart = DBuf(P, sizeof(int) * lstart.size());
         x->d_vlab = DBuf(P, sizeof(int) * std::max(n, 1));
         x->d_misc = DBuf(P, 64);
+        HIPCHECK(gsk::fill_i32(x->d_misc.as<int>(), 0, 16, x->stream));
         x->d_tiles = DBuf(P, sizeof(int) * 2 * GS_TILE_QUEUE);
         h2d(x, x->d_offsets.as<int>(), offsets, n + 1);
         h2d(x, x->d_src.as<int>(), src.data(), nnz);
@@ -1731,6 +1797,7 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->min_sup = min_support;
         c->mni_mode = c->opts.mni_mode;
         c->reports.clear();
+        clear_guard(c);
         c->reset_tile_queues();
         mine_seeds(c);
         sort_reports(c);
@@ -1751,11 +1818,15 @@ int gs_support(gs_ctx* c, const int* code5, int m, const int* const* layers,
     return guard(c, [&]() {
         HIPCHECK(hipSetDevice(c->device));
         HPat h = code_to_hpat(c, code5, m);
-        std::vector<const int2*> ptrs;
+        std::vector<Layer> ptrs;
+        clear_guard(c);
         std::vector<DBuf> bufs = upload_layers(c, m, layers, counts, ptrs);
         PatDev d = to_dev(h);
         LayerPtrs L{};
-        for (int l = 0; l < m; ++l) L.layer[l] = ptrs[l];
+        for (int l = 0; l < m; ++l) {
+            L.layer[l] = ptrs[l].p;
+            L.rows[l] = ptrs[l].rows;
+        }
         DBuf d_pat(&c->pool, sizeof(PatDev));
         h2d(c, d_pat.as<PatDev>(), &d, 1);
         long long words = 0;
@@ -1766,9 +1837,11 @@ int gs_support(gs_ctx* c, const int* code5, int m, const int* const* layers,
         h2d(c, d_set.as<SetDesc>(), &sd, 1);
         HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, words, c->stream));
         HIPCHECK(gsk::support_rows(c->dg, d_pat.as<PatDev>(), counts[m - 1], L, mni_mode,
-                                   d_set.as<SetDesc>(), d_pool.as<uint32_t>(), c->stream));
+                                   d_set.as<SetDesc>(), d_pool.as<uint32_t>(),
+                                   std::max(words, 4LL), c->guard_word(), c->stream));
         HIPCHECK(gsk::set_counts(d_set.as<SetDesc>(), 1, d_pool.as<uint32_t>(),
-                                 d_counts.as<int>(), c->stream));
+                                 std::max(words, 4LL), d_counts.as<int>(), c->guard_word(),
+                                 c->stream));
         int cnt[GS_MAXV];
         d2h(c, cnt, d_counts.as<int>(), GS_MAXV);
         sync(c);
@@ -1786,7 +1859,8 @@ int gs_extend(gs_ctx* c, const int* code5, int m, const int* const* layers,
         HPat h = code_to_hpat(c, code5, m);
         h.row_begin = 0;
         h.row_count = counts[m - 1];
-        std::vector<const int2*> ptrs;
+        std::vector<Layer> ptrs;
+        clear_guard(c);
         std::vector<DBuf> bufs = upload_layers(c, m, layers, counts, ptrs);
         c->mni_mode = mni_mode;
         c->reset_tile_queues();

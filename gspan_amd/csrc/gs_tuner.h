@@ -1,7 +1,7 @@
 // Online autotuner of the extension kernels' launch shape (items per stolen tile and
 // persistent workgroups per CU).
 //
-// Launches are grouped into buckets by (pass, single-pass last level, parent pattern
+// Launches are grouped into buckets by (kernel mode, single-pass last level, parent pattern
 // size).  Within a bucket, every candidate shape is tried once on a launch large enough
 // to time (HIP events around the launch, read at the next host synchronisation); after
 // that the shape with the least time per work item is used.  The shapes never change
@@ -21,11 +21,11 @@ class LaunchTuner {
 
     void init(int cus, bool enabled, int forced_tile) {
         enabled_ = enabled && forced_tile == 0;
-        for (int f = 0; f < 2; ++f) {
+        for (int f = 0; f < GS_ITEMS_MODES; ++f) {
             cfgs_[f].clear();
-            for (int tile : {1024, 512, 256}) {  // the first shape is the default
+            for (int tile : {512, 1024, 256}) {  // the first shape is the default
                 if (forced_tile && tile != forced_tile) continue;
-                const int occ = gsk::items_blocks_per_cu(f == 1, tile);
+                const int occ = gsk::items_blocks_per_cu(f, tile);
                 for (int blocks : {occ, occ / 2}) {
                     if (blocks < 1) continue;
                     bool dup = false;
@@ -37,14 +37,14 @@ class LaunchTuner {
         }
     }
 
-    // Shape for a launch over R parent rows; *bucket / *cfg identify it for record()
-    // (-1 when the launch is not part of the search).
-    gsk::ItemsLaunch choose(bool fill, bool single, int m, long long R, int* bucket, int* cfg) {
-        const auto& cs = cfgs_[fill ? 1 : 0];
+    // Shape for a launch over R parent rows in a kernel mode (GS_ITEMS_*); *bucket / *cfg
+    // identify it for record() (-1 when the launch is not part of the search).
+    gsk::ItemsLaunch choose(int mode, bool single, int m, long long R, int* bucket, int* cfg) {
+        const auto& cs = cfgs_[mode];
         *bucket = -1;
         *cfg = 0;
         if (!enabled_) return cs[0];
-        const int b = (fill ? 1000 : 0) + (single ? 100 : 0) + m;
+        const int b = mode * 1000 + (single ? 100 : 0) + m;
         Bucket& bk = buckets_[b];
         if (bk.ms.empty()) bk.resize(cs.size());
         int best = best_of(bk);
@@ -77,11 +77,13 @@ class LaunchTuner {
         for (const auto& kv : buckets_) {
             const int b = kv.first;
             const Bucket& bk = kv.second;
-            const auto& cs = cfgs_[b >= 1000 ? 1 : 0];
+            const int mode = b / 1000;
+            const auto& cs = cfgs_[mode];
             if (!first) s += ", ";
             first = false;
-            s += "\"" + std::string(b >= 1000 ? "fill" : "count") +
-                 ((b % 1000) >= 100 ? " single" : "") + " m=" + std::to_string(b % 100) + "\": ";
+            static const char* const names[GS_ITEMS_MODES] = {"count", "sets", "rows"};
+            s += "\"" + std::string(names[mode]) + ((b % 1000) >= 100 ? " single" : "") +
+                 " m=" + std::to_string(b % 100) + "\": ";
             const int best = best_of(bk);
             s += "{\"best\": [" + std::to_string(cs[best].tile) + ", " +
                  std::to_string(cs[best].grid) + "], \"ns_per_item\": [";
@@ -123,6 +125,6 @@ class LaunchTuner {
     }
 
     bool enabled_ = false;
-    std::vector<gsk::ItemsLaunch> cfgs_[2];
+    std::vector<gsk::ItemsLaunch> cfgs_[GS_ITEMS_MODES];
     std::map<int, Bucket> buckets_;
 };
