@@ -504,6 +504,9 @@ static bool is_canonical(const DFSCode& code, bool* overrun) {
 
 // ---------------------------------------------------------------- driver
 struct Stats {
+    // rows of canonical codes with m edges (level_rows[m]) and child rows extension emits
+    // into level m (level_emitted[m]), for the level-size tests
+    long long level_rows[9] = {0}, level_emitted[9] = {0};
     long long extended = 0;       // child rows emitted by extension (gspan.cu:708,717,730)
     long long candidates = 0;     // mine() invocations
     long long canonical = 0;      // candidates passing is_canonical
@@ -556,6 +559,7 @@ struct Miner {
         if (overrun) ++st.overruns;
         if (!canon) return;
         ++st.canonical;
+        if (code.size() < 9) st.level_rows[code.size()] += (long long)L.back()->size();
         t0 = omp_get_wtime();
         int sup = support(g, code, L, mode, nthreads, bm);
         st.t_support += omp_get_wtime() - t0;
@@ -568,7 +572,9 @@ struct Miner {
         std::vector<int> rm = right_most_path(code);
         int v_n = code[rm[0]].to;
         t0 = omp_get_wtime();
-        st.extended += extend_omp(g, code, L, rm, fwd, bwd, nthreads);
+        const long long emitted = extend_omp(g, code, L, rm, fwd, bwd, nthreads);
+        st.extended += emitted;
+        if (code.size() + 1 < 9) st.level_emitted[code.size() + 1] += emitted;
         st.t_extend += omp_get_wtime() - t0;
 
         for (auto& a : bwd)  // backward children: ascending v_i, then edge label
@@ -610,6 +616,60 @@ struct Miner {
         }
     }
 };
+
+// ---------------------------------------------------------------- level sizes
+// Not a reference function: an estimate of how many rows each DFS level holds when every
+// pattern is frequent, for sizing the workloads (graphs too large to mine on the CPU).
+// Knuth's random-probe estimator of a tree's level sizes (D. E. Knuth, "Estimating the
+// efficiency of backtrack programs", Math. Comp. 29, 1975): a probe starts at a uniformly
+// random single-edge row (weight = number of single-edge rows) and at each level moves to
+// a uniformly random child row among those whose code is canonical (the rows the miner
+// keeps and extends), multiplying its weight by their number.  The mean weight at depth m
+// is the number of rows of canonical m-edge codes; the mean of weight x (all children of
+// the row, canonical or not) is the number of rows extension emits into depth m + 1.
+// Children follow gspan.cu:694-735 exactly (the functions above).
+struct ChildRow {
+    PatternEdge e;
+    int entry;
+};
+
+static void children_of_row(const CSR& g, const DFSCode& code, const std::vector<Edge>& h,
+                            std::vector<ChildRow>& out) {
+    out.clear();
+    const std::vector<int> rm = right_most_path(code);
+    const int v_n = code[rm[0]].to;
+    const int u_n = h[rm[0]].to;
+    const int min_label = code[0].from_label;
+    for (int i = (int)rm.size() - 1; i >= 1; --i) {  // gspan.cu:701-711
+        int idx;
+        if (backward_edge(g, h[rm[i]], u_n, h, &idx))
+            out.push_back(ChildRow{PatternEdge{v_n, code[rm[i]].from, -1, g.edges[idx].label, -1},
+                                   idx});
+    }
+    first_forward(g, u_n, min_label, h, [&](int el, int tl, int i) {  // gspan.cu:713-720
+        out.push_back(ChildRow{PatternEdge{v_n, v_n + 1, -1, el, tl}, i});
+    });
+    for (size_t i = 0; i < rm.size(); ++i) {  // gspan.cu:722-734
+        const int v_i = code[rm[i]].from;
+        other_forward(g, h[rm[i]], min_label, h, [&](int el, int tl, int e) {
+            out.push_back(ChildRow{PatternEdge{v_i, v_n + 1, -1, el, tl}, e});
+        });
+    }
+}
+
+static std::string code_key(const DFSCode& c) {
+    std::string s;
+    s.resize(c.size() * sizeof(PatternEdge));
+    std::memcpy(&s[0], c.data(), s.size());
+    return s;
+}
+
+static uint64_t splitmix(uint64_t& st) {
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
 
 static DFSCode decode(const int* code5, int m) {
     DFSCode c(m);
@@ -673,6 +733,110 @@ int oracle_fsm(const int* offsets, const int* vlabels, const int* edges3, int n,
 }
 
 // is_canonical (gspan.cu:586-606).  Returns 1/0; 2 when the greedy walk ran past the code.
+// Exact level sizes: rows of canonical codes per edge count (rows[1..k]) and child rows
+// emitted into each level (emitted[2..k]) of a full mining run at min_support 1, where
+// every canonical code with a row is reported and (below k) extended.
+int oracle_level_counts(const int* offsets, const int* vlabels, const int* edges3, int n,
+                        int nnz, int k, int nthreads, long long* rows, long long* emitted) {
+    try {
+        if (nthreads <= 0) nthreads = omp_get_max_threads();
+        CSR g = make_csr(offsets, vlabels, edges3, n, nnz);
+        Miner mn(g, k, 1, 0, nthreads, nullptr, nullptr);
+        mn.run();
+        for (int m = 0; m <= k && m < 9; ++m) {
+            rows[m] = mn.st.level_rows[m];
+            emitted[m] = mn.st.level_emitted[m];
+        }
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+// Knuth estimates of the same sizes from `probes` random probes (see children_of_row):
+// rows[m], emitted[m] = mean estimates; rows_sq[m], emitted_sq[m] = mean squared weights
+// (for standard errors).  Probes are split over threads with independent generators.
+int oracle_level_estimate(const int* offsets, const int* vlabels, const int* edges3, int n,
+                          int nnz, int k, long long probes, unsigned long long seed, int nthreads,
+                          double* rows, double* rows_sq, double* emitted, double* emitted_sq) {
+    try {
+        CSR g = make_csr(offsets, vlabels, edges3, n, nnz);
+        if (k < 1 || k > 8) return 1;
+        long long seeds_total = 0;
+        for (int i = 0; i < nnz; ++i)
+            if (vlabels[g.edges[i].from] <= vlabels[g.edges[i].to]) ++seeds_total;
+        for (int m = 0; m <= k; ++m) rows[m] = rows_sq[m] = emitted[m] = emitted_sq[m] = 0;
+        if (seeds_total == 0 || probes <= 0) return 0;
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+        std::vector<double> acc_r(9, 0), acc_r2(9, 0), acc_e(9, 0), acc_e2(9, 0);
+#pragma omp parallel
+        {
+            std::vector<double> r(9, 0), r2(9, 0), e(9, 0), e2(9, 0);
+            std::unordered_map<std::string, bool> canon;
+            std::vector<ChildRow> kids, ckids;
+            uint64_t st = seed * 0x100000001B3ull + (uint64_t)omp_get_thread_num() * 7919ull + 1;
+#pragma omp for schedule(dynamic, 256)
+            for (long long pr = 0; pr < probes; ++pr) {
+                int entry;
+                do {
+                    entry = (int)(splitmix(st) % (uint64_t)nnz);
+                } while (vlabels[g.edges[entry].from] > vlabels[g.edges[entry].to]);
+                const Edge& s0 = g.edges[entry];
+                DFSCode code{PatternEdge{0, 1, vlabels[s0.from], s0.label, vlabels[s0.to]}};
+                std::vector<Edge> h{s0};
+                double w = (double)seeds_total;
+                r[1] += w;
+                r2[1] += w * w;
+                for (int m = 1; m < k; ++m) {
+                    children_of_row(g, code, h, kids);
+                    const double we = w * (double)kids.size();
+                    e[m + 1] += we;
+                    e2[m + 1] += we * we;
+                    ckids.clear();
+                    for (const ChildRow& c : kids) {
+                        code.push_back(c.e);
+                        const std::string key = code_key(code);
+                        auto it = canon.find(key);
+                        bool ok;
+                        if (it == canon.end()) {
+                            bool ov = false;
+                            ok = is_canonical(code, &ov);
+                            canon.emplace(key, ok);
+                        } else {
+                            ok = it->second;
+                        }
+                        code.pop_back();
+                        if (ok) ckids.push_back(c);
+                    }
+                    if (ckids.empty()) break;
+                    const ChildRow& pick = ckids[splitmix(st) % (uint64_t)ckids.size()];
+                    w *= (double)ckids.size();
+                    code.push_back(pick.e);
+                    h.push_back(g.edges[pick.entry]);
+                    r[m + 1] += w;
+                    r2[m + 1] += w * w;
+                }
+            }
+#pragma omp critical
+            for (int m = 0; m < 9; ++m) {
+                acc_r[m] += r[m];
+                acc_r2[m] += r2[m];
+                acc_e[m] += e[m];
+                acc_e2[m] += e2[m];
+            }
+        }
+        for (int m = 0; m <= k; ++m) {
+            rows[m] = acc_r[m] / (double)probes;
+            rows_sq[m] = acc_r2[m] / (double)probes;
+            emitted[m] = acc_e[m] / (double)probes;
+            emitted_sq[m] = acc_e2[m] / (double)probes;
+        }
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
 int oracle_is_canonical(const int* code5, int m) {
     bool overrun = false;
     bool r = is_canonical(decode(code5, m), &overrun);

@@ -64,6 +64,14 @@ def lib():
         L.oracle_pattern_graph.argtypes = [P(c_int), c_int, P(c_int), P(c_int), P(c_int),
                                            P(c_int)]
         L.oracle_pattern_graph.restype = c_int
+        L.oracle_level_counts.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, c_int,
+                                          c_int, P(c_ll), P(c_ll)]
+        L.oracle_level_counts.restype = c_int
+        c_d = ctypes.c_double
+        L.oracle_level_estimate.argtypes = [P(c_int), P(c_int), P(c_int), c_int, c_int, c_int,
+                                            c_ll, ctypes.c_ulonglong, c_int, P(c_d), P(c_d),
+                                            P(c_d), P(c_d)]
+        L.oracle_level_estimate.restype = c_int
         _lib = L
     return _lib
 
@@ -156,6 +164,28 @@ def seed(g):
 
     lib().oracle_seed(*_graph_args(g), CHILD_CB(cb), None)
     return out
+
+
+def level_counts(g, k: int, nthreads: int = 0):
+    """Exact level sizes of a full mining run at min_support 1: (rows, emitted), where
+    rows[m] = rows of the canonical m-edge codes and emitted[m] = child rows extension
+    emits into level m (index 0 unused)."""
+    rows = (ctypes.c_longlong * 9)()
+    emitted = (ctypes.c_longlong * 9)()
+    if lib().oracle_level_counts(*_graph_args(g), k, nthreads, rows, emitted) != 0:
+        raise RuntimeError("oracle_level_counts failed")
+    return list(rows)[:k + 1], list(emitted)[:k + 1]
+
+
+def level_estimate(g, k: int, probes: int, seed: int = 1, nthreads: int = 0):
+    """Knuth random-probe estimates of the same level sizes (oracle_level_estimate):
+    dict of lists rows, rows_se, emitted, emitted_se (standard errors), index 0 unused."""
+    arr = [(ctypes.c_double * 9)() for _ in range(4)]
+    if lib().oracle_level_estimate(*_graph_args(g), k, probes, seed, nthreads, *arr) != 0:
+        raise RuntimeError("oracle_level_estimate failed")
+    r, r2, e, e2 = ([x for x in a][:k + 1] for a in arr)
+    se = lambda m1, m2: [max(b - a * a, 0.0) ** 0.5 / probes ** 0.5 for a, b in zip(m1, m2)]
+    return {"rows": r, "rows_se": se(r, r2), "emitted": e, "emitted_se": se(e, e2)}
 
 
 def pattern_graph(code):
