@@ -361,6 +361,10 @@ int gs_text_parse(const char* text, long long len, int device, gs_text_graph** o
                                   hipMemcpyDeviceToHost, s));
             ICHECK(hipStreamSynchronize(s));
             L = nl + 1;
+            if (L >= (1LL << 31)) {  // one thread per line in launches of < 2^32 work-items
+                (void)hipStreamDestroy(s);
+                return GS_ERR_LIMIT;
+            }
             d_starts = Dev(sizeof(long long) * L);
             hipLaunchKernelGGL(k_nl_write, dim3((unsigned)nb), dim3(IB), 0, s,
                                d_text.as<const char>(), len, d_base.as<long long>(),

@@ -1010,32 +1010,33 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
         p[i] = v;
 }
 
-// distinct images per (set, vertex): popcount of a bitmap / non-empty hash slots
-__global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets,
+// distinct images per (set, vertex): popcount of a bitmap / bytemap, or occupied hash
+// slots.  One wave per (set, vertex) pair, pairs taken grid-stride, so any number of sets
+// fits one launch (a launch holds fewer than 2^32 work-items).
+__global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets, long long nitems,
                                                          const uint32_t* pool,
                                                          long long pool_words, int* counts,
                                                          int* gerr) {
-    const int s = blockIdx.x / GS_MAXV, v = blockIdx.x % GS_MAXV;
-    const SetDesc d = sets[s];
-    __shared__ int s_red[GS_BLOCK / 64];
-    int c = 0;
-    const bool ok = d.off >= 0 && d.off + (long long)d.nv * d.size <= pool_words;
-    if (!ok && v == 0 && threadIdx.x == 0) atomicOr(gerr, GS_GUARD_POOL);
-    if (ok && v < d.nv) {
-        const uint32_t* base = pool + d.off + (long long)v * d.size;
-        if (d.kind != GS_SET_HASH) {  // bitmap bits, or bytemap bytes that are 0 or 1
-            for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += __popc(base[i]);
-        } else {
-            for (int i = threadIdx.x; i < d.size; i += blockDim.x) c += base[i] != 0;
+    constexpr int WPB = GS_BLOCK / 64;  // waves per block
+    const int lane = lane_id();
+    const long long waves = (long long)gridDim.x * WPB;
+    for (long long it = (long long)blockIdx.x * WPB + threadIdx.x / 64; it < nitems; it += waves) {
+        const long long si = it / GS_MAXV;
+        const int v = (int)(it % GS_MAXV);
+        const SetDesc d = sets[si];
+        const bool ok = d.off >= 0 && d.off + (long long)d.nv * d.size <= pool_words;
+        if (!ok && v == 0 && lane == 0) atomicOr(gerr, GS_GUARD_POOL);
+        int c = 0;
+        if (ok && v < d.nv) {
+            const uint32_t* base = pool + d.off + (long long)v * d.size;
+            if (d.kind != GS_SET_HASH) {  // bitmap bits, or bytemap bytes that are 0 or 1
+                for (long long i = lane; i < d.size; i += 64) c += __popc(base[i]);
+            } else {
+                for (long long i = lane; i < d.size; i += 64) c += base[i] != 0;
+            }
         }
-    }
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-    if (lane_id() == 0) s_red[threadIdx.x / 64] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < GS_BLOCK / 64; ++w) t += s_red[w];
-        counts[blockIdx.x] = v < d.nv ? t : 0;
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0) counts[it] = c;
     }
 }
 
@@ -1449,6 +1450,12 @@ __global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes
     }
 }
 
+// A dispatch holds fewer than 2^32 work-items (the grid size of an AQL packet is 32-bit);
+// launchers whose grid follows the data check it with this.
+inline bool grid_fits(long long blocks, int threads) {
+    return blocks > 0 && blocks * (long long)threads < (1LL << 32);
+}
+
 inline int grid_for(long long n, int block, int cap) {
     long long g = (n + block - 1) / block;
     if (g < 1) g = 1;
@@ -1482,6 +1489,7 @@ hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, 
                 unsigned long long* work_full, int* guard_err, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     long long grid = (R + GS_BLOCK - 1) / GS_BLOCK;
+    if (!grid_fits(grid, GS_BLOCK)) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pats, npats, b0, R,
                        L, ctx, work, work_full ? partial : nullptr, guard_err);
     if (work_full)
@@ -1499,7 +1507,7 @@ hipError_t scan_exclusive(const int* in, long long* out, long long n, long long*
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int*>(out), 0, 2);
         return hipGetLastError();
     }
-    if (tmp_elems < nb + 1) return hipErrorInvalidValue;
+    if (tmp_elems < nb + 1 || !grid_fits(nb, GS_BLOCK)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(GS_BLOCK), 0, s, in, n, tmp);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(GS_BLOCK), 0, s, tmp, nb);
     hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(GS_BLOCK), 0, s, in, n, tmp, nb, out);
@@ -1642,6 +1650,7 @@ hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s) {
 
 hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s) {
     if (nkeys <= 0) return hipSuccess;
+    if (!grid_fits((nkeys + 255) / 256, 256)) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_scatter_slots, dim3((nkeys + 255) / 256), dim3(256), 0, s, key_slot, keys,
                        nkeys);
     return hipGetLastError();
@@ -1650,8 +1659,9 @@ hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t 
 hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
                       long long pool_words, int* counts, int* guard_err, hipStream_t s) {
     if (nsets <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_set_counts, dim3(nsets * GS_MAXV), dim3(GS_BLOCK), 0, s, sets, pool,
-                       pool_words, counts, guard_err);
+    const long long nitems = (long long)nsets * GS_MAXV;
+    hipLaunchKernelGGL(k_set_counts, dim3(grid_for(nitems, GS_BLOCK / 64, 16384)), dim3(GS_BLOCK),
+                       0, s, sets, nitems, pool, pool_words, counts, guard_err);
     return hipGetLastError();
 }
 
@@ -1661,6 +1671,7 @@ hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, Layer
     (void)mni_mode;
     if (N <= 0) return hipSuccess;
     long long grid = (N + GS_BLOCK - 1) / GS_BLOCK;
+    if (!grid_fits(grid, GS_BLOCK)) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_support_rows, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pat, N, L,
                        set, pool, pool_words, guard_err);
     return hipGetLastError();

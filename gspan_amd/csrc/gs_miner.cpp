@@ -875,7 +875,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     }
     const KeySpace ks{c->EL, c->VL};
 
-    const long long chunk = c->opts.chunk_rows > 0 ? c->opts.chunk_rows : (16LL << 20);
+    // parent rows per pass (one prep thread per row: launches stay far below 2^32 items)
+    const long long chunk =
+        c->opts.chunk_rows > 0 ? std::min(c->opts.chunk_rows, 1LL << 26) : (16LL << 20);
     const long long crow = std::min(R, chunk);
     const int nchunks = (int)((R + chunk - 1) / chunk);
     DBuf d_ctx(P, sizeof(RowCtx) * std::max<long long>(crow, 1));
@@ -1954,8 +1956,12 @@ int gs_is_canonical_batch(gs_ctx* c, const int* codes5, const int* lens, int nco
     });
 }
 
+// samples per network pass: one thread per (sample, hidden unit) in launches of fewer than
+// 2^32 work-items
+constexpr long long kMaxSamples = 1LL << 24;
+
 int gs_predictor_log(gs_ctx* c, int on, long long capacity) {
-    if (!c || capacity < 0) return GS_ERR_INVALID;
+    if (!c || capacity < 0 || capacity > kMaxSamples) return GS_ERR_INVALID;
     return guard(c, [&]() {
         HIPCHECK(hipSetDevice(c->device));
         c->logging = on != 0;
@@ -2044,7 +2050,7 @@ int gs_predictor_set_weights(gs_ctx* c, const float* weights) {
 }
 
 int gs_mlp_forward(gs_ctx* c, const float* x, long long n, float* y) {
-    if (!c || n < 0 || (n > 0 && (!x || !y))) return GS_ERR_INVALID;
+    if (!c || n < 0 || n > kMaxSamples || (n > 0 && (!x || !y))) return GS_ERR_INVALID;
     return guard(c, [&]() {
         if (n == 0) return;
         DBuf dX(&c->pool, sizeof(float) * n * GS_FEATURES);
@@ -2059,7 +2065,7 @@ int gs_mlp_forward(gs_ctx* c, const float* x, long long n, float* y) {
 
 int gs_mlp_gradients(gs_ctx* c, const float* x, const float* t, long long n, float* grads,
                      float* loss) {
-    if (!c || n <= 0 || !x || !t || !grads) return GS_ERR_INVALID;
+    if (!c || n <= 0 || n > kMaxSamples || !x || !t || !grads) return GS_ERR_INVALID;
     return guard(c, [&]() {
         DBuf dX(&c->pool, sizeof(float) * n * GS_FEATURES), dT(&c->pool, sizeof(float) * n);
         MlpScratch sc(&c->pool, n);

@@ -145,7 +145,8 @@ int gs_is_canonical_batch_noalloc(gs_ctx* ctx, const int* codes5, const int* len
 /* Child-count predictor (MLP 32 -> 128 -> 128 -> 1, ReLU, bf16 weights, fp32 accumulation;
  * no counterpart in the reference, whose count pass is gspan.cu:737-755).
  * gs_predictor_log: while on, every exact count pass appends (features, log1p(count)) of its
- * candidate child keys to a device buffer of at most `capacity` samples.
+ * candidate child keys to a device buffer of at most `capacity` samples (at most 2^24, as
+ * for n of gs_mlp_forward / gs_mlp_gradients).
  * gs_predictor_train: full-batch Adam on the logged samples with the library's own forward
  * and backward kernels; *loss = final mean squared error (log space).
  * Weights are GS_MLP_PARAMS floats: W1[32][128], b1[128], W2[128][128], b2[128], W3[128], b3. */

@@ -36,3 +36,20 @@ def test_estimate_matches_exact_level_sizes(oracle_mod, name, spec, k):
         for key, exact in (("rows", rows[m]), ("emitted", emitted[m])):
             got, se = est[key][m], est[key + "_se"][m]
             assert abs(got - exact) <= 4 * se + 1e-9 * exact, (name, key, m, got, exact, se)
+
+
+@pytest.mark.parametrize("cfg,scale,ms", [("cfg2", 10, 8), ("cfg3", 10, 2), ("cfg4", 10, 10),
+                                          ("cfg5", 10, 3)])
+def test_k2_closed_form_matches_oracle(oracle_mod, cfg, scale, ms):
+    """The closed form of `extended` at k = 2 and of the single-edge supports that the
+    R-MAT scale-16 GPU test checks against (tests/test_gpu_configs.py) equals the oracle's."""
+    from test_gpu_configs import SPECS, _closed_form_extended_k2, _graph
+    g = _graph(cfg, scale)
+    ext, sup = _closed_form_extended_k2(g, ms)
+    pats, st = oracle_mod.fsm(g, 2, ms)
+    assert ext == st["extended"]
+    nl, _, EL, _, _ = SPECS[cfg]
+    seeds = [(c, s) for c, s in pats if len(c) == 1]
+    assert len(seeds) > 10
+    for c, s in seeds:
+        assert s == sup[(c[0][2] * EL + c[0][3]) * nl + c[0][4]]
