@@ -71,6 +71,9 @@ class GsStats(ctypes.Structure):
         ("predicted_steps", ctypes.c_longlong),
         ("fallbacks", ctypes.c_longlong),
         ("spilled_rows", ctypes.c_longlong),
+        ("t_sync_ms", ctypes.c_double),
+        ("t_order_ms", ctypes.c_double),
+        ("t_plan_ms", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:

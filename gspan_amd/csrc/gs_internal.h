@@ -238,6 +238,30 @@ int ctx_device(const gs_ctx* c);
 hipStream_t ctx_stream(const gs_ctx* c);
 void ctx_set_error(gs_ctx* c, const char* msg);
 
+// Step tail on the device (steps with many candidate keys), over the active keys a < na
+// of an ordered compaction (keys, counts) and their canonicality flags:
+// plan_children: set words (plan_sets) and rows (write_rows) of every canonical child;
+// build_sets: its set descriptor (planned from the scanned words, or key_sets[key] when
+// the sets were planned per key), slot = a in key_slot, first row in cursor;
+// supports: sup[a] = MNI support when >= min_sup, else 0; totals[0] += child rows,
+// totals[1] += canonical keys;
+// gather_frequent: keys, counts and first rows of the frequent children (fidx, *nf).
+hipError_t plan_children(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
+                         const int* acnt, const int* flags, int na, int n, bool bytes,
+                         bool plan_sets, bool write_rows, int* set_words, int* rows,
+                         hipStream_t s);
+hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
+                      const int* acnt, const int* flags, int na, int n, bool bytes,
+                      const long long* set_off, const long long* row_off,
+                      const SetDesc* key_sets, SetDesc* sets, int* key_slot,
+                      unsigned long long* cursor, hipStream_t s);
+hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
+                    const int* acnt, const int* flags, int na, const int* counts, int mni_mode,
+                    int min_sup, int* sup, unsigned long long* totals, hipStream_t s);
+hipError_t gather_frequent(const int* fidx, const int* nf, int max_f, const int* akeys,
+                           const int* acnt, const long long* row_off, int* fkey, int* fcnt,
+                           long long* frow, hipStream_t s);
+
 hipError_t fill_i32(int* p, int v, long long n, hipStream_t s);
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s);

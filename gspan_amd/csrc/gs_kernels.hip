@@ -1450,6 +1450,197 @@ __global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes
     }
 }
 
+// ------------------------------------------------------------------ step tail
+// Planning, supports and the frequent children of a step on the device, for steps with
+// many candidate keys: the host then sees only the frequent children.
+
+// Pattern owning batch key `key` (pats ordered by key_base).
+__device__ __forceinline__ int owner_of(const PatDev* __restrict__ pats, int npats, int key) {
+    int lo = 0, hi = npats - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pats[mid].key_base <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Child of active key `key`: its vertex count and whether (GS_MNI_OMP_LITERAL) its vertex 0
+// takes part in the support -- only when an edge after the first leaves vertex 0, as
+// support_omp inserts an edge's source only below layer 0 (gspan.cu:307, 360).
+__device__ __forceinline__ void child_shape(const PatDev& P, int slot, KeySpace ks, int* nv,
+                                            bool* v0) {
+    const int nb = (P.r - 1) * ks.EL;
+    bool from0 = false;
+    for (int j = 1; j < GS_MAXE; ++j)
+        if (j < P.m && P.cfrom[j] == 0) from0 = true;
+    int from;
+    if (slot < nb) {
+        from = P.v_n;  // backward: from the rightmost vertex
+        *nv = P.nv;
+    } else {
+        const int seg = (slot - nb) / (ks.EL * ks.VL);
+        from = seg == 0 ? P.v_n : P.cfrom[P.rm[seg - 1]];
+        *nv = P.nv + 1;
+    }
+    *v0 = from0 || from == 0;
+}
+
+__device__ __forceinline__ long long child_rows(int c) {  // a negative count: > INT_MAX
+    return c < 0 ? (long long)INT_MAX + 1 : (long long)c;
+}
+
+// Set of a child with `rows` rows (the host's plan_set): an open-addressing table of at
+// least 2 * rows slots (16 at least, at most 2n) when that is smaller than a bitmap, else a
+// bytemap or a bitmap; words = vertices x words per vertex set, padded to 4 words.
+__device__ __forceinline__ void plan_one(long long rows, int nv, int n, int bytes, int* kind,
+                                         int* size, long long* words) {
+    const long long bw = (((long long)n + 31) / 32 + 3) / 4 * 4;
+    long long cap = 16;
+    const long long want = min(2 * rows, 2LL * n);
+    while (cap < want) cap <<= 1;
+    if (cap < bw) {
+        *kind = GS_SET_HASH;
+        *size = (int)cap;
+    } else if (bytes) {
+        *kind = GS_SET_BYTEMAP;
+        *size = (int)(((long long)n + 15) / 16 * 4);
+    } else {
+        *kind = GS_SET_BITMAP;
+        *size = (int)bw;
+    }
+    *words = ((long long)nv * *size + 3) / 4 * 4;
+}
+
+// Per active key a < na: set words and rows of its child (0 unless canonical; no set words
+// when the sets were planned for every key beforehand, rows only when rows are written).
+__global__ void k_plan_children(const PatDev* __restrict__ pats, int npats, KeySpace ks,
+                                const int* __restrict__ akeys, const int* __restrict__ acnt,
+                                const int* __restrict__ flags, int na, int n, int bytes,
+                                int plan_sets, int write_rows, int* set_words, int* rows) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
+        int w = 0, r = 0;
+        if (flags[a]) {
+            const int key = akeys[a];
+            const PatDev& P = pats[owner_of(pats, npats, key)];
+            int nv;
+            bool v0;
+            child_shape(P, key - P.key_base, ks, &nv, &v0);
+            const long long cr = child_rows(acnt[a]);
+            if (plan_sets) {
+                int kind, size;
+                long long words;
+                plan_one(cr, nv, n, bytes, &kind, &size, &words);
+                w = (int)words;
+            }
+            if (write_rows) r = (int)min(cr, (long long)INT_MAX);
+        }
+        set_words[a] = w;
+        rows[a] = r;
+    }
+}
+
+// Set descriptor, slot and row cursor of every active key a < na (slot = a).  Keys that
+// are not canonical get an empty descriptor (nv = 0) and no slot.
+__global__ void k_build_sets(const PatDev* __restrict__ pats, int npats, KeySpace ks,
+                             const int* __restrict__ akeys, const int* __restrict__ acnt,
+                             const int* __restrict__ flags, int na, int n, int bytes,
+                             const long long* __restrict__ set_off,
+                             const long long* __restrict__ row_off,
+                             const SetDesc* __restrict__ key_sets, SetDesc* sets, int* key_slot,
+                             unsigned long long* cursor) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
+        SetDesc d{0, GS_SET_HASH, 1, 0, 0};
+        const int key = akeys[a];
+        if (flags[a]) {
+            if (key_sets) {  // planned per key beforehand (single pass)
+                d = key_sets[key];
+            } else {
+                const PatDev& P = pats[owner_of(pats, npats, key)];
+                int nv;
+                bool v0;
+                child_shape(P, key - P.key_base, ks, &nv, &v0);
+                long long words;
+                plan_one(child_rows(acnt[a]), nv, n, bytes, &d.kind, &d.size, &words);
+                d.nv = nv;
+                d.off = set_off[a];
+            }
+            if (key_slot) key_slot[key] = (int)a;
+        }
+        sets[a] = d;
+        if (cursor) cursor[a] = row_off ? (unsigned long long)row_off[a] : 0ull;
+    }
+}
+
+// Support of the child of active key a (gspan.cu:282-288 / 367-373: the least non-zero
+// count of distinct images over its vertices, vertex 0 only where it takes part);
+// sup[a] = the support when it reaches min_sup, else 0.  Totals: [0] += child rows of all
+// active keys (a negative count adds 2^31), [1] += canonical keys.
+__global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace ks,
+                           const int* __restrict__ akeys, const int* __restrict__ acnt,
+                           const int* __restrict__ flags, int na, const int* __restrict__ counts,
+                           int mni_mode, int min_sup, int* sup, unsigned long long* totals) {
+    __shared__ unsigned long long s_t[2][GS_BLOCK / 64];
+    unsigned long long rows = 0, canon = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
+        rows += (unsigned long long)child_rows(acnt[a]);
+        int out = 0;
+        if (flags[a]) {
+            ++canon;
+            const int key = akeys[a];
+            const PatDev& P = pats[owner_of(pats, npats, key)];
+            int nv;
+            bool v0;
+            child_shape(P, key - P.key_base, ks, &nv, &v0);
+            if (mni_mode != GS_MNI_OMP_LITERAL) v0 = true;
+            long long s = INT_MAX;
+            for (int v = 0; v < nv; ++v) {
+                if (v == 0 && !v0) continue;
+                const int c = counts[a * GS_MAXV + v];
+                if (c > 0 && c < s) s = c;
+            }
+            out = s >= min_sup ? (int)s : 0;
+        }
+        sup[a] = out;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rows += __shfl_xor(rows, off);
+        canon += __shfl_xor(canon, off);
+    }
+    if (lane_id() == 0) {
+        s_t[0][threadIdx.x / 64] = rows;
+        s_t[1][threadIdx.x / 64] = canon;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long r = 0, k = 0;
+        for (int w = 0; w < GS_BLOCK / 64; ++w) {
+            r += s_t[0][w];
+            k += s_t[1][w];
+        }
+        if (r) atomicAdd(&totals[0], r);
+        if (k) atomicAdd(&totals[1], k);
+    }
+}
+
+// The frequent children's keys, counts and first rows, by their active index.
+__global__ void k_gather_frequent(const int* __restrict__ fidx, const int* __restrict__ nf,
+                                  const int* __restrict__ akeys, const int* __restrict__ acnt,
+                                  const long long* __restrict__ row_off, int* fkey, int* fcnt,
+                                  long long* frow) {
+    const int F = *nf;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += stride) {
+        const int a = fidx[i];
+        fkey[i] = akeys[a];
+        fcnt[i] = acnt[a];
+        frow[i] = row_off ? row_off[a] : 0;
+    }
+}
+
 // A dispatch holds fewer than 2^32 work-items (the grid size of an AQL packet is 32-bit);
 // launchers whose grid follows the data check it with this.
 inline bool grid_fits(long long blocks, int threads) {
@@ -1662,6 +1853,48 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
     const long long nitems = (long long)nsets * GS_MAXV;
     hipLaunchKernelGGL(k_set_counts, dim3(grid_for(nitems, GS_BLOCK / 64, 16384)), dim3(GS_BLOCK),
                        0, s, sets, nitems, pool, pool_words, counts, guard_err);
+    return hipGetLastError();
+}
+
+hipError_t plan_children(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
+                         const int* acnt, const int* flags, int na, int n, bool bytes,
+                         bool plan_sets, bool write_rows, int* set_words, int* rows,
+                         hipStream_t s) {
+    if (na <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plan_children, dim3(grid_for(na, 256, 16384)), dim3(256), 0, s, pats,
+                       npats, ks, akeys, acnt, flags, na, n, bytes ? 1 : 0, plan_sets ? 1 : 0,
+                       write_rows ? 1 : 0, set_words, rows);
+    return hipGetLastError();
+}
+
+hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
+                      const int* acnt, const int* flags, int na, int n, bool bytes,
+                      const long long* set_off, const long long* row_off,
+                      const SetDesc* key_sets, SetDesc* sets, int* key_slot,
+                      unsigned long long* cursor, hipStream_t s) {
+    if (na <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_build_sets, dim3(grid_for(na, 256, 16384)), dim3(256), 0, s, pats, npats,
+                       ks, akeys, acnt, flags, na, n, bytes ? 1 : 0, set_off, row_off, key_sets,
+                       sets, key_slot, cursor);
+    return hipGetLastError();
+}
+
+hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
+                    const int* acnt, const int* flags, int na, const int* counts, int mni_mode,
+                    int min_sup, int* sup, unsigned long long* totals, hipStream_t s) {
+    if (na <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_supports, dim3(grid_for(na, GS_BLOCK, 16384)), dim3(GS_BLOCK), 0, s,
+                       pats, npats, ks, akeys, acnt, flags, na, counts, mni_mode, min_sup, sup,
+                       totals);
+    return hipGetLastError();
+}
+
+hipError_t gather_frequent(const int* fidx, const int* nf, int max_f, const int* akeys,
+                           const int* acnt, const long long* row_off, int* fkey, int* fcnt,
+                           long long* frow, hipStream_t s) {
+    if (max_f <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_frequent, dim3(grid_for(max_f, 256, 16384)), dim3(256), 0, s,
+                       fidx, nf, akeys, acnt, row_off, fkey, fcnt, frow);
     return hipGetLastError();
 }
 

@@ -166,6 +166,8 @@ struct HPat {
     int fl0 = 0;                                           // from label of edge 0
     long long row_begin = 0, row_count = 0;
     int support = 0;
+    int report_id = -1;      // index of this pattern's report, once reported
+    int parent_report = -1;  // report of the parent pattern (-1 for single edges)
 
     int nv() const {
         int mx = 0;
@@ -216,7 +218,8 @@ struct OrderKey {
 struct Report {
     int m;
     int code5[5 * GS_MAXE];  // original labels, reference placeholder form
-    OrderKey order;
+    uint64_t edge_key;       // order word of the last code edge (see order_key)
+    int parent;              // report of the parent pattern, -1 for a single edge
     int support;
 };
 
@@ -288,7 +291,9 @@ struct gs_ctx {
     std::vector<PendingD2H> pending_d2h;
     DBuf d_offsets, d_src, d_adj, d_vlab, d_adjl, d_eidl, d_lstart;
     DevGraph dg{};
-    DBuf d_misc;  // [0] guard word, [1] n_active, [2] err, [3] overflow, [4..5] int64 total
+    // [0] guard word, [1] n_active, [2] err, [3] overflow, [4..5] int64 total,
+    // [6..7] items without the label filter, [8..11] two 64-bit step totals
+    DBuf d_misc;
     DBuf d_tiles; // per-XCD tile queues of the item kernels
     LaunchTuner tuner;  // launch shape of the item kernels
     std::string err;
@@ -336,6 +341,10 @@ struct gs_ctx {
     int* err_flag() const { return d_misc.as<int>() + 2; }
     int* overflow_flag() const { return d_misc.as<int>() + 3; }
     long long* total64() const { return reinterpret_cast<long long*>(d_misc.as<int>() + 4); }
+    // child rows and canonical keys of a step whose tail runs on the device
+    unsigned long long* step_totals() const {
+        return reinterpret_cast<unsigned long long*>(d_misc.as<int>() + 8);
+    }
     // items of a step without the label filter (the algorithmic-bytes figure counts them)
     unsigned long long* work_full64() const {
         return reinterpret_cast<unsigned long long*>(d_misc.as<int>() + 6);
@@ -365,7 +374,10 @@ void sync(gs_ctx* c) {
                                 c->stream));
         c->pending_d2h.push_back(PendingD2H{&c->guard_host, p, sizeof(int)});
     }
+    const auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipStreamSynchronize(c->stream));
+    c->st.t_sync_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (const PendingD2H& p : c->pending_d2h) std::memcpy(p.dst, p.staged, p.bytes);
     c->pending_d2h.clear();
     c->stage.reset();
@@ -467,6 +479,8 @@ PatDev to_dev(const HPat& h) {
 void decode_child_into(const HPat& p, const PatDev& d, int slot, int EL, int VL, HPat& c) {
     c = p;
     c.m = p.m + 1;
+    c.report_id = -1;
+    c.parent_report = p.report_id;
     const int j = p.m;
     const int nb = (d.r - 1) * EL;
     if (slot < nb) {
@@ -493,36 +507,63 @@ HPat decode_child(const HPat& p, const PatDev& d, int slot, int EL, int VL) {
 
 // Order of the reference's report lines: seeds by (fl, el, tl); within a node, backward
 // children (ascending v_i, el) before forward children (descending v_i, ascending el, tl);
-// a pattern precedes its descendants (gspan.cu:1142-1168, 1185).
+// a pattern precedes its descendants (gspan.cu:1142-1168, 1185).  Each code edge maps to
+// one 64-bit word, 16 bits per field by order-preserving maps (label ranks are below
+// 2^16, vertex ids within +-8), and a code's position is the sequence of its words.
+uint64_t edge_word(const HPat& h, int j) {
+    auto put = [](int a, int b, int c, int d) {
+        return (uint64_t)a << 48 | (uint64_t)b << 32 | (uint64_t)c << 16 | (uint64_t)d;
+    };
+    if (j == 0) return put(h.fl0, h.el[0], h.tl[0], 0);  // label ranks
+    // backward: (0, to, el, 0); forward: (1, -from, el, tl); vertex ids shifted by 8
+    if (h.f[j] > h.t[j]) return put(0, h.t[j] + 8, h.el[j], 0);
+    return put(1, 8 - h.f[j], h.el[j], h.tl[j]);
+}
+
 OrderKey order_key(const HPat& h) {
     OrderKey k;
-    // a, b, c, d each mapped into [0, 2^16) keeping their order
-    auto put = [&](int a, int b, int c, int d) {
-        k.w[k.n++] = (uint64_t)a << 48 | (uint64_t)b << 32 | (uint64_t)c << 16 | (uint64_t)d;
-    };
-    put(h.fl0, h.el[0], h.tl[0], 0);  // label ranks
-    for (int j = 1; j < h.m; ++j) {
-        // backward: (0, to, el, 0); forward: (1, -from, el, tl); vertex ids shifted by 8
-        if (h.f[j] > h.t[j]) put(0, h.t[j] + 8, h.el[j], 0);
-        else put(1, 8 - h.f[j], h.el[j], h.tl[j]);
-    }
+    for (int j = 0; j < h.m; ++j) k.w[k.n++] = edge_word(h, j);
     return k;
 }
 
-// Reports into the reference's order: indices are sorted (a Report is ~200 bytes), then the
-// reports moved once.  Distinct patterns have distinct keys, so the order is total.
-void sort_reports(gs_ctx* c) {
+// Reports into the reference's order: the reported patterns form a tree (a pattern's
+// parent is reported before it is extended), and the order is the tree's pre-order with
+// each node's children -- and the single edges -- sorted by their last edge's word.
+// Siblings have distinct last edges, so the order is total.  Linear apart from the sorts
+// of sibling groups.
+void order_reports(gs_ctx* c) {
     std::vector<Report>& r = c->reports;
-    std::vector<int> idx(r.size());
-    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
-    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return r[a].order < r[b].order; });
+    const int n = (int)r.size();
+    std::vector<int> start(n + 2, 0), kids(n);
+    for (const Report& x : r) ++start[x.parent + 2];  // parent -1 counts at index 1
+    for (int i = 0; i <= n; ++i) start[i + 1] += start[i];
+    {
+        std::vector<int> at(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; ++i) kids[at[r[i].parent + 1]++] = i;
+    }
+    // group g (parent g - 1) occupies kids[start[g] .. start[g+1])
+    for (int g = 0; g <= n; ++g)
+        std::sort(kids.begin() + start[g], kids.begin() + start[g + 1],
+                  [&](int a, int b) { return r[a].edge_key < r[b].edge_key; });
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<int> stack;
+    for (int i = start[1] - 1; i >= start[0]; --i) stack.push_back(kids[i]);
+    while (!stack.empty()) {
+        const int x = stack.back();
+        stack.pop_back();
+        order.push_back(x);
+        for (int i = start[x + 2] - 1; i >= start[x + 1]; --i) stack.push_back(kids[i]);
+    }
+    if ((int)order.size() != n) throw GsError(GS_ERR_INTERNAL, "report tree is not connected");
     std::vector<Report> sorted;
-    sorted.reserve(r.size());
-    for (int i : idx) sorted.push_back(std::move(r[i]));
+    sorted.reserve(n);
+    for (int i : order) sorted.push_back(std::move(r[i]));
     r.swap(sorted);
 }
 
-void add_report(gs_ctx* c, const HPat& h) {
+// Appends the report of h; returns its index.
+int add_report(gs_ctx* c, const HPat& h) {
     Report r;
     r.m = h.m;
     for (int j = 0; j < h.m; ++j) {
@@ -532,10 +573,12 @@ void add_report(gs_ctx* c, const HPat& h) {
         r.code5[5 * j + 3] = c->el_vals[h.el[j]];
         r.code5[5 * j + 4] = h.tl[j] < 0 ? -1 : c->vl_vals[h.tl[j]];
     }
-    r.order = order_key(h);
+    r.edge_key = edge_word(h, h.m - 1);
+    r.parent = h.parent_report;
     r.support = h.support;
     c->reports.push_back(std::move(r));
     c->st.reported++;
+    return (int)c->reports.size() - 1;
 }
 
 // Support from the per-vertex distinct-image counts (gspan.cu:282-288 / 367-373).
@@ -693,11 +736,16 @@ void resolve_timing(gs_ctx* c) {
 // When the key space is small they are copied whole in the same synchronisation as the
 // active count (one host round trip instead of two); otherwise after it.
 constexpr long long kSpeculativeKeys = 1 << 16;
+// steps with more active keys plan their children and compute supports on the device
+constexpr long long kDeviceTailKeys = 1 << 16;
 struct ActiveHost {
     std::vector<int> keys, cnt, flag;
 };
+// limit: when more than `limit` keys are active only the head is read (the step then
+// continues on the device).
 void fetch_active(gs_ctx* c, long long K, const int* d_active, const int* d_acnt,
-                  const int* d_flags, int* head, int nhead, ActiveHost& h) {
+                  const int* d_flags, int* head, int nhead, ActiveHost& h,
+                  long long limit = LLONG_MAX) {
     const bool spec = K <= kSpeculativeKeys;
     d2h(c, head, c->n_active(), (size_t)nhead);
     if (spec) {
@@ -718,6 +766,7 @@ void fetch_active(gs_ctx* c, long long K, const int* d_active, const int* d_acnt
         if (d_flags) h.flag.resize(A);
         return;
     }
+    if (A > limit) return;
     h.keys.resize(A);
     h.cnt.resize(A);
     d2h(c, h.keys.data(), d_active, (size_t)A);
@@ -818,7 +867,10 @@ struct Layer {
 };
 
 struct StepOut {
-    std::vector<ChildInfo> children;  // every candidate child (active key), key order
+    // children with a canonical code (every child when all_live), in key order
+    std::vector<ChildInfo> children;
+    long long candidates = 0;         // child codes that occurred (active keys)
+    long long canonical = 0;          // of them with a canonical code
     LevelRows rows;                   // child rows of the written keys
     long long rows_total = 0;         // rows in `rows`
     long long emitted = 0;
@@ -1094,18 +1146,21 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                     const int a = order[oi];
                     const int key = akeys[a];
                     while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
-                    ChildInfo ci;
-                    if (aflag[a] || all_live)  // other children only count as candidates
-                        ci.pat = decode_child(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL);
+                    emitted += child_count(acnt[a]);
+                    if (!aflag[a]) continue;  // other children only count as candidates
+                    out.children.emplace_back();
+                    ChildInfo& ci = out.children.back();
+                    decode_child_into(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL,
+                                      ci.pat);
                     ci.count = child_count(acnt[a]);
-                    ci.canonical = aflag[a];
+                    ci.canonical = 1;
                     ci.support = 0;
                     ci.row_off = (long long)row_off[key];
-                    emitted += ci.count;
-                    if (ci.canonical) live_sets.push_back(psets[key]);
-                    out.children.push_back(ci);
+                    live_sets.push_back(psets[key]);
                 }
                 out.emitted = emitted;
+                out.candidates = A;
+                out.canonical = (long long)out.children.size();
                 const int NL = (int)live_sets.size();
                 if (NL > 0) {
                     DBuf d_ls(P, sizeof(SetDesc) * NL), d_counts(P, sizeof(int) * NL * GS_MAXV);
@@ -1199,7 +1254,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     int na_err[2] = {0, 0};
     d2h(c, &W_full, c->work_full64(), 1);
     ActiveHost ah;
-    fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(), na_err, 2, ah);
+    fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(), na_err, 2, ah,
+                 all_live ? LLONG_MAX : kDeviceTailKeys);
     resolve_timing(c);
     for (long long w : Wc_host) W_total += w;
     c->st.items += W_total;
@@ -1207,6 +1263,163 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
     if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
     const int A = na_err[0];
+    // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
+    // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
+    auto account_bytes = [&](long long emitted) {
+        double bytes = 16.0 * (double)W_full;
+        for (int i = 0; i < B; ++i)
+            bytes += (double)pats[i].row_count *
+                     (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
+        if (!single) c->st.alg_bytes_count += bytes;
+        c->st.alg_bytes_fill += bytes + 8.0 * (double)emitted;
+    };
+
+    // ---- many candidate keys: plan, fill, supports and frequent children on the device
+    if (!all_live && A > kDeviceTailKeys) {
+        const auto t_plan0 = std::chrono::steady_clock::now();
+        const bool plan_sets = !single;
+        DBuf d_words(P, sizeof(int) * A), d_rows(P, sizeof(int) * A);
+        DBuf d_soff(P, sizeof(long long) * (A + 1)), d_roff(P, sizeof(long long) * (A + 1));
+        const long long te = gsk::scan_tmp_elems(A);
+        DBuf d_stmp(P, sizeof(long long) * te);
+        bool bytes = want_bytes(c);
+        long long pool_words = 0, rows_total = 0;
+        auto plan = [&](bool by) {
+            HIPCHECK(gsk::plan_children(d_pats.as<PatDev>(), B, ks, d_active.as<int>(),
+                                        d_acnt.as<int>(), d_flags.as<int>(), A, c->n, by,
+                                        plan_sets, write_rows, d_words.as<int>(),
+                                        d_rows.as<int>(), c->stream));
+            HIPCHECK(gsk::scan_exclusive(d_words.as<int>(), d_soff.as<long long>(), A,
+                                         d_stmp.as<long long>(), te, c->stream));
+            HIPCHECK(gsk::scan_exclusive(d_rows.as<int>(), d_roff.as<long long>(), A,
+                                         d_stmp.as<long long>(), te, c->stream));
+            d2h(c, &pool_words, d_soff.as<long long>() + A, 1);
+            d2h(c, &rows_total, d_roff.as<long long>() + A, 1);
+            sync(c);
+        };
+        auto need_of = [&]() {
+            return (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
+                   (size_t)A * (sizeof(SetDesc) + 8 + 4 * GS_MAXV + 24) + (size_t)K * 4;
+        };
+        plan(bytes);
+        if (plan_sets && bytes && c->opts.set_kind == GS_SETS_AUTO &&
+            c->pool.in_use() + need_of() > c->budget) {
+            bytes = false;  // bytemaps do not fit: bitmaps take an eighth of the memory
+            plan(bytes);
+        }
+        if (rows_total >= INT_MAX) {
+            if (B > 1) return STEP_SPLIT;
+            throw GsError(GS_ERR_BUDGET, "child rows of one pattern exceed 2^31");
+        }
+        bool spill = false;
+        if (c->pool.in_use() + need_of() > c->budget) {
+            if (B > 1) return STEP_SPLIT;
+            spill = write_rows && rows_total > 0;  // one pattern: rows to pinned host memory
+        }
+        DBuf d_sets(P, sizeof(SetDesc) * A), d_cursor(P, sizeof(unsigned long long) * A);
+        DBuf d_slot(P, sizeof(int) * K);
+        DBuf d_pool(P, sizeof(uint32_t) * std::max<long long>(plan_sets ? pool_words : 0, 4));
+        if (write_rows) {
+            if (spill) {
+                out.rows.on_host(rows_total);
+                c->st.spilled_rows += rows_total;
+            } else {
+                out.rows.on_device(P, rows_total);
+            }
+            out.rows_total = rows_total;
+        }
+        HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
+        if (plan_sets) HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
+        HIPCHECK(gsk::build_sets(d_pats.as<PatDev>(), B, ks, d_active.as<int>(),
+                                 d_acnt.as<int>(), d_flags.as<int>(), A, c->n, bytes,
+                                 d_soff.as<long long>(), write_rows ? d_roff.as<long long>() : nullptr,
+                                 single ? d_all_sets.as<SetDesc>() : nullptr, d_sets.as<SetDesc>(),
+                                 d_slot.as<int>(), d_cursor.as<unsigned long long>(), c->stream));
+        c->st.t_plan_ms += std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - t_plan0)
+                               .count();
+        if (!single) {  // pass 2: child rows and MNI sets
+            for (int ci = 0; ci < nchunks; ++ci) {
+                const long long b0 = (long long)ci * chunk;
+                const long long Rc =
+                    nchunks > 1 ? prep_chunk(ci, false) : std::min(chunk, R - b0);
+                const gsk::ItemsLaunch lc =
+                    shape(write_rows ? GS_ITEMS_ROWS : GS_ITEMS_SETS, false, Rc);
+                time_begin();
+                HIPCHECK(gsk::items_fill(
+                    lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(), d_woff.as<long long>(),
+                    b0, Rc, ks, nullptr, d_slot.as<int>(), d_cursor.as<unsigned long long>(),
+                    nullptr, write_rows ? out.rows.ptr() : nullptr, d_sets.as<SetDesc>(),
+                    d_pool.as<uint32_t>(), c->overflow_flag(), c->tq, A, false,
+                    guard_of(A, write_rows ? rows_total : 0, std::max<long long>(pool_words, 4)),
+                    c->stream));
+                time_end(1, &Wc_host[ci]);
+                c->st.fill_launches++;
+            }
+        }
+        // supports, then the frequent children in key order
+        DBuf d_counts(P, sizeof(int) * (size_t)A * GS_MAXV), d_sup(P, sizeof(int) * A);
+        HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), A,
+                                 single ? d_all_pool.as<uint32_t>() : d_pool.as<uint32_t>(),
+                                 std::max<long long>(single ? all_words : pool_words, 4),
+                                 d_counts.as<int>(), c->guard_word(), c->stream));
+        HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
+        HIPCHECK(gsk::supports(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), d_acnt.as<int>(),
+                               d_flags.as<int>(), A, d_counts.as<int>(), c->mni_mode, c->min_sup,
+                               d_sup.as<int>(), c->step_totals(), c->stream));
+        DBuf d_fidx(P, sizeof(int) * A), d_fsup(P, sizeof(int) * A);
+        compact_ordered(c, d_sup.as<int>(), A, d_fidx.as<int>(), d_fsup.as<int>());
+        DBuf d_fkey(P, sizeof(int) * A), d_fcnt(P, sizeof(int) * A), d_frow(P, sizeof(long long) * A);
+        HIPCHECK(gsk::gather_frequent(d_fidx.as<int>(), c->n_active(), A, d_active.as<int>(),
+                                      d_acnt.as<int>(),
+                                      write_rows ? d_roff.as<long long>() : nullptr,
+                                      d_fkey.as<int>(), d_fcnt.as<int>(), d_frow.as<long long>(),
+                                      c->stream));
+        int F = 0;
+        unsigned long long totals[2] = {0, 0};
+        d2h(c, &F, c->n_active(), 1);
+        d2h(c, totals, c->step_totals(), 2);
+        sync(c);
+        resolve_timing(c);
+        std::vector<int> fkey(F), fcnt(F), fsup(F);
+        std::vector<long long> frow(F);
+        d2h(c, fkey.data(), d_fkey.as<int>(), F);
+        d2h(c, fcnt.data(), d_fcnt.as<int>(), F);
+        d2h(c, fsup.data(), d_fsup.as<int>(), F);
+        d2h(c, frow.data(), d_frow.as<long long>(), F);
+        sync(c);
+        const auto t_dec0 = std::chrono::steady_clock::now();
+        if (!c->child_bufs.empty()) {
+            out.children = std::move(c->child_bufs.back());
+            c->child_bufs.pop_back();
+            out.children.clear();
+        }
+        out.children.reserve(F);
+        int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
+        for (int i = 0; i < F; ++i) {
+            const int key = fkey[i];
+            while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
+            out.children.emplace_back();
+            ChildInfo& ch = out.children.back();
+            decode_child_into(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL, ch.pat);
+            ch.count = child_count(fcnt[i]);
+            ch.canonical = 1;
+            ch.support = fsup[i];
+            ch.row_off = write_rows ? frow[i] : -1;
+            ch.pat.support = ch.support;
+            ch.pat.row_begin = ch.row_off;
+            ch.pat.row_count = ch.count;
+        }
+        out.candidates = A;
+        out.canonical = (long long)totals[1];
+        out.emitted = (long long)totals[0];
+        c->st.t_plan_ms += std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - t_dec0)
+                               .count();
+        account_bytes(out.emitted);
+        c->st.batches++;
+        return STEP_OK;
+    }
     const std::vector<int>& akeys = ah.keys;
     const std::vector<int>& acnt = ah.cnt;
     const std::vector<int>& aflag = ah.flag;
@@ -1215,6 +1428,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     for (int i = 0; i < A; ++i) order[i] = i;
 
     // ---- plan child rows and sets of the live keys
+    const auto t_plan0 = std::chrono::steady_clock::now();
     std::vector<int> live_keys, live_child;
     std::vector<unsigned long long> cursor;
     std::vector<SetDesc> sets;
@@ -1223,35 +1437,35 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         c->child_bufs.pop_back();
         out.children.clear();
     }
-    out.children.reserve(A);
-    long long rows_total = 0, pool_words = 0, emitted = 0;
+    long long rows_total = 0, pool_words = 0, emitted = 0, canon_cnt = 0;
     int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
     for (int oi = 0; oi < A; ++oi) {
         const int a = order[oi];
         const int key = akeys[a];
         while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
+        emitted += child_count(acnt[a]);
+        canon_cnt += aflag[a] ? 1 : 0;
+        if (!(aflag[a] || all_live)) continue;  // other children only count as candidates
         out.children.emplace_back();  // built in place: a ChildInfo is ~170 bytes
         ChildInfo& ci = out.children.back();
-        if (aflag[a] || all_live)  // other children only count as candidates
-            decode_child_into(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL, ci.pat);
+        decode_child_into(pats[p], pd[p], key - pd[p].key_base, c->EL, c->VL, ci.pat);
         ci.count = child_count(acnt[a]);
         ci.canonical = aflag[a];
         ci.support = 0;
         ci.row_off = -1;
-        emitted += ci.count;
-        if (ci.canonical || all_live) {
-            live_child.push_back((int)out.children.size() - 1);
-            live_keys.push_back(key);
-            if (write_rows) {
-                ci.row_off = rows_total;
-                cursor.push_back((unsigned long long)rows_total);
-                rows_total += ci.count;
-            } else {
-                cursor.push_back(0);
-            }
+        live_child.push_back((int)out.children.size() - 1);
+        live_keys.push_back(key);
+        if (write_rows) {
+            ci.row_off = rows_total;
+            cursor.push_back((unsigned long long)rows_total);
+            rows_total += ci.count;
+        } else {
+            cursor.push_back(0);
         }
     }
     out.emitted = emitted;
+    out.candidates = A;
+    out.canonical = canon_cnt;
     const int NL = (int)live_keys.size();
     if (rows_total > INT_MAX) {
         if (B > 1) return STEP_SPLIT;
@@ -1285,6 +1499,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         // cannot hold fails with GS_ERR_HIP)
         spill = write_rows && rows_total > 0;
     }
+    c->st.t_plan_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_plan0)
+            .count();
     if (NL == 0) {
         c->st.batches++;
         return STEP_OK;
@@ -1332,16 +1549,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             c->st.fill_launches++;
         }
     }
-    // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
-    // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
-    {
-        double bytes = 16.0 * (double)W_full;
-        for (int i = 0; i < B; ++i)
-            bytes += (double)pats[i].row_count *
-                     (20.0 * m + 8.0 * (pd[i].r + 1) - 16.0 * (pd[i].r - 1));
-        if (!single) c->st.alg_bytes_count += bytes;
-        c->st.alg_bytes_fill += bytes + 8.0 * (double)emitted;
-    }
+    account_bytes(emitted);
     HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
                              single ? d_all_pool.as<uint32_t>() : d_pool.as<uint32_t>(),
                              std::max<long long>(single ? all_words : pool_words, 4),
@@ -1384,13 +1592,12 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) {
         return;
     }
     c->st.extended += so.emitted;
+    c->st.candidates += so.candidates;
+    c->st.canonical += so.canonical;
     std::vector<HPat> next;
-    for (auto& ch : so.children) {
-        c->st.candidates++;
-        if (!ch.canonical) continue;
-        c->st.canonical++;
+    for (auto& ch : so.children) {  // canonical children (the others only count)
         if (ch.support < c->min_sup) continue;
-        add_report(c, ch.pat);
+        ch.pat.report_id = add_report(c, ch.pat);
         if (!last) next.push_back(ch.pat);
     }
     if (!last && !next.empty()) {
@@ -1513,7 +1720,7 @@ void mine_seeds(gs_ctx* c) {
     for (int i = 0; i < NL; ++i) {
         seeds[i].support = support_of(seeds[i], &counts[(size_t)i * GS_MAXV], c->mni_mode);
         if (seeds[i].support < c->min_sup) continue;
-        add_report(c, seeds[i]);
+        seeds[i].report_id = add_report(c, seeds[i]);
         freq.push_back(seeds[i]);
     }
     if (rows && !freq.empty()) {
@@ -1802,7 +2009,11 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         clear_guard(c);
         c->reset_tile_queues();
         mine_seeds(c);
-        sort_reports(c);
+        const auto t_ord = std::chrono::steady_clock::now();
+        order_reports(c);
+        c->st.t_order_ms += std::chrono::duration<double, std::milli>(
+                                std::chrono::steady_clock::now() - t_ord)
+                                .count();
         auto t1 = std::chrono::steady_clock::now();
         c->st.t_mine_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         c->st.peak_device_bytes = c->pool.peak();

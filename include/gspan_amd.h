@@ -93,6 +93,9 @@ typedef struct gs_stats {
     long long predicted_steps; /* steps run in one pass from predicted sizes               */
     long long fallbacks;       /* predicted steps redone exactly after a misprediction     */
     long long spilled_rows;    /* level rows kept in pinned host memory (budget exceeded)  */
+    double t_sync_ms;          /* host time waiting for the device at synchronisations    */
+    double t_order_ms;         /* host time putting the reports into the reference order  */
+    double t_plan_ms;          /* host time turning candidate keys into children and sets */
 } gs_stats;
 
 /* Receives one frequent pattern.  Return 0 to continue, non-zero to stop mining. */
