@@ -4,14 +4,25 @@ A *step* is one full mining run (gs_mine: seeds, every extension level, MNI supp
 canonicality checks, ordered report) over one synthetic graph already resident in HBM.
 Prints ONE JSON line.
 
-Workloads (seeded, synthetic, reference binary-CSR layout):
-  rmat20    R-MAT scale 20 (a,b,c,d = .57,.19,.19,.05), edge factor 8, 20 vertex labels
-            Zipf(1.1), 2 edge labels, MNI min-support 2000; default k = 2
-  rmat20b   the same graph shape with 100 labels Zipf(1.0), 4 edge labels, min-support 500
-  rmat22    R-MAT scale 22, edge factor 16, 50 labels Zipf(1.2), 4 edge labels,
-            min-support 5000
+Workloads (seeded, synthetic, reference binary-CSR layout).  The R-MAT generator is
+(a,b,c,d) = (.57,.19,.19,.05) with self-loops removed and pairs deduplicated; the support
+threshold of a scaled-down graph is scaled with its vertex count.
+
+  rmat18b   (default) the batched configuration's graph -- 100 vertex labels Zipf(1.0),
+            4 edge labels -- at R-MAT scale 18, edge factor 8; min-support 500 / 4 = 125;
+            k = 2, the deepest level the full-size configuration can mine (level 3 holds
+            ~4e13 rows at scale 20)
+  rmat18    the plain-path configuration's graph (20 labels Zipf(1.1), 2 edge labels) at
+            scale 18, min-support 2000 / 4 = 500, k = 2
+  rmat18d   the learned-predictor configuration's graph (50 labels Zipf(1.2), 4 edge
+            labels, edge factor 16) at scale 18, min-support 5000 / 16 = 312, k = 2
+  rmat14b3  the batched configuration's distributions at scale 14, min-support 8, k = 3
+            (a level below the last, where the learned sizes apply)
   er10k     Erdos-Renyi 10,000 vertices / 40,000 edges, 5 uniform labels, 1 edge label,
-            min-support 200, k = 4
+            min-support 200, k = 4 (the CPU-runnable configuration)
+  rmat20, rmat20b, rmat22   the full-size R-MAT configurations; their graphs are only used
+            by --ingest here (an R-MAT scale-20 mining run once faulted the GPU with an
+            earlier version of the extension kernels, and its cause has not been shown)
 """
 from __future__ import annotations
 
@@ -25,6 +36,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 WORKLOADS = {
+    "rmat18b": dict(kind="rmat", scale=18, edge_factor=8, n_vlabels=100, vlabel_zipf=1.0,
+                    n_elabels=4, min_sup=125, k=2,
+                    of="batched R-MAT scale 20, min-support 500 (k = 2: its deepest level)"),
+    "rmat18": dict(kind="rmat", scale=18, edge_factor=8, n_vlabels=20, vlabel_zipf=1.1,
+                   n_elabels=2, min_sup=500, k=2,
+                   of="plain-path R-MAT scale 20, min-support 2000"),
+    "rmat18d": dict(kind="rmat", scale=18, edge_factor=16, n_vlabels=50, vlabel_zipf=1.2,
+                    n_elabels=4, min_sup=312, k=2,
+                    of="predictor R-MAT scale 22, min-support 5000"),
+    "rmat14b3": dict(kind="rmat", scale=14, edge_factor=8, n_vlabels=100, vlabel_zipf=1.0,
+                     n_elabels=4, min_sup=8, k=3),
     "rmat20": dict(kind="rmat", scale=20, edge_factor=8, n_vlabels=20, vlabel_zipf=1.1,
                    n_elabels=2, min_sup=2000, k=2),
     "rmat20b": dict(kind="rmat", scale=20, edge_factor=8, n_vlabels=100, vlabel_zipf=1.0,
@@ -35,9 +57,10 @@ WORKLOADS = {
                   min_sup=200, k=4),
 }
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per fill launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this bench,
-# scaled by a 1 GiB copy's readings (profiles/k_items_er10k_k4.md); None where not measured
-FILL_TRAFFIC_BYTES = {"er10k": 431_515_558}
+# HBM bytes per launch of the dominant kernel, measured by rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs of this bench's default steps and scaled by a plain copy's readings;
+# profiles/traffic.json holds them per workload (absent: traffic null)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(msg: str) -> None:
@@ -54,40 +77,44 @@ def make_graph(w: dict):
                      n_elabels=w["n_elabels"], seed=1)
 
 
+def physical_cores() -> int:
+    """Physical cores of this host (distinct (core, socket) pairs in lscpu's table)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             check=True).stdout
+        n = len({l for l in out.splitlines() if l and not l.startswith("#")})
+        return n or (os.cpu_count() or 1)
+    except Exception:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(g, w: dict, k: int, target_s: float) -> dict:
-    """The oracle (CPU/OpenMP restatement of the reference path) on a bounded sample."""
+    """The oracle (CPU/OpenMP restatement of the reference's path: extend_omp, support,
+    is_canonical, the mine() recursion) mining the workload in the reference's order, five
+    runs each cut at the first pattern visited after target_s / 5 seconds (the whole run
+    when it ends first); the median rate of the five.  Threads: every physical core, or the
+    OMP_NUM_THREADS share of this host when that is set and smaller."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    if w["kind"] == "er":
+    phys = physical_cores()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(phys, share) if share > 0 else phys
+    rates, secs, cut = [], [], False
+    for _ in range(5):
         t0 = time.perf_counter()
-        _, st = oracle.fsm(g, k, w["min_sup"], nthreads=threads, max_seconds=target_s)
+        _, st = oracle.fsm(g, k, w["min_sup"], nthreads=threads, max_seconds=target_s / 5)
         dt = time.perf_counter() - t0
-        sample = (f"full mining run of the workload (k={k})" if not st["truncated"] else
-                  f"mining in reference order cut after {target_s:.0f}s")
-        return {"value": st["extended"] / dt, "unit": "extended embeddings/s", "cores": threads,
-                "kind": "port", "sample": sample, "seconds": dt}
-    # R-MAT: one extension step of the first frequent single-edge code (reference order),
-    # over a prefix of its rows grown until the step takes ~target/4 seconds
-    seeds = oracle.seed(g)
-    freq = [(e5, rows) for e5, rows in seeds
-            if oracle.support(g, (e5,), [rows], 0, threads) >= w["min_sup"]]
-    e5, rows = freq[0]
-    n = 1 << 12
-    while True:
-        part = rows[:n]
-        t0 = time.perf_counter()
-        kids = oracle.extend(g, (e5,), [part], nthreads=threads)
-        dt = time.perf_counter() - t0
-        emitted = sum(r.shape[0] for _, r in kids)
-        del kids
-        if dt >= target_s / 4 or n >= rows.shape[0]:
-            break
-        n = min(rows.shape[0], n * 4)
-    return {"value": emitted / dt, "unit": "extended embeddings/s", "cores": threads,
-            "kind": "port",
-            "sample": f"extend_omp restatement on the first {part.shape[0]} rows of single-edge "
-                      f"code {tuple(e5)} ({emitted} child rows)", "seconds": dt}
+        rates.append(st["extended"] / dt)
+        secs.append(dt)
+        cut |= bool(st["truncated"])
+    rates.sort()
+    sample = (f"median of 5 full mining runs of the workload (k={k})" if not cut else
+              f"median of 5 runs of mining the workload in the reference's order, each cut "
+              f"after {target_s / 5:.0f} s")
+    return {"value": rates[2], "unit": "extended embeddings/s", "cores": threads,
+            "physical_cores": phys, "kind": "port", "sample": sample,
+            "runs_per_s": rates, "seconds": sum(secs)}
 
 
 def ingest_bench(args) -> int:
@@ -134,7 +161,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     # warmup runs also let the launch-shape tuner try each shape once per bucket
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--workload", default="er10k", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rmat18b", choices=sorted(WORKLOADS))
     ap.add_argument("--k", type=int, default=0)
     ap.add_argument("--min-sup", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0,
@@ -206,20 +233,23 @@ def main() -> int:
     total_s = sum(times)
     value = st["extended"] / total_s
 
-    # dominant extension kernel: per-launch algorithmic bytes / per-launch time (HIP events)
+    # dominant extension kernel: per-launch algorithmic bytes / per-launch time, the time
+    # from HIP events the library records around each launch on its own stream
     fill_ms, count_ms = st["t_fill_ms"], st["t_count_ms"]
     if fill_ms >= count_ms:
-        kname, kms, kbytes, klaunch = "k_items<true> (fill)", fill_ms, st["alg_bytes_fill"], \
-            st["fill_launches"]
+        kname, kms, kbytes, klaunch = "k_items fill (MNI sets / child rows)", fill_ms, \
+            st["alg_bytes_fill"], st["fill_launches"]
     else:
-        kname, kms, kbytes, klaunch = "k_items<false> (count)", count_ms, st["alg_bytes_count"], \
+        kname, kms, kbytes, klaunch = "k_items count", count_ms, st["alg_bytes_count"], \
             st["count_launches"]
     achieved = (kbytes / klaunch) / (kms / klaunch * 1e-3) / 1e9 if kms > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and not args.k and not args.min_sup:
+        rec = json.load(open(TRAFFIC_FILE)).get(args.workload)
+        if rec and rec.get("kernel") == kname:
+            traffic = rec["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": (FILL_TRAFFIC_BYTES.get(args.workload)
-                            if kname.startswith("k_items<true>") and not args.k and not args.min_sup
-                            else None),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kname, "launches_per_step": klaunch / steps,
                 "avg_launch_ms": kms / max(klaunch, 1),
                 "alg_bytes_per_launch": kbytes / max(klaunch, 1)}
@@ -268,7 +298,11 @@ def main() -> int:
         predicted = {"value": qst["extended"] / sum(qt), "ms_per_step": sum(qt) / len(qt) * 1e3,
                      "predicted_steps_per_step": qst["predicted_steps"] / args.steps,
                      "fallbacks_per_step": qst["fallbacks"] / args.steps,
+                     "items_per_step": qst["items"] / args.steps,
                      "train_samples": samples, "train_loss": loss, "train_s": train_s}
+        if k <= 2:
+            predicted["note"] = ("learned sizes apply to levels below the last; k = 2 has none, "
+                                 "so this run is the batched method")
         log(f"predicted: {predicted['ms_per_step']:.2f} ms/step")
 
     cpu = None
@@ -297,6 +331,7 @@ def main() -> int:
                         + f"{w['n_vlabels']} vlabels"
                         + (f" Zipf({w['vlabel_zipf']})" if w["vlabel_zipf"] else " uniform")
                         + f", {w['n_elabels']} elabels, min_sup {ms}, k {k}",
+            "downsized_from": w.get("of"),
             "k": k, "min_support": ms, "mode": "batched", "mni": "gpu",
             "graph": gstats,
         },
