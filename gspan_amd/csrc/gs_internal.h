@@ -106,14 +106,65 @@ struct ItemGuard {
     int* err;              // guard word (bits GS_GUARD_*)
 };
 
-// MNI set descriptor of one child key that survived canonicality.
+// MNI image sets of one child key: one set per child vertex.  Every image of child vertex v
+// carries v's label, so v's set covers only the graph vertices of that label and is
+// indexed by a vertex's rank among them (DevGraph::lidx): a bitmap or bytemap of |V_l|
+// entries, or an open-addressing table of 2 x min(rows, |V_l|) slots when that is
+// smaller.  Sets are 16-byte aligned in the pool (word offsets are multiples of 4).
 struct SetDesc {
-    long long off;  // first u32 word in the set pool
-    int kind;       // GS_SET_BITMAP / GS_SET_HASH / GS_SET_BYTEMAP
-    int size;       // words per vertex set (bitmap, bytemap) or slots (hash, power of two)
-    int nv;         // vertices of the child pattern
-    int pad;
+    long long off;        // first u32 word of the key's sets in the pool
+    int nv;               // vertex sets (vertices of the child pattern); 0 = no sets
+    int kinds;            // kind of vertex set v (GS_SET_*) in bits [2v, 2v + 2)
+    int vo[GS_MAXV + 1];  // vertex set v: words [off + vo[v], off + vo[v + 1]); vo[nv] = all
+    int pad[2];
 };
+static_assert(sizeof(SetDesc) == 64, "SetDesc is four 16-byte loads");
+
+__host__ __device__ inline int set_kind(const SetDesc& d, int v) { return (d.kinds >> (2 * v)) & 3; }
+
+// Kind and words of one vertex set: `rows` inserts at most, over `ent` label-class
+// vertices.  A hash table of >= 2 * min(rows, ent) slots (16 at least, a power of two) when
+// that is smaller than a bitmap, else a bytemap (bytes) or a bitmap; 16-byte padded.
+__host__ __device__ inline void plan_vertex_set(long long rows, long long ent, bool bytes,
+                                                int* kind, long long* words) {
+    const long long bw = ((ent + 31) / 32 + 3) / 4 * 4;
+    long long cap = 16;
+    const long long want = 2 * (rows < ent ? rows : ent);
+    while (cap < want) cap <<= 1;
+    if (cap < bw) {
+        *kind = GS_SET_HASH;
+        *words = cap;
+    } else if (bytes) {
+        *kind = GS_SET_BYTEMAP;
+        *words = (ent + 15) / 16 * 4;
+    } else {
+        *kind = GS_SET_BITMAP;
+        *words = bw;
+    }
+}
+
+// Sets of a child with nv vertices of label ranks lab[0 .. nv) and `rows` rows; lcnt[l] =
+// graph vertices of label rank l.  Returns the key's words (a multiple of 4).
+__host__ __device__ inline long long plan_key_sets(long long rows, int nv, const int* lab,
+                                                   const int* lcnt, bool bytes, SetDesc* d) {
+    long long w = 0;
+    d->nv = nv;
+    d->kinds = 0;
+    for (int v = 0; v <= GS_MAXV; ++v) {
+        if (v < nv) {
+            int kind;
+            long long words;
+            plan_vertex_set(rows, lcnt[lab[v]], bytes, &kind, &words);
+            d->vo[v] = (int)w;
+            d->kinds |= kind << (2 * v);
+            w += words;
+        } else {
+            d->vo[v] = (int)w;
+        }
+    }
+    d->pad[0] = d->pad[1] = 0;
+    return w;
+}
 
 // Batch-wide key decode: key = key_base[p] + slot;
 //   slot <  (r-1)*EL                    : backward, rmpath index i = slot/EL + 1, el = slot%EL
@@ -139,6 +190,11 @@ struct DevGraph {
     // (l = 0 .. VL); null when the table would be too large (then binary search)
     const int* lstart;
     int lstride;
+    // label classes: lidx[v] = rank of v among the vertices of its label (ascending id),
+    // adjlx[i] = lidx[adjl[i].x], lcnt[l] = vertices of label rank l (VL entries)
+    const int* lidx;
+    const int* adjlx;
+    const int* lcnt;
 };
 
 // MLP parameters on the device: w32 holds every parameter in fp32 (W1[32x128], b1, W2[128x128],
@@ -239,7 +295,8 @@ hipStream_t ctx_stream(const gs_ctx* c);
 void ctx_set_error(gs_ctx* c, const char* msg);
 
 // Step tail on the device (steps with many candidate keys), over the active keys a < na
-// of an ordered compaction (keys, counts) and their canonicality flags:
+// of an ordered compaction (keys, counts) and their canonicality flags (lcnt: graph
+// vertices per label rank, for the label-class sets):
 // plan_children: set words (plan_sets) and rows (write_rows) of every canonical child;
 // build_sets: its set descriptor (planned from the scanned words, or key_sets[key] when
 // the sets were planned per key), slot = a in key_slot, first row in cursor;
@@ -247,11 +304,11 @@ void ctx_set_error(gs_ctx* c, const char* msg);
 // totals[1] += canonical keys;
 // gather_frequent: keys, counts and first rows of the frequent children (fidx, *nf).
 hipError_t plan_children(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
-                         const int* acnt, const int* flags, int na, int n, bool bytes,
+                         const int* acnt, const int* flags, int na, const int* lcnt, bool bytes,
                          bool plan_sets, bool write_rows, int* set_words, int* rows,
                          hipStream_t s);
 hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
-                      const int* acnt, const int* flags, int na, int n, bool bytes,
+                      const int* acnt, const int* flags, int na, const int* lcnt, bool bytes,
                       const long long* set_off, const long long* row_off,
                       const SetDesc* key_sets, SetDesc* sets, int* key_slot,
                       unsigned long long* cursor, hipStream_t s);

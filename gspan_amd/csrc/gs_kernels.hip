@@ -34,36 +34,6 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
     return x ^ (x >> 15);
 }
 
-// Inserts graph vertex x into set v of descriptor d.  Bits / slots only go from empty
-// to full, so a stale plain read can only send us to the atomic, never skip a new key.
-// A table sized from exact counts (>= 2x the rows) never fills; one sized from predicted
-// counts can, and then raises *overflow after probing every slot once.
-__device__ __forceinline__ void set_insert(const SetDesc& d, uint32_t* pool, int v, uint32_t x,
-                                           int* overflow = nullptr) {
-    if (d.kind == GS_SET_BYTEMAP) {
-        reinterpret_cast<unsigned char*>(pool + d.off + (long long)v * d.size)[x] = 1;
-    } else if (d.kind == GS_SET_BITMAP) {
-        uint32_t* w = pool + d.off + (long long)v * d.size + (x >> 5);
-        uint32_t bit = 1u << (x & 31);
-        if (!(*w & bit)) atomicOr(w, bit);
-    } else {
-        uint32_t* base = pool + d.off + (long long)v * d.size;
-        uint32_t mask = (uint32_t)d.size - 1;
-        uint32_t key = x + 1;  // 0 marks an empty slot
-        uint32_t h = hash32(x) & mask;
-        for (int probe = 0; probe < d.size; ++probe) {
-            uint32_t cur = __hip_atomic_load(base + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (cur == key) return;
-            if (cur == 0) {
-                uint32_t prev = atomicCAS(base + h, 0u, key);
-                if (prev == 0 || prev == key) return;
-            }
-            h = (h + 1) & mask;
-        }
-        if (overflow) atomicOr(overflow, 1);
-    }
-}
-
 // A row context in registers: eight-lane integer vectors, indexed by compare-and-select
 // (a per-lane index into an array would put the array in scratch memory).
 typedef int int8v __attribute__((ext_vector_type(8)));
@@ -99,6 +69,79 @@ __device__ __forceinline__ int pick(const int (&a)[N], int i) {
     return r;
 }
 
+// A set descriptor in registers (four 16-byte loads); its offsets are picked by
+// compare-and-select, like a row context's arrays.
+struct SetRegs {
+    long long off;
+    int nv, kinds;
+    int8v vo;  // vo[0 .. 7]
+    int vo8;   // vo[8]
+};
+
+__device__ __forceinline__ SetRegs load_set(const SetDesc* p) {
+    static_assert(GS_MAXV == 8, "SetRegs holds 9 offsets");
+    const int4* q = reinterpret_cast<const int4*>(p);
+    const int4 a = q[0], b = q[1], c = q[2], e = q[3];
+    SetRegs r;
+    r.off = (long long)(((unsigned long long)(unsigned)a.y << 32) | (unsigned)a.x);
+    r.nv = a.z;
+    r.kinds = a.w;
+    r.vo = int8v{b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+    r.vo8 = e.x;
+    return r;
+}
+
+__device__ __forceinline__ int vo_at(const SetRegs& d, int i) {
+    return i >= GS_MAXV ? d.vo8 : pick(d.vo, i);
+}
+
+// Descriptor d addresses words [d.off, d.off + d.vo[d.nv]) of a pool of pool_words words,
+// for a child of at least `need` vertices.
+__device__ __forceinline__ bool set_fits(const SetRegs& d, int need, long long pool_words) {
+    const int all = vo_at(d, d.nv);
+    return d.off >= 0 && d.nv >= need && d.nv <= GS_MAXV && all >= 0 && d.off + all <= pool_words;
+}
+
+// Inserts the vertex of label-class rank x into vertex set v of descriptor d (x indexes
+// the vertices of v's label).  Bits / slots only go from empty to full, so a stale plain
+// read can only send us to the atomic, never skip a new key.  A table sized from exact
+// counts (>= 2x the rows) never fills; one sized from predicted counts can, and then raises
+// *overflow after probing every slot once.  An x past a bitmap or bytemap sets the guard
+// bit GS_GUARD_POOL in *gerr and is skipped.
+__device__ __forceinline__ void set_insert(const SetRegs& d, uint32_t* pool, int v, uint32_t x,
+                                           int* gerr, int* overflow = nullptr) {
+    const int kind = (d.kinds >> (2 * v)) & 3;
+    const int v0 = vo_at(d, v);
+    uint32_t* base = pool + d.off + v0;
+    const uint32_t words = (uint32_t)(vo_at(d, v + 1) - v0);
+    if (kind == GS_SET_BYTEMAP) {
+        if (x < 4u * words) reinterpret_cast<unsigned char*>(base)[x] = 1;
+        else atomicOr(gerr, GS_GUARD_POOL);
+    } else if (kind == GS_SET_BITMAP) {
+        if ((x >> 5) < words) {
+            uint32_t* w = base + (x >> 5);
+            const uint32_t bit = 1u << (x & 31);
+            if (!(*w & bit)) atomicOr(w, bit);
+        } else {
+            atomicOr(gerr, GS_GUARD_POOL);
+        }
+    } else {
+        const uint32_t mask = words - 1;  // a power of two
+        const uint32_t key = x + 1;       // 0 marks an empty slot
+        uint32_t h = hash32(x) & mask;
+        for (uint32_t probe = 0; probe < words; ++probe) {
+            uint32_t cur = __hip_atomic_load(base + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == key) return;
+            if (cur == 0) {
+                uint32_t prev = atomicCAS(base + h, 0u, key);
+                if (prev == 0 || prev == key) return;
+            }
+            h = (h + 1) & mask;
+        }
+        if (overflow) atomicOr(overflow, 1);
+    }
+}
+
 // Element i of a pattern table: when the pattern is the same for the whole wave (UNI)
 // its tables are wave-uniform, read with scalar loads and selected per lane; otherwise
 // one vector load.
@@ -128,7 +171,7 @@ struct ItemOut {
     bool valid;
     int key;
     int eid;
-    int newv;  // image of the new vertex (forward) or -1 (backward)
+    int newv;  // label-class rank of the new vertex's image (forward; NEWX) or -1
     int nv;    // vertices of the parent pattern
     long long pbase;  // parent row in its level = pbase + (row in the batch)
 };
@@ -141,8 +184,9 @@ __device__ __forceinline__ bool in_range(long long x, long long n) {
 // q+1 (gspan.cu:428-445); the rest walk the neighbours of u_n (first forward,
 // gspan.cu:447-462) and of every rightmost-path source (other forward, gspan.cu:464-483).
 // c is the row's context in registers.  Vertex ids and CSR entries are checked against
-// the graph before they are used as indices (GS_GUARD_GRAPH).
-template <bool UNI>
+// the graph before they are used as indices (GS_GUARD_GRAPH).  NEWX: a forward child also
+// returns the label-class rank of its new vertex (for its MNI set).
+template <bool UNI, bool NEWX>
 __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowRegs& c,
                                              int q, KeySpace ks, int* gerr) {
     ItemOut o{false, 0, 0, -1, P.nv, P.row_begin - P.brow_begin};
@@ -201,7 +245,7 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
     if (!(e_el < el || (e_el == el && pget<UNI>(P.seg_tl, j) <= tl))) return o;
     o.valid = true;
     o.eid = e;
-    o.newv = to;
+    o.newv = NEWX ? g.adjlx[el_idx] : 0;
     o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
     return o;
 }
@@ -284,13 +328,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
                     if (in_range(at, gd.rows_cap)) rows_out[at] = make_int2(-1, (int)i);
                     else atomicOr(gd.err, GS_GUARD_ROW);
                 }
-                const SetDesc d = sets[slot];
-                if (d.off < 0 || d.off + 2LL * d.size > gd.pool_words) {
+                const SetRegs d = load_set(sets + slot);
+                if (!set_fits(d, 2, gd.pool_words)) {
                     atomicOr(gd.err, GS_GUARD_POOL);
                     continue;
                 }
-                set_insert(d, pool, 0, (uint32_t)g.src[i]);
-                set_insert(d, pool, 1, g.adj[i].x);
+                set_insert(d, pool, 0, (uint32_t)g.lidx[g.src[i]], gd.err);
+                set_insert(d, pool, 1, (uint32_t)g.lidx[g.adj[i].x], gd.err);
             }
         }
         __syncthreads();
@@ -786,9 +830,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             // the pattern's tables are read once per wave with scalar loads
             const int p0 = __builtin_amdgcn_readfirstlane(p);
             if (__all(p == p0 || p < 0)) {
-                if (it < tn) o = eval_item<true>(g, pats[p0], c, f, ks, gd.err);
+                if (it < tn) o = eval_item<true, FILL>(g, pats[p0], c, f, ks, gd.err);
             } else if (it < tn) {
-                o = eval_item<false>(g, pats[p], c, f, ks, gd.err);
+                o = eval_item<false, FILL>(g, pats[p], c, f, ks, gd.err);
             }
             if (!o.valid) continue;
             if (!in_range(o.key, gd.keys)) {
@@ -803,19 +847,22 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 continue;
             }
             if (FILL) {
-                const SetDesc d = sets[agg];
+                const SetRegs d = load_set(sets + agg);
                 hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
-                const bool lead = !dedup || lds_first<LDS_SLOTS>(s_pair, agg * TILE + row);
+                // the pair is named by the key's slot in this tile's table (one slot per
+                // live key of the tile) and the row: below 2 * TILE * TILE, so distinct
+                // pairs never share a name (agg * TILE would wrap past 2^31 / TILE keys)
+                const bool lead =
+                    !dedup || lds_first<LDS_SLOTS>(s_pair, hs[q] * TILE + row);
                 if (ROWS) {
                     eid[q] = o.eid;
                     prow[q] = (int)(o.pbase + b0 + rlo + row);
                 }
                 // the child's sets: one per child vertex (parent's nv, plus the new vertex
-                // of a forward child)
+                // of a forward child), indexed by label-class rank
                 const int nv = o.nv;
-                if (d.off < 0 || d.nv < nv + (o.newv >= 0 ? 1 : 0) ||
-                    d.off + (long long)d.nv * d.size > gd.pool_words) {
+                if (!set_fits(d, nv + (o.newv >= 0 ? 1 : 0), gd.pool_words)) {
                     atomicOr(gd.err, GS_GUARD_POOL);
                     continue;
                 }
@@ -824,14 +871,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     for (int v = 0; v < GS_MAXV; ++v) {
                         if (v >= nv) continue;
                         const int x = c.phi[v];
-                        if (in_range(x, g.n)) set_insert(d, pool, v, (uint32_t)x, overflow);
+                        if (in_range(x, g.n))
+                            set_insert(d, pool, v, (uint32_t)g.lidx[x], gd.err, overflow);
                         else atomicOr(gd.err, GS_GUARD_GRAPH);
                     }
                 }
-                if (o.newv >= 0) {
-                    if (in_range(o.newv, g.n)) set_insert(d, pool, nv, (uint32_t)o.newv, overflow);
-                    else atomicOr(gd.err, GS_GUARD_GRAPH);
-                }
+                if (o.newv >= 0) set_insert(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
             } else {
                 hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
@@ -1010,33 +1055,41 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
         p[i] = v;
 }
 
-// distinct images per (set, vertex): popcount of a bitmap / bytemap, or occupied hash
-// slots.  One wave per (set, vertex) pair, pairs taken grid-stride, so any number of sets
-// fits one launch (a launch holds fewer than 2^32 work-items).
-__global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets, long long nitems,
+// distinct images per (set, vertex): popcount of a bitmap / bytemap (bytes are 0 or 1), or
+// occupied hash slots.  One wave per set, sets taken grid-stride (a launch holds fewer than
+// 2^32 work-items); each vertex set is read with 16-byte loads.  counts[8 s + v] for all
+// v < 8 (0 past the set's vertices).
+__global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets, long long nsets,
                                                          const uint32_t* pool,
                                                          long long pool_words, int* counts,
                                                          int* gerr) {
     constexpr int WPB = GS_BLOCK / 64;  // waves per block
     const int lane = lane_id();
     const long long waves = (long long)gridDim.x * WPB;
-    for (long long it = (long long)blockIdx.x * WPB + threadIdx.x / 64; it < nitems; it += waves) {
-        const long long si = it / GS_MAXV;
-        const int v = (int)(it % GS_MAXV);
+    for (long long si = (long long)blockIdx.x * WPB + threadIdx.x / 64; si < nsets; si += waves) {
         const SetDesc d = sets[si];
-        const bool ok = d.off >= 0 && d.off + (long long)d.nv * d.size <= pool_words;
-        if (!ok && v == 0 && lane == 0) atomicOr(gerr, GS_GUARD_POOL);
-        int c = 0;
-        if (ok && v < d.nv) {
-            const uint32_t* base = pool + d.off + (long long)v * d.size;
-            if (d.kind != GS_SET_HASH) {  // bitmap bits, or bytemap bytes that are 0 or 1
-                for (long long i = lane; i < d.size; i += 64) c += __popc(base[i]);
-            } else {
-                for (long long i = lane; i < d.size; i += 64) c += base[i] != 0;
+        const bool ok = d.nv == 0 || set_fits(load_set(sets + si), 0, pool_words);
+        if (!ok && lane == 0) atomicOr(gerr, GS_GUARD_POOL);
+        for (int v = 0; v < GS_MAXV; ++v) {
+            int c = 0;
+            if (ok && v < d.nv) {
+                const uint4* base = reinterpret_cast<const uint4*>(pool + d.off + d.vo[v]);
+                const int n4 = (d.vo[v + 1] - d.vo[v]) / 4;
+                if (set_kind(d, v) != GS_SET_HASH) {
+                    for (int i = lane; i < n4; i += 64) {
+                        const uint4 w = base[i];
+                        c += __popc(w.x) + __popc(w.y) + __popc(w.z) + __popc(w.w);
+                    }
+                } else {
+                    for (int i = lane; i < n4; i += 64) {
+                        const uint4 w = base[i];
+                        c += (w.x != 0) + (w.y != 0) + (w.z != 0) + (w.w != 0);
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
             }
+            if (lane == 0) counts[si * GS_MAXV + v] = c;
         }
-        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-        if (lane == 0) counts[it] = c;
     }
 }
 
@@ -1067,13 +1120,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_support_rows(DevGraph g, const Pat
         if (l == 0) phi[0] = g.src[rr.y];
         idx = rr.x;
     }
-    const SetDesc d = *set;
-    if (d.off < 0 || d.off + (long long)P.nv * d.size > pool_words) {
+    const SetRegs d = load_set(set);
+    if (!set_fits(d, P.nv, pool_words)) {
         atomicOr(gerr, GS_GUARD_POOL);
         return;
     }
     for (int v = 0; v < P.nv; ++v) {
-        if (in_range(phi[v], g.n)) set_insert(d, pool, v, (uint32_t)phi[v]);
+        if (in_range(phi[v], g.n)) set_insert(d, pool, v, (uint32_t)g.lidx[phi[v]], gerr);
         else atomicOr(gerr, GS_GUARD_GRAPH);
     }
 }
@@ -1465,15 +1518,17 @@ __device__ __forceinline__ int owner_of(const PatDev* __restrict__ pats, int npa
     return lo;
 }
 
-// Child of active key `key`: its vertex count and whether (GS_MNI_OMP_LITERAL) its vertex 0
-// takes part in the support -- only when an edge after the first leaves vertex 0, as
-// support_omp inserts an edge's source only below layer 0 (gspan.cu:307, 360).
+// Child of active key `key`: its vertex count, the label rank of each of its vertices
+// (lab, GS_MAXV entries) and whether (GS_MNI_OMP_LITERAL) its vertex 0 takes part in the
+// support -- only when an edge after the first leaves vertex 0, as support_omp inserts an
+// edge's source only below layer 0 (gspan.cu:307, 360).
 __device__ __forceinline__ void child_shape(const PatDev& P, int slot, KeySpace ks, int* nv,
-                                            bool* v0) {
+                                            bool* v0, int* lab) {
     const int nb = (P.r - 1) * ks.EL;
     bool from0 = false;
     for (int j = 1; j < GS_MAXE; ++j)
         if (j < P.m && P.cfrom[j] == 0) from0 = true;
+    for (int v = 0; v < GS_MAXV; ++v) lab[v] = v < P.nv ? P.vlab[v] : 0;
     int from;
     if (slot < nb) {
         from = P.v_n;  // backward: from the rightmost vertex
@@ -1482,6 +1537,7 @@ __device__ __forceinline__ void child_shape(const PatDev& P, int slot, KeySpace 
         const int seg = (slot - nb) / (ks.EL * ks.VL);
         from = seg == 0 ? P.v_n : P.cfrom[P.rm[seg - 1]];
         *nv = P.nv + 1;
+        if (P.nv < GS_MAXV) lab[P.nv] = (slot - nb) % ks.VL;  // the new vertex's label
     }
     *v0 = from0 || from == 0;
 }
@@ -1490,49 +1546,26 @@ __device__ __forceinline__ long long child_rows(int c) {  // a negative count: >
     return c < 0 ? (long long)INT_MAX + 1 : (long long)c;
 }
 
-// Set of a child with `rows` rows (the host's plan_set): an open-addressing table of at
-// least 2 * rows slots (16 at least, at most 2n) when that is smaller than a bitmap, else a
-// bytemap or a bitmap; words = vertices x words per vertex set, padded to 4 words.
-__device__ __forceinline__ void plan_one(long long rows, int nv, int n, int bytes, int* kind,
-                                         int* size, long long* words) {
-    const long long bw = (((long long)n + 31) / 32 + 3) / 4 * 4;
-    long long cap = 16;
-    const long long want = min(2 * rows, 2LL * n);
-    while (cap < want) cap <<= 1;
-    if (cap < bw) {
-        *kind = GS_SET_HASH;
-        *size = (int)cap;
-    } else if (bytes) {
-        *kind = GS_SET_BYTEMAP;
-        *size = (int)(((long long)n + 15) / 16 * 4);
-    } else {
-        *kind = GS_SET_BITMAP;
-        *size = (int)bw;
-    }
-    *words = ((long long)nv * *size + 3) / 4 * 4;
-}
-
 // Per active key a < na: set words and rows of its child (0 unless canonical; no set words
 // when the sets were planned for every key beforehand, rows only when rows are written).
 __global__ void k_plan_children(const PatDev* __restrict__ pats, int npats, KeySpace ks,
                                 const int* __restrict__ akeys, const int* __restrict__ acnt,
-                                const int* __restrict__ flags, int na, int n, int bytes,
-                                int plan_sets, int write_rows, int* set_words, int* rows) {
+                                const int* __restrict__ flags, int na, const int* lcnt,
+                                int bytes, int plan_sets, int write_rows, int* set_words,
+                                int* rows) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
         int w = 0, r = 0;
         if (flags[a]) {
             const int key = akeys[a];
             const PatDev& P = pats[owner_of(pats, npats, key)];
-            int nv;
+            int nv, lab[GS_MAXV];
             bool v0;
-            child_shape(P, key - P.key_base, ks, &nv, &v0);
+            child_shape(P, key - P.key_base, ks, &nv, &v0, lab);
             const long long cr = child_rows(acnt[a]);
             if (plan_sets) {
-                int kind, size;
-                long long words;
-                plan_one(cr, nv, n, bytes, &kind, &size, &words);
-                w = (int)words;
+                SetDesc d;
+                w = (int)plan_key_sets(cr, nv, lab, lcnt, bytes != 0, &d);
             }
             if (write_rows) r = (int)min(cr, (long long)INT_MAX);
         }
@@ -1545,26 +1578,24 @@ __global__ void k_plan_children(const PatDev* __restrict__ pats, int npats, KeyS
 // are not canonical get an empty descriptor (nv = 0) and no slot.
 __global__ void k_build_sets(const PatDev* __restrict__ pats, int npats, KeySpace ks,
                              const int* __restrict__ akeys, const int* __restrict__ acnt,
-                             const int* __restrict__ flags, int na, int n, int bytes,
-                             const long long* __restrict__ set_off,
+                             const int* __restrict__ flags, int na, const int* lcnt,
+                             int bytes, const long long* __restrict__ set_off,
                              const long long* __restrict__ row_off,
                              const SetDesc* __restrict__ key_sets, SetDesc* sets, int* key_slot,
                              unsigned long long* cursor) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
-        SetDesc d{0, GS_SET_HASH, 1, 0, 0};
+        SetDesc d{};  // no sets (nv = 0)
         const int key = akeys[a];
         if (flags[a]) {
             if (key_sets) {  // planned per key beforehand (single pass)
                 d = key_sets[key];
             } else {
                 const PatDev& P = pats[owner_of(pats, npats, key)];
-                int nv;
+                int nv, lab[GS_MAXV];
                 bool v0;
-                child_shape(P, key - P.key_base, ks, &nv, &v0);
-                long long words;
-                plan_one(child_rows(acnt[a]), nv, n, bytes, &d.kind, &d.size, &words);
-                d.nv = nv;
+                child_shape(P, key - P.key_base, ks, &nv, &v0, lab);
+                plan_key_sets(child_rows(acnt[a]), nv, lab, lcnt, bytes != 0, &d);
                 d.off = set_off[a];
             }
             if (key_slot) key_slot[key] = (int)a;
@@ -1592,9 +1623,9 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
             ++canon;
             const int key = akeys[a];
             const PatDev& P = pats[owner_of(pats, npats, key)];
-            int nv;
+            int nv, lab[GS_MAXV];
             bool v0;
-            child_shape(P, key - P.key_base, ks, &nv, &v0);
+            child_shape(P, key - P.key_base, ks, &nv, &v0, lab);
             if (mni_mode != GS_MNI_OMP_LITERAL) v0 = true;
             long long s = INT_MAX;
             for (int v = 0; v < nv; ++v) {
@@ -1850,31 +1881,30 @@ hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t 
 hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
                       long long pool_words, int* counts, int* guard_err, hipStream_t s) {
     if (nsets <= 0) return hipSuccess;
-    const long long nitems = (long long)nsets * GS_MAXV;
-    hipLaunchKernelGGL(k_set_counts, dim3(grid_for(nitems, GS_BLOCK / 64, 16384)), dim3(GS_BLOCK),
-                       0, s, sets, nitems, pool, pool_words, counts, guard_err);
+    hipLaunchKernelGGL(k_set_counts, dim3(grid_for(nsets, GS_BLOCK / 64, 16384)), dim3(GS_BLOCK),
+                       0, s, sets, (long long)nsets, pool, pool_words, counts, guard_err);
     return hipGetLastError();
 }
 
 hipError_t plan_children(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
-                         const int* acnt, const int* flags, int na, int n, bool bytes,
+                         const int* acnt, const int* flags, int na, const int* lcnt, bool bytes,
                          bool plan_sets, bool write_rows, int* set_words, int* rows,
                          hipStream_t s) {
     if (na <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_plan_children, dim3(grid_for(na, 256, 16384)), dim3(256), 0, s, pats,
-                       npats, ks, akeys, acnt, flags, na, n, bytes ? 1 : 0, plan_sets ? 1 : 0,
+                       npats, ks, akeys, acnt, flags, na, lcnt, bytes ? 1 : 0, plan_sets ? 1 : 0,
                        write_rows ? 1 : 0, set_words, rows);
     return hipGetLastError();
 }
 
 hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
-                      const int* acnt, const int* flags, int na, int n, bool bytes,
+                      const int* acnt, const int* flags, int na, const int* lcnt, bool bytes,
                       const long long* set_off, const long long* row_off,
                       const SetDesc* key_sets, SetDesc* sets, int* key_slot,
                       unsigned long long* cursor, hipStream_t s) {
     if (na <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_build_sets, dim3(grid_for(na, 256, 16384)), dim3(256), 0, s, pats, npats,
-                       ks, akeys, acnt, flags, na, n, bytes ? 1 : 0, set_off, row_off, key_sets,
+                       ks, akeys, acnt, flags, na, lcnt, bytes ? 1 : 0, set_off, row_off, key_sets,
                        sets, key_slot, cursor);
     return hipGetLastError();
 }

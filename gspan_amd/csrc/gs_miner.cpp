@@ -290,6 +290,10 @@ struct gs_ctx {
     HostStage stage;
     std::vector<PendingD2H> pending_d2h;
     DBuf d_offsets, d_src, d_adj, d_vlab, d_adjl, d_eidl, d_lstart;
+    // label classes: rank of each vertex within its label, the same per adjl entry, and the
+    // vertices of each label rank (lcnt on the host too: set planning)
+    DBuf d_lidx, d_adjlx, d_lcnt;
+    std::vector<int> lcnt;
     DevGraph dg{};
     // [0] guard word, [1] n_active, [2] err, [3] overflow, [4..5] int64 total,
     // [6..7] items without the label filter, [8..11] two 64-bit step totals
@@ -598,39 +602,28 @@ int support_of(const HPat& h, const int* counts, int mni_mode) {
     return (int)sup;
 }
 
-// Set memory per key: a bitmap over all graph vertices, or a power-of-two open-addressing
-// table of at least twice the key's rows when that is smaller.
-// u32 words of one vertex set over all graph vertices: a bit or a byte per vertex, padded
-// to 16 bytes.
-long long bitmap_words(const gs_ctx* c) { return (((long long)c->n + 31) / 32 + 3) / 4 * 4; }
-long long bytemap_words(const gs_ctx* c) { return ((long long)c->n + 15) / 16 * 4; }
-
 // Byte-per-vertex sets unless the caller asked for bits.
 bool want_bytes(const gs_ctx* c) { return c->opts.set_kind != GS_SETS_BITS; }
 
-// Sets of a child with `rows` rows: a hash table of >= 2*rows slots when that is smaller
-// than a bitmap, else a bytemap (bytes) or a bitmap.
-void plan_set(const gs_ctx* c, long long rows, int nv, bool bytes, SetDesc& d,
+// Sets of a child with `rows` rows and nv vertices of label ranks lab[]: one set per
+// vertex over the graph vertices of its label (plan_key_sets), placed at pool_words.
+void plan_set(const gs_ctx* c, long long rows, int nv, const int* lab, bool bytes, SetDesc& d,
               long long& pool_words) {
-    const long long bw = bitmap_words(c);
-    long long cap = 16;
-    const long long want = std::min<long long>(2 * rows, 2LL * c->n);
-    while (cap < want) cap <<= 1;
-    d.nv = nv;
-    if (cap < bw) {
-        d.kind = GS_SET_HASH;
-        d.size = (int)cap;
-    } else if (bytes) {
-        d.kind = GS_SET_BYTEMAP;
-        d.size = (int)bytemap_words(c);
-    } else {
-        d.kind = GS_SET_BITMAP;
-        d.size = (int)bw;
-    }
+    const long long w = plan_key_sets(rows, nv, lab, c->lcnt.data(), bytes, &d);
     d.off = pool_words;
-    pool_words += (long long)nv * d.size;
-    pool_words = (pool_words + 3) / 4 * 4;
-    d.pad = 0;
+    pool_words += w;
+}
+
+// Rows large enough that every vertex set is a bitmap or bytemap over its label class (sets
+// planned before the counts are known).
+constexpr long long kAllRows = 1LL << 40;
+
+// Words of a bitmap (bytes = false) or bytemap vertex set over label rank l.
+long long class_words(const gs_ctx* c, int l, bool bytes) {
+    int kind;
+    long long w;
+    plan_vertex_set(kAllRows, c->lcnt[l], bytes, &kind, &w);
+    return w;
 }
 
 // Label statistics for the predictor's features: vertex-label and edge-label frequencies,
@@ -949,29 +942,33 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                        (1.0 + (pd[i].r - 1) + (double)(pd[i].r + 1) * c->max_deg);
     const bool saturate = items_bound > (double)INT_MAX;
 
-    // Single pass at the last level: when a bitmap for every potential child key fits in a
-    // quarter of the free budget, the fill pass counts children and fills the MNI sets in
-    // one go (no count pass); canonicality is then checked on the keys that occurred.
+    // Single pass at the last level: when label-class sets for every potential child key fit
+    // in a quarter of the free budget, the fill pass counts children and fills the MNI sets
+    // in one go (no count pass); canonicality is then checked on the keys that occurred.
     std::vector<SetDesc> all_sets;
     long long all_words = 0;
-    bool single = false;
-    int single_kind = GS_SET_BITMAP;
-    long long set_words = bitmap_words(c);
+    bool single = false, single_bytes = false;
     if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
-        long long sets_per_batch = 0;  // vertex sets over every potential child key
-        for (int i = 0; i < B; ++i) {
-            const long long nb = (long long)(pd[i].r - 1) * c->EL;
-            const long long nf = (long long)(pd[i].r + 1) * c->EL * c->VL;
-            sets_per_batch += nb * pd[i].nv + nf * (pd[i].nv + 1);
-        }
+        // words of every potential child key's sets: per pattern, its backward keys hold the
+        // parent's vertex sets, each forward (segment, edge label) holds VL keys with the
+        // parent's sets plus one set over each label class
+        auto total_words = [&](bool by) {
+            long long tw = 0, all_l = 0;
+            for (int l = 0; l < c->VL; ++l) all_l += class_words(c, l, by);
+            for (int i = 0; i < B; ++i) {
+                long long pw = 0;
+                for (int v = 0; v < pd[i].nv; ++v) pw += class_words(c, pd[i].vlab[v], by);
+                const long long nb = (long long)(pd[i].r - 1) * c->EL;
+                const long long nf = (long long)(pd[i].r + 1) * c->EL;
+                tw += nb * pw + nf * ((long long)c->VL * pw + all_l);
+            }
+            return tw;
+        };
         const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
-        if (want_bytes(c) &&
-            (size_t)(sets_per_batch * bytemap_words(c)) * 4 <= avail / 4) {
-            single = true;
-            single_kind = GS_SET_BYTEMAP;
-            set_words = bytemap_words(c);
+        if (want_bytes(c) && (size_t)total_words(true) * 4 <= avail / 4) {
+            single = single_bytes = true;
         } else if (c->opts.set_kind != GS_SETS_BYTES) {
-            single = (size_t)(sets_per_batch * bitmap_words(c)) * 4 <= avail / 4;
+            single = (size_t)total_words(false) * 4 <= avail / 4;
         }
     }
     DBuf d_all_sets, d_all_pool;
@@ -980,14 +977,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         for (int i = 0; i < B; ++i) {
             const int K_p = (int)key_space(pd[i].r, c->EL, c->VL);
             const int nb = (pd[i].r - 1) * c->EL;
+            int lab[GS_MAXV];
+            for (int v = 0; v < GS_MAXV; ++v) lab[v] = pd[i].vlab[v];
             for (int sl = 0; sl < K_p; ++sl) {
                 SetDesc& d = all_sets[pd[i].key_base + sl];
-                d.kind = single_kind;
-                d.size = (int)set_words;
-                d.nv = pd[i].nv + (sl < nb ? 0 : 1);
-                d.off = all_words;
-                d.pad = 0;
-                all_words += (long long)d.nv * set_words;
+                int nv = pd[i].nv;
+                if (sl >= nb) lab[nv++] = (sl - nb) % c->VL;  // the forward child's new vertex
+                plan_set(c, kAllRows, nv, lab, single_bytes, d, all_words);
             }
         }
         d_all_sets = DBuf(P, sizeof(SetDesc) * K);
@@ -1069,7 +1065,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         for (long long key = 0; key < K; ++key) {
             while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
             const int slot = (int)(key - pd[p].key_base);
-            const int nv_c = pd[p].nv + (slot < (pd[p].r - 1) * c->EL ? 0 : 1);
+            const int nb_p = (pd[p].r - 1) * c->EL;
+            const int nv_c = pd[p].nv + (slot < nb_p ? 0 : 1);
+            int lab_c[GS_MAXV];
+            for (int v = 0; v < GS_MAXV; ++v) lab_c[v] = pd[p].vlab[v];
+            if (slot >= nb_p) lab_c[pd[p].nv] = (slot - nb_p) % c->VL;
             long long cap = 0;
             if (poss[key]) cap = predicted_capacity(y[key]);
             row_off[key] = (unsigned long long)std::min<long long>(rows_total, INT_MAX);
@@ -1077,9 +1077,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             cap_end[key] = (int)std::min<long long>(rows_total, INT_MAX);
             SetDesc& d = psets[key];
             if (cap > 0) {
-                plan_set(c, cap, nv_c, want_bytes(c), d, pool_words);
+                plan_set(c, cap, nv_c, lab_c, want_bytes(c), d, pool_words);
             } else {
-                d = SetDesc{0, GS_SET_HASH, 1, 0, 0};
+                d = SetDesc{};  // no sets
             }
         }
         const size_t need = (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
@@ -1286,7 +1286,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         long long pool_words = 0, rows_total = 0;
         auto plan = [&](bool by) {
             HIPCHECK(gsk::plan_children(d_pats.as<PatDev>(), B, ks, d_active.as<int>(),
-                                        d_acnt.as<int>(), d_flags.as<int>(), A, c->n, by,
+                                        d_acnt.as<int>(), d_flags.as<int>(), A,
+                                        c->d_lcnt.as<int>(), by,
                                         plan_sets, write_rows, d_words.as<int>(),
                                         d_rows.as<int>(), c->stream));
             HIPCHECK(gsk::scan_exclusive(d_words.as<int>(), d_soff.as<long long>(), A,
@@ -1331,7 +1332,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
         if (plan_sets) HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
         HIPCHECK(gsk::build_sets(d_pats.as<PatDev>(), B, ks, d_active.as<int>(),
-                                 d_acnt.as<int>(), d_flags.as<int>(), A, c->n, bytes,
+                                 d_acnt.as<int>(), d_flags.as<int>(), A, c->d_lcnt.as<int>(),
+                                 bytes,
                                  d_soff.as<long long>(), write_rows ? d_roff.as<long long>() : nullptr,
                                  single ? d_all_sets.as<SetDesc>() : nullptr, d_sets.as<SetDesc>(),
                                  d_slot.as<int>(), d_cursor.as<unsigned long long>(), c->stream));
@@ -1477,8 +1479,13 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         for (int li = 0; li < NL; ++li) {
             SetDesc d;
             const ChildInfo& ch = out.children[live_child[li]];
-            if (single) d = all_sets[live_keys[li]];
-            else plan_set(c, ch.count, ch.pat.nv(), bytes, d, pool_words);
+            if (single) {
+                d = all_sets[live_keys[li]];
+            } else {
+                int lab[GS_MAXV];
+                ch.pat.labels(lab);
+                plan_set(c, ch.count, ch.pat.nv(), lab, bytes, d, pool_words);
+            }
             sets.push_back(d);
         }
     };
@@ -1666,7 +1673,8 @@ void mine_seeds(gs_ctx* c) {
         h.row_begin = rows_total;
         h.row_count = acnt[a];
         SetDesc d;
-        plan_set(c, acnt[a], 2, want_bytes(c), d, pool_words);
+        const int lab[2] = {h.fl0, h.tl[0]};
+        plan_set(c, acnt[a], 2, lab, want_bytes(c), d, pool_words);
         sets.push_back(d);
         live.push_back(key);
         cursor.push_back((int)rows_total);
@@ -1677,8 +1685,10 @@ void mine_seeds(gs_ctx* c) {
     if (want_bytes(c) && c->opts.set_kind == GS_SETS_AUTO &&
         c->pool.in_use() + (size_t)pool_words * 4 + (size_t)rows_total * 8 > c->budget) {
         pool_words = 0;  // bytemaps do not fit: plan bitmaps
-        for (size_t i = 0; i < sets.size(); ++i)
-            plan_set(c, seeds[i].row_count, 2, false, sets[i], pool_words);
+        for (size_t i = 0; i < sets.size(); ++i) {
+            const int lab[2] = {seeds[i].fl0, seeds[i].tl[0]};
+            plan_set(c, seeds[i].row_count, 2, lab, false, sets[i], pool_words);
+        }
     }
     const int NL = (int)live.size();
     c->st.candidates += NL;
@@ -1936,6 +1946,11 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
             });
             for (int i = o0; i < o1; ++i) adjl[i] = adj[eidl[i]];
         }
+        // label classes: each vertex's rank among the vertices of its label (ascending id)
+        x->lcnt.assign(x->VL, 0);
+        std::vector<int> lidx(n), adjlx(nnz);
+        for (int v = 0; v < n; ++v) lidx[v] = x->lcnt[vr[v]]++;
+        for (int i = 0; i < nnz; ++i) adjlx[i] = lidx[adjl[i].x];
         // label start table, when it is at most 1 GiB
         const long long lstride = (long long)x->VL + 1;
         std::vector<int> lstart;
@@ -1960,6 +1975,9 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->d_eidl = DBuf(P, sizeof(int) * std::max(nnz, 1));
         if (!lstart.empty()) x->d_lstart = DBuf(P, sizeof(int) * lstart.size());
         x->d_vlab = DBuf(P, sizeof(int) * std::max(n, 1));
+        x->d_lidx = DBuf(P, sizeof(int) * std::max(n, 1));
+        x->d_adjlx = DBuf(P, sizeof(int) * std::max(nnz, 1));
+        x->d_lcnt = DBuf(P, sizeof(int) * x->VL);
         x->d_misc = DBuf(P, 64);
         HIPCHECK(gsk::fill_i32(x->d_misc.as<int>(), 0, 16, x->stream));
         x->d_tiles = DBuf(P, sizeof(int) * 2 * GS_TILE_QUEUE);
@@ -1970,6 +1988,9 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         h2d(x, x->d_eidl.as<int>(), eidl.data(), nnz);
         if (!lstart.empty()) h2d(x, x->d_lstart.as<int>(), lstart.data(), lstart.size());
         h2d(x, x->d_vlab.as<int>(), vr.data(), n);
+        h2d(x, x->d_lidx.as<int>(), lidx.data(), n);
+        h2d(x, x->d_adjlx.as<int>(), adjlx.data(), nnz);
+        h2d(x, x->d_lcnt.as<int>(), x->lcnt.data(), x->VL);
         init_label_stats(x, vr, src, adj);
         mlp_init(x);
         sync(x);
@@ -1983,6 +2004,9 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->dg.lstart = lstart.empty() ? nullptr : x->d_lstart.as<int>();
         x->dg.lstride = (int)lstride;
         x->dg.vlab = x->d_vlab.as<int>();
+        x->dg.lidx = x->d_lidx.as<int>();
+        x->dg.adjlx = x->d_adjlx.as<int>();
+        x->dg.lcnt = x->d_lcnt.as<int>();
     });
     if (rc != GS_OK) {
         std::fprintf(stderr, "gs_create: %s\n", c->err.c_str());
@@ -2044,7 +2068,9 @@ int gs_support(gs_ctx* c, const int* code5, int m, const int* const* layers,
         h2d(c, d_pat.as<PatDev>(), &d, 1);
         long long words = 0;
         SetDesc sd;
-        plan_set(c, counts[m - 1], h.nv(), want_bytes(c), sd, words);
+        int lab[GS_MAXV];
+        h.labels(lab);
+        plan_set(c, counts[m - 1], h.nv(), lab, want_bytes(c), sd, words);
         DBuf d_set(&c->pool, sizeof(SetDesc)), d_pool(&c->pool, 4 * std::max(words, 4LL));
         DBuf d_counts(&c->pool, sizeof(int) * GS_MAXV);
         h2d(c, d_set.as<SetDesc>(), &sd, 1);
@@ -2388,6 +2414,9 @@ int gs_destroy(gs_ctx* c) {
     c->d_adjl.reset();
     c->d_eidl.reset();
     c->d_lstart.reset();
+    c->d_lidx.reset();
+    c->d_adjlx.reset();
+    c->d_lcnt.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();
