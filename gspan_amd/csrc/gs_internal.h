@@ -67,6 +67,7 @@ struct PatDev {
     long long row_begin;   // first row in the parent level array
     long long row_count;   // rows of this pattern
     long long brow_begin;  // first row in the batch row space
+    long long set_base;    // first pool word of its children's sets (sets of every key)
 };
 
 // Per parent row context written by the prep kernel and read by the item kernels.
@@ -173,6 +174,39 @@ __host__ __device__ inline long long plan_key_sets(long long rows, int nv, const
 struct KeySpace {
     int EL, VL;
 };
+
+// Sets of potential child key `slot` of pattern P when every key of a batch has sets
+// (single-pass last level): bitmaps or bytemaps (kind) over the label classes of the
+// child's vertices, lw[l] words for label rank l, lcum[l] = lw[0] + ... + lw[l-1].  The
+// pattern's keys start at P.set_base: its backward keys (the parent's vertex sets), then
+// per (segment, edge label) one key per label rank (the parent's sets plus the new vertex's).
+__host__ __device__ inline void all_key_set(const PatDev& P, int slot, KeySpace ks, const int* lw,
+                                            const int* lcum, int kind, SetDesc* d) {
+    const int nb = (P.r - 1) * ks.EL;
+    const bool fwd = slot >= nb;
+    const int tl = fwd ? (slot - nb) % ks.VL : 0;
+    d->nv = P.nv + (fwd ? 1 : 0);
+    d->kinds = 0;
+    int w = 0;
+    for (int v = 0; v <= GS_MAXV; ++v) {
+        d->vo[v] = w;
+        if (v < d->nv) {
+            d->kinds |= kind << (2 * v);
+            w += lw[v < P.nv ? P.vlab[v] : tl];
+        }
+    }
+    const long long pw = d->vo[P.nv];  // words of the parent's vertex sets
+    long long off = P.set_base;
+    if (!fwd) {
+        off += (long long)slot * pw;
+    } else {
+        const int grp = (slot - nb) / ks.VL;  // segment * EL + edge label
+        off += (long long)nb * pw + (long long)grp * ((long long)ks.VL * pw + lcum[ks.VL]) +
+               (long long)tl * pw + lcum[tl];
+    }
+    d->off = off;
+    d->pad[0] = d->pad[1] = 0;
+}
 
 struct DevGraph {
     int n, nnz;
@@ -315,6 +349,12 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
                     const int* acnt, const int* flags, int na, const int* counts, int mni_mode,
                     int min_sup, int* sup, unsigned long long* totals, hipStream_t s);
+// Sets of every potential child key of a batch (single-pass last level): each key's vertex
+// sets are bitmaps or bytemaps over their label classes, lw[l] words for label rank l and
+// lcum[l] = lw[0] + ... + lw[l-1] (VL + 1 entries); a pattern's keys start at its set_base:
+// its backward keys (parent sets), then per (segment, edge label) one key per label rank.
+hipError_t all_key_sets(const PatDev* pats, int npats, KeySpace ks, long long K, const int* lw,
+                        const int* lcum, int kind, SetDesc* sets, hipStream_t s);
 hipError_t gather_frequent(const int* fidx, const int* nf, int max_f, const int* akeys,
                            const int* acnt, const long long* row_off, int* fkey, int* fcnt,
                            long long* frow, hipStream_t s);

@@ -739,6 +739,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     const int x0 = blockIdx.x % GS_XCDS;
     const int TB = (int)max(1LL, min(16LL, ntiles / (8LL * gridDim.x)));
     const bool w0 = threadIdx.x < 64;  // wave 0 claims tiles and finds their first rows
+    const int lane = lane_id();
     int xi = 0;                        // queues found empty (lane 0 of wave 0 only)
     long long nxt = 0, claim_end = 0;  // claimed, not yet handed out (lane 0 of wave 0)
     auto claim = [&]() -> long long {  // lane 0 of wave 0
@@ -834,38 +835,75 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             } else if (it < tn) {
                 o = eval_item<false, FILL>(g, pats[p], c, f, ks, gd.err);
             }
-            if (!o.valid) continue;
-            if (!in_range(o.key, gd.keys)) {
-                atomicOr(gd.err, GS_GUARD_KEY);
-                continue;
+            // aggregation key: child key (count, single pass) or live slot (fill); a child
+            // that is not written (not canonical) has no slot
+            bool act = false;
+            int agg = -1;
+            if (o.valid) {
+                if (!in_range(o.key, gd.keys)) {
+                    atomicOr(gd.err, GS_GUARD_KEY);
+                } else {
+                    agg = (FILL && key_slot) ? key_slot[o.key] : o.key;
+                    if (FILL && agg >= 0 && !in_range(agg, gd.slots)) atomicOr(gd.err, GS_GUARD_SLOT);
+                    else act = agg >= 0;
+                }
             }
-            // aggregation key: child key (count, single pass) or live slot (fill)
-            const int agg = (FILL && key_slot) ? key_slot[o.key] : o.key;
-            if (agg < 0) continue;  // a child that is not written (not canonical)
-            if (FILL && !in_range(agg, gd.slots)) {
-                atomicOr(gd.err, GS_GUARD_SLOT);
-                continue;
+            // the child's sets: one per child vertex (parent's nv, plus the new vertex of a
+            // forward child), indexed by label-class rank
+            SetRegs d{};
+            if (FILL && act) {
+                d = load_set(sets + agg);
+                if (!set_fits(d, o.nv + (o.newv >= 0 ? 1 : 0), gd.pool_words)) {
+                    atomicOr(gd.err, GS_GUARD_POOL);
+                    act = false;
+                }
             }
-            if (FILL) {
-                const SetRegs d = load_set(sets + agg);
+            // Wave-level aggregation: consecutive items mostly share a key (a row's
+            // neighbours are ordered by (label, edge label)), so the lanes of one key take
+            // their tile counts with one LDS atomic by their first lane; keys left after a
+            // few rounds take one atomic per lane.  lead: this lane inserts the parent's
+            // images for its (slot, row) pair -- the first item of the pair in the tile.
+            bool lead = true;
+            unsigned long long pend = __ballot(act);
+            for (int round = 0; round < 4 && pend; ++round) {
+                const int leader = __ffsll((long long)pend) - 1;
+                const int lkey = __shfl(agg, leader);
+                const int lrow = __shfl(row, leader);
+                const bool mine = act && agg == lkey;
+                const unsigned long long m = __ballot(mine);
+                int h = 0, base = 0, first = 0;
+                if (lane == leader) {
+                    h = lds_slot<LDS_SLOTS>(s_key, lkey);
+                    base = atomicAdd(&s_cnt[h], (int)__popcll(m));
+                    // a pair is named by the key's slot in this tile's table and the row:
+                    // below 2 * TILE * TILE, so distinct pairs never share a name
+                    if (FILL) first = (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + lrow));
+                }
+                h = __shfl(h, leader);
+                base = __shfl(base, leader);
+                if (FILL) first = __shfl(first, leader);
+                if (mine) {
+                    hs[q] = h;
+                    rank[q] = base + (int)mbcnt64(m);
+                    if (FILL) {
+                        lead = row == lrow ? (lane == leader && first != 0)
+                                           : (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + row));
+                    }
+                    act = false;
+                }
+                pend &= ~m;
+            }
+            if (act) {
                 hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
                 rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
-                // the pair is named by the key's slot in this tile's table (one slot per
-                // live key of the tile) and the row: below 2 * TILE * TILE, so distinct
-                // pairs never share a name (agg * TILE would wrap past 2^31 / TILE keys)
-                const bool lead =
-                    !dedup || lds_first<LDS_SLOTS>(s_pair, hs[q] * TILE + row);
+                if (FILL) lead = !dedup || lds_first<LDS_SLOTS>(s_pair, hs[q] * TILE + row);
+            }
+            if (FILL && hs[q] >= 0) {
                 if (ROWS) {
                     eid[q] = o.eid;
                     prow[q] = (int)(o.pbase + b0 + rlo + row);
                 }
-                // the child's sets: one per child vertex (parent's nv, plus the new vertex
-                // of a forward child), indexed by label-class rank
                 const int nv = o.nv;
-                if (!set_fits(d, nv + (o.newv >= 0 ? 1 : 0), gd.pool_words)) {
-                    atomicOr(gd.err, GS_GUARD_POOL);
-                    continue;
-                }
                 if (lead) {
 #pragma unroll
                     for (int v = 0; v < GS_MAXV; ++v) {
@@ -877,9 +915,6 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     }
                 }
                 if (o.newv >= 0) set_insert(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
-            } else {
-                hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
-                rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
             }
         }
         // wave 0: the next tile's first row -- from this tile's offsets when it follows
@@ -1657,6 +1692,20 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
     }
 }
 
+// Descriptor of every potential child key k < K (see gsk::all_key_sets).
+__global__ void k_all_key_sets(const PatDev* __restrict__ pats, int npats, KeySpace ks,
+                               long long K, const int* __restrict__ lw,
+                               const int* __restrict__ lcum, int kind, SetDesc* sets) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long key = (long long)blockIdx.x * blockDim.x + threadIdx.x; key < K;
+         key += stride) {
+        const PatDev& P = pats[owner_of(pats, npats, (int)key)];
+        SetDesc d;
+        all_key_set(P, (int)(key - P.key_base), ks, lw, lcum, kind, &d);
+        sets[key] = d;
+    }
+}
+
 // The frequent children's keys, counts and first rows, by their active index.
 __global__ void k_gather_frequent(const int* __restrict__ fidx, const int* __restrict__ nf,
                                   const int* __restrict__ akeys, const int* __restrict__ acnt,
@@ -1916,6 +1965,14 @@ hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys
     hipLaunchKernelGGL(k_supports, dim3(grid_for(na, GS_BLOCK, 16384)), dim3(GS_BLOCK), 0, s,
                        pats, npats, ks, akeys, acnt, flags, na, counts, mni_mode, min_sup, sup,
                        totals);
+    return hipGetLastError();
+}
+
+hipError_t all_key_sets(const PatDev* pats, int npats, KeySpace ks, long long K, const int* lw,
+                        const int* lcum, int kind, SetDesc* sets, hipStream_t s) {
+    if (K <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_all_key_sets, dim3(grid_for(K, 256, 16384)), dim3(256), 0, s, pats,
+                       npats, ks, K, lw, lcum, kind, sets);
     return hipGetLastError();
 }
 

@@ -437,6 +437,13 @@ long long key_space(int r, int EL, int VL) {
     return (long long)(r - 1) * EL + (long long)(r + 1) * EL * VL;
 }
 
+// Pattern of a batch owning child key `key` (patterns ordered by key_base).
+int owner_of(const std::vector<PatDev>& pd, int key) {
+    const auto it = std::upper_bound(pd.begin(), pd.end(), key,
+                                     [](int k, const PatDev& d) { return k < d.key_base; });
+    return int(it - pd.begin()) - 1;
+}
+
 PatDev to_dev(const HPat& h) {
     PatDev d{};
     d.m = h.m;
@@ -945,50 +952,54 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // Single pass at the last level: when label-class sets for every potential child key fit
     // in a quarter of the free budget, the fill pass counts children and fills the MNI sets
     // in one go (no count pass); canonicality is then checked on the keys that occurred.
-    std::vector<SetDesc> all_sets;
+    // A pattern's potential keys: its backward keys hold the parent's vertex sets; each
+    // forward (segment, edge label) holds VL keys, the parent's sets plus one set over each
+    // label class.  The descriptors are written on the device (gsk::all_key_sets).
     long long all_words = 0;
     bool single = false, single_bytes = false;
-    if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
-        // words of every potential child key's sets: per pattern, its backward keys hold the
-        // parent's vertex sets, each forward (segment, edge label) holds VL keys with the
-        // parent's sets plus one set over each label class
-        auto total_words = [&](bool by) {
-            long long tw = 0, all_l = 0;
-            for (int l = 0; l < c->VL; ++l) all_l += class_words(c, l, by);
-            for (int i = 0; i < B; ++i) {
-                long long pw = 0;
-                for (int v = 0; v < pd[i].nv; ++v) pw += class_words(c, pd[i].vlab[v], by);
-                const long long nb = (long long)(pd[i].r - 1) * c->EL;
-                const long long nf = (long long)(pd[i].r + 1) * c->EL;
-                tw += nb * pw + nf * ((long long)c->VL * pw + all_l);
-            }
-            return tw;
-        };
-        const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
-        if (want_bytes(c) && (size_t)total_words(true) * 4 <= avail / 4) {
-            single = single_bytes = true;
-        } else if (c->opts.set_kind != GS_SETS_BYTES) {
-            single = (size_t)total_words(false) * 4 <= avail / 4;
+    std::vector<int> lw(c->VL), lcum(c->VL + 1);
+    auto class_table = [&](bool by) {
+        lcum[0] = 0;
+        for (int l = 0; l < c->VL; ++l) {
+            lw[l] = (int)class_words(c, l, by);
+            lcum[l + 1] = lcum[l] + lw[l];
         }
+    };
+    auto total_words = [&](bool by) {  // also places each pattern's keys (set_base)
+        class_table(by);
+        long long tw = 0;
+        for (int i = 0; i < B; ++i) {
+            long long pw = 0;
+            for (int v = 0; v < pd[i].nv; ++v) pw += lw[pd[i].vlab[v]];
+            const long long nb = (long long)(pd[i].r - 1) * c->EL;
+            const long long nf = (long long)(pd[i].r + 1) * c->EL;
+            pd[i].set_base = tw;
+            tw += nb * pw + nf * ((long long)c->VL * pw + lcum[c->VL]);
+        }
+        return tw;
+    };
+    if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
+        const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
+        long long tw = 0;
+        if (want_bytes(c) && (size_t)(tw = total_words(true)) * 4 <= avail / 4) {
+            single = single_bytes = true;
+        } else if (c->opts.set_kind != GS_SETS_BYTES &&
+                   (size_t)(tw = total_words(false)) * 4 <= avail / 4) {
+            single = true;
+        }
+        if (single) all_words = tw;
     }
     DBuf d_all_sets, d_all_pool;
     if (single) {
-        all_sets.resize(K);
-        for (int i = 0; i < B; ++i) {
-            const int K_p = (int)key_space(pd[i].r, c->EL, c->VL);
-            const int nb = (pd[i].r - 1) * c->EL;
-            int lab[GS_MAXV];
-            for (int v = 0; v < GS_MAXV; ++v) lab[v] = pd[i].vlab[v];
-            for (int sl = 0; sl < K_p; ++sl) {
-                SetDesc& d = all_sets[pd[i].key_base + sl];
-                int nv = pd[i].nv;
-                if (sl >= nb) lab[nv++] = (sl - nb) % c->VL;  // the forward child's new vertex
-                plan_set(c, kAllRows, nv, lab, single_bytes, d, all_words);
-            }
-        }
+        DBuf d_lw(P, sizeof(int) * c->VL), d_lcum(P, sizeof(int) * (c->VL + 1));
+        h2d(c, d_pats.as<PatDev>(), pd.data(), B);  // with set_base
+        h2d(c, d_lw.as<int>(), lw.data(), c->VL);
+        h2d(c, d_lcum.as<int>(), lcum.data(), c->VL + 1);
         d_all_sets = DBuf(P, sizeof(SetDesc) * K);
         d_all_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(all_words, 4));
-        h2d(c, d_all_sets.as<SetDesc>(), all_sets.data(), K);
+        HIPCHECK(gsk::all_key_sets(d_pats.as<PatDev>(), B, ks, K, d_lw.as<int>(),
+                                   d_lcum.as<int>(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP,
+                                   d_all_sets.as<SetDesc>(), c->stream));
         HIPCHECK(gsk::fill_u32(d_all_pool.as<uint32_t>(), 0u, all_words, c->stream));
     }
 
@@ -1480,7 +1491,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             SetDesc d;
             const ChildInfo& ch = out.children[live_child[li]];
             if (single) {
-                d = all_sets[live_keys[li]];
+                const int key = live_keys[li];
+                const int p = owner_of(pd, key);
+                all_key_set(pd[p], key - pd[p].key_base, KeySpace{c->EL, c->VL}, lw.data(),
+                            lcum.data(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP, &d);
             } else {
                 int lab[GS_MAXV];
                 ch.pat.labels(lab);
@@ -1935,15 +1949,16 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
             adj[i].x = (unsigned)e[1];
             adj[i].y = ((unsigned)emap[e[2]] << 16) | (unsigned)vr[e[1]];
         }
-        // each vertex's entries ordered by neighbour label, with their CSR indices
+        // each vertex's entries ordered by (neighbour label, edge label), with their CSR
+        // indices: the entries of one child key are then consecutive work items
         std::vector<uint2> adjl(nnz);
         std::vector<int> eidl(nnz);
+        auto lab_el = [&](int i) { return ((adj[i].y & 0xffffu) << 16) | (adj[i].y >> 16); };
         for (int v = 0; v < n; ++v) {
             const int o0 = offsets[v], o1 = offsets[v + 1];
             for (int i = o0; i < o1; ++i) eidl[i] = i;
-            std::stable_sort(eidl.begin() + o0, eidl.begin() + o1, [&](int a, int b) {
-                return (adj[a].y & 0xffffu) < (adj[b].y & 0xffffu);
-            });
+            std::stable_sort(eidl.begin() + o0, eidl.begin() + o1,
+                             [&](int a, int b) { return lab_el(a) < lab_el(b); });
             for (int i = o0; i < o1; ++i) adjl[i] = adj[eidl[i]];
         }
         // label classes: each vertex's rank among the vertices of its label (ascending id)
