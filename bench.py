@@ -346,6 +346,8 @@ def main() -> int:
         "autotune": tuning,
         "cpu_baseline": cpu,
         "peak_device_bytes": st["peak_device_bytes"],
+        # host time per step: waiting at synchronisations, planning children, ordering reports
+        "host_ms_per_step": {k: round(st[f"t_{k}_ms"] / steps, 3) for k in ("sync", "plan", "order")},
     }
     print(json.dumps(out), flush=True)
     return 0

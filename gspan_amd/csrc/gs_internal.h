@@ -266,7 +266,12 @@ long long scan_tmp_elems(long long n);
 struct ItemsLaunch {
     int tile;
     int grid;
+    int m;  // edges of the parent patterns (the kernel is specialised by size class)
 };
+// Size classes of the extension kernel: parents of 1 edge, 2 edges, up to GS_MAXE - 1.
+inline int items_pm(int m) { return m <= 1 ? 1 : m == 2 ? 2 : GS_MAXE - 1; }
+inline int items_pm_class(int m) { return m <= 1 ? 0 : m == 2 ? 1 : 2; }
+#define GS_PM_CLASSES 3
 
 // Modes of the extension kernel: count children per key; fill the MNI image sets of the
 // children (and count them, in the single-pass last level) without writing rows; fill
@@ -303,7 +308,7 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
                       int nlive, bool saturate, const ItemGuard& gd, hipStream_t s);
 // resident workgroups per CU of the extension kernel in a mode (GS_ITEMS_*) at a tile size
-int items_blocks_per_cu(int mode, int tile);
+int items_blocks_per_cu(int mode, int tile, int m);
 
 // Keys with a non-zero count, in ascending key order, with their counts (flags / pos: K
 // and K+1 elements; scan_tmp as for scan_exclusive over K); *n_active is written and

@@ -69,6 +69,26 @@ __device__ __forceinline__ int pick(const int (&a)[N], int i) {
     return r;
 }
 
+// The same over the first N entries only (i < N): a kernel specialised for patterns of at
+// most PM edges selects among PM + 1 vertices or segments instead of all eight.
+template <int N>
+__device__ __forceinline__ int pickv(const int8v& v, int i) {
+    static_assert(N >= 1 && N <= 8, "int8v entries");
+    int r = v[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) r = (i == k) ? v[k] : r;
+    return r;
+}
+
+template <int N, int M>
+__device__ __forceinline__ int picka(const int (&a)[M], int i) {
+    static_assert(N >= 1 && N <= M, "array entries");
+    int r = a[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) r = (i == k) ? a[k] : r;
+    return r;
+}
+
 // A set descriptor in registers (four 16-byte loads); its offsets are picked by
 // compare-and-select, like a row context's arrays.
 struct SetRegs {
@@ -91,15 +111,21 @@ __device__ __forceinline__ SetRegs load_set(const SetDesc* p) {
     return r;
 }
 
+// d.vo[i] for i <= NV (a descriptor of at most NV vertex sets)
+template <int NV>
 __device__ __forceinline__ int vo_at(const SetRegs& d, int i) {
-    return i >= GS_MAXV ? d.vo8 : pick(d.vo, i);
+    static_assert(NV >= 1 && NV <= GS_MAXV, "vertex sets");
+    if constexpr (NV == GS_MAXV) return i >= GS_MAXV ? d.vo8 : pickv<GS_MAXV>(d.vo, i);
+    else return pickv<NV + 1>(d.vo, i);
 }
 
 // Descriptor d addresses words [d.off, d.off + d.vo[d.nv]) of a pool of pool_words words,
-// for a child of at least `need` vertices.
+// for a child of at least `need` and at most NV vertices.
+template <int NV>
 __device__ __forceinline__ bool set_fits(const SetRegs& d, int need, long long pool_words) {
-    const int all = vo_at(d, d.nv);
-    return d.off >= 0 && d.nv >= need && d.nv <= GS_MAXV && all >= 0 && d.off + all <= pool_words;
+    if (d.nv < need || d.nv > NV) return false;
+    const int all = vo_at<NV>(d, d.nv);
+    return d.off >= 0 && all >= 0 && d.off + all <= pool_words;
 }
 
 // Inserts the vertex of label-class rank x into vertex set v of descriptor d (x indexes
@@ -108,12 +134,14 @@ __device__ __forceinline__ bool set_fits(const SetRegs& d, int need, long long p
 // counts (>= 2x the rows) never fills; one sized from predicted counts can, and then raises
 // *overflow after probing every slot once.  An x past a bitmap or bytemap sets the guard
 // bit GS_GUARD_POOL in *gerr and is skipped.
+// NV: the descriptor holds at most NV vertex sets (v < NV).
+template <int NV>
 __device__ __forceinline__ void set_insert(const SetRegs& d, uint32_t* pool, int v, uint32_t x,
                                            int* gerr, int* overflow = nullptr) {
     const int kind = (d.kinds >> (2 * v)) & 3;
-    const int v0 = vo_at(d, v);
+    const int v0 = vo_at<NV>(d, v);
     uint32_t* base = pool + d.off + v0;
-    const uint32_t words = (uint32_t)(vo_at(d, v + 1) - v0);
+    const uint32_t words = (uint32_t)(vo_at<NV>(d, v + 1) - v0);
     if (kind == GS_SET_BYTEMAP) {
         if (x < 4u * words) reinterpret_cast<unsigned char*>(base)[x] = 1;
         else atomicOr(gerr, GS_GUARD_POOL);
@@ -145,22 +173,23 @@ __device__ __forceinline__ void set_insert(const SetRegs& d, uint32_t* pool, int
 // Element i of a pattern table: when the pattern is the same for the whole wave (UNI)
 // its tables are wave-uniform, read with scalar loads and selected per lane; otherwise
 // one vector load.
-template <bool UNI, int N>
-__device__ __forceinline__ int pget(const int (&a)[N], int i) {
-    if (UNI) return pick(a, i);
+// (i < N)
+template <bool UNI, int N, int M>
+__device__ __forceinline__ int pget(const int (&a)[M], int i) {
+    if (UNI) return picka<N>(a, i);
     return a[i];
 }
 
 // has_edge (gspan.cu:396-401) on a row given by its vertex images: some pattern edge
-// with label el joins u and w.
-template <bool UNI>
+// with label el joins u and w (a pattern of at most PM edges).
+template <bool UNI, int PM>
 __device__ __forceinline__ bool row_has_edge(const PatDev& P, const RowRegs& c, int u, int w,
                                              int el) {
     bool hit = false;
 #pragma unroll
-    for (int j = 0; j < GS_MAXE; ++j) {
+    for (int j = 0; j < PM; ++j) {
         if (j < P.m && P.cel[j] == el) {
-            const int a = pick(c.phi, P.cfrom[j]), b = pick(c.phi, P.cto[j]);
+            const int a = pickv<PM + 1>(c.phi, P.cfrom[j]), b = pickv<PM + 1>(c.phi, P.cto[j]);
             hit |= (a == u && b == w) || (a == w && b == u);
         }
     }
@@ -186,17 +215,18 @@ __device__ __forceinline__ bool in_range(long long x, long long n) {
 // c is the row's context in registers.  Vertex ids and CSR entries are checked against
 // the graph before they are used as indices (GS_GUARD_GRAPH).  NEWX: a forward child also
 // returns the label-class rank of its new vertex (for its MNI set).
-template <bool UNI, bool NEWX>
+template <bool UNI, bool NEWX, int PM>
 __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowRegs& c,
                                              int q, KeySpace ks, int* gerr) {
+    constexpr int NV = PM + 1;  // parent vertices; rightmost path <= PM edges
     ItemOut o{false, 0, 0, -1, P.nv, P.row_begin - P.brow_begin};
     const int r = P.r;
     if (q < r - 1) {
         const int i = q + 1;
-        const int u_n = pick(c.phi, P.v_n);
-        const int target = pick(c.phi, pget<UNI>(P.bw_v, i));
-        const int e_el = pget<UNI>(P.bw_el, i);
-        const bool le = pget<UNI>(P.bw_le, i) != 0;
+        const int u_n = pickv<NV>(c.phi, P.v_n);
+        const int target = pickv<NV>(c.phi, pget<UNI, PM>(P.bw_v, i));
+        const int e_el = pget<UNI, PM>(P.bw_el, i);
+        const bool le = pget<UNI, PM>(P.bw_le, i) != 0;
         if (!in_range(u_n, g.n)) {
             atomicOr(gerr, GS_GUARD_GRAPH);
             return o;
@@ -213,7 +243,7 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
             const uint2 a = g.adj[e];
             if ((int)a.x != target) break;
             const int el = (int)(a.y >> 16);
-            if (row_has_edge<UNI>(P, c, u_n, target, el)) continue;
+            if (row_has_edge<UNI, PM>(P, c, u_n, target, el)) continue;
             if (e_el < el || (e_el == el && le)) {
                 o.valid = true;
                 o.eid = e;
@@ -224,11 +254,11 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
         return o;
     }
     const int f = q - (r - 1);
-    if (f >= pick(c.fcum, r)) return o;  // the empty item of a row with no forward item
+    if (f >= pickv<NV>(c.fcum, r)) return o;  // the empty item of a row with no forward item
     int j = 0;  // segment: fcum is non-decreasing
 #pragma unroll
-    for (int s = 0; s < GS_MAXE - 1; ++s) j += (s < r && f >= c.fcum[s]) ? 1 : 0;
-    const int el_idx = pick(c.base, j) + f;  // in adjl: neighbours by label from min_label
+    for (int s = 0; s < PM; ++s) j += (s < r && f >= c.fcum[s]) ? 1 : 0;
+    const int el_idx = pickv<NV>(c.base, j) + f;  // in adjl: neighbours by label from min_label
     if (!in_range(el_idx, g.nnz)) {
         atomicOr(gerr, GS_GUARD_GRAPH);
         return o;
@@ -239,10 +269,10 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
     if (tl < P.min_label) return o;
     bool seen = false;
 #pragma unroll
-    for (int v = 0; v < GS_MAXV; ++v) seen |= v < P.nv && c.phi[v] == to;
+    for (int v = 0; v < NV; ++v) seen |= v < P.nv && c.phi[v] == to;
     if (seen) return o;
-    const int e_el = pget<UNI>(P.seg_el, j);
-    if (!(e_el < el || (e_el == el && pget<UNI>(P.seg_tl, j) <= tl))) return o;
+    const int e_el = pget<UNI, NV>(P.seg_el, j);
+    if (!(e_el < el || (e_el == el && pget<UNI, NV>(P.seg_tl, j) <= tl))) return o;
     o.valid = true;
     o.eid = e;
     o.newv = NEWX ? g.adjlx[el_idx] : 0;
@@ -329,12 +359,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
                     else atomicOr(gd.err, GS_GUARD_ROW);
                 }
                 const SetRegs d = load_set(sets + slot);
-                if (!set_fits(d, 2, gd.pool_words)) {
+                if (!set_fits<2>(d, 2, gd.pool_words)) {
                     atomicOr(gd.err, GS_GUARD_POOL);
                     continue;
                 }
-                set_insert(d, pool, 0, (uint32_t)g.lidx[g.src[i]], gd.err);
-                set_insert(d, pool, 1, (uint32_t)g.lidx[g.adj[i].x], gd.err);
+                set_insert<2>(d, pool, 0, (uint32_t)g.lidx[g.src[i]], gd.err);
+                set_insert<2>(d, pool, 1, (uint32_t)g.lidx[g.adj[i].x], gd.err);
             }
         }
         __syncthreads();
@@ -676,7 +706,7 @@ __device__ __forceinline__ int wave_find_lds(const int* off, int n, int item) {
 // half full, after 2^14 tiles (so no LDS count passes 2^24) and when the workgroup ends:
 // the counters of a level's hot keys see one atomic per workgroup instead of one per tile
 // (device-scope atomics are performed at the memory side and serialise on one address).
-template <int MODE, int TILE>
+template <int MODE, int TILE, int PM>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
                                                     const long long* __restrict__ woff,
@@ -691,6 +721,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     ItemGuard gd, int dedup, int saturate) {
     constexpr bool FILL = MODE != GS_ITEMS_COUNT;
     constexpr bool ROWS = MODE == GS_ITEMS_ROWS;
+    constexpr int CV = PM + 2;  // vertices of a child (sets of a child key)
+    static_assert(PM >= 1 && PM < GS_MAXE, "parent edges");
     constexpr int LDS_SLOTS = 2 * TILE;
     constexpr int IPT = TILE / GS_BLOCK;
     constexpr int STAGE = (TILE + 1 + GS_BLOCK - 1) / GS_BLOCK;  // offsets per thread
@@ -831,9 +863,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             // the pattern's tables are read once per wave with scalar loads
             const int p0 = __builtin_amdgcn_readfirstlane(p);
             if (__all(p == p0 || p < 0)) {
-                if (it < tn) o = eval_item<true, FILL>(g, pats[p0], c, f, ks, gd.err);
+                if (it < tn) o = eval_item<true, FILL, PM>(g, pats[p0], c, f, ks, gd.err);
             } else if (it < tn) {
-                o = eval_item<false, FILL>(g, pats[p], c, f, ks, gd.err);
+                o = eval_item<false, FILL, PM>(g, pats[p], c, f, ks, gd.err);
             }
             // aggregation key: child key (count, single pass) or live slot (fill); a child
             // that is not written (not canonical) has no slot
@@ -853,7 +885,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             SetRegs d{};
             if (FILL && act) {
                 d = load_set(sets + agg);
-                if (!set_fits(d, o.nv + (o.newv >= 0 ? 1 : 0), gd.pool_words)) {
+                if (!set_fits<CV>(d, o.nv + (o.newv >= 0 ? 1 : 0), gd.pool_words)) {
                     atomicOr(gd.err, GS_GUARD_POOL);
                     act = false;
                 }
@@ -866,9 +898,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             bool lead = true;
             unsigned long long pend = __ballot(act);
             for (int round = 0; round < 4 && pend; ++round) {
+                // the leader is wave-uniform: values move by readlane, not by LDS shuffles
                 const int leader = __ffsll((long long)pend) - 1;
-                const int lkey = __shfl(agg, leader);
-                const int lrow = __shfl(row, leader);
+                const int lkey = __builtin_amdgcn_readlane(agg, leader);
+                const int lrow = __builtin_amdgcn_readlane(row, leader);
                 const bool mine = act && agg == lkey;
                 const unsigned long long m = __ballot(mine);
                 int h = 0, base = 0, first = 0;
@@ -879,9 +912,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     // below 2 * TILE * TILE, so distinct pairs never share a name
                     if (FILL) first = (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + lrow));
                 }
-                h = __shfl(h, leader);
-                base = __shfl(base, leader);
-                if (FILL) first = __shfl(first, leader);
+                h = __builtin_amdgcn_readlane(h, leader);
+                base = __builtin_amdgcn_readlane(base, leader);
+                if (FILL) first = __builtin_amdgcn_readlane(first, leader);
                 if (mine) {
                     hs[q] = h;
                     rank[q] = base + (int)mbcnt64(m);
@@ -906,15 +939,15 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 const int nv = o.nv;
                 if (lead) {
 #pragma unroll
-                    for (int v = 0; v < GS_MAXV; ++v) {
+                    for (int v = 0; v < PM + 1; ++v) {
                         if (v >= nv) continue;
                         const int x = c.phi[v];
                         if (in_range(x, g.n))
-                            set_insert(d, pool, v, (uint32_t)g.lidx[x], gd.err, overflow);
+                            set_insert<CV>(d, pool, v, (uint32_t)g.lidx[x], gd.err, overflow);
                         else atomicOr(gd.err, GS_GUARD_GRAPH);
                     }
                 }
-                if (o.newv >= 0) set_insert(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
+                if (o.newv >= 0) set_insert<CV>(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
             }
         }
         // wave 0: the next tile's first row -- from this tile's offsets when it follows
@@ -1103,7 +1136,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets, lo
     const long long waves = (long long)gridDim.x * WPB;
     for (long long si = (long long)blockIdx.x * WPB + threadIdx.x / 64; si < nsets; si += waves) {
         const SetDesc d = sets[si];
-        const bool ok = d.nv == 0 || set_fits(load_set(sets + si), 0, pool_words);
+        const bool ok = d.nv == 0 || set_fits<GS_MAXV>(load_set(sets + si), 0, pool_words);
         if (!ok && lane == 0) atomicOr(gerr, GS_GUARD_POOL);
         for (int v = 0; v < GS_MAXV; ++v) {
             int c = 0;
@@ -1156,12 +1189,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_support_rows(DevGraph g, const Pat
         idx = rr.x;
     }
     const SetRegs d = load_set(set);
-    if (!set_fits(d, P.nv, pool_words)) {
+    if (!set_fits<GS_MAXV>(d, P.nv, pool_words)) {
         atomicOr(gerr, GS_GUARD_POOL);
         return;
     }
     for (int v = 0; v < P.nv; ++v) {
-        if (in_range(phi[v], g.n)) set_insert(d, pool, v, (uint32_t)g.lidx[phi[v]], gerr);
+        if (in_range(phi[v], g.n)) set_insert<GS_MAXV>(d, pool, v, (uint32_t)g.lidx[phi[v]], gerr);
         else atomicOr(gerr, GS_GUARD_GRAPH);
     }
 }
@@ -1794,16 +1827,23 @@ hipError_t launch_items(const ItemsLaunch& lc, const DevGraph& g, const PatDev* 
                         const ItemGuard& gd, int dedup, int saturate, hipStream_t s) {
     int* tile_counters = tq.base + tq.cur * GS_TILE_QUEUE;
     int* next_counters = tq.base + (1 - tq.cur) * GS_TILE_QUEUE;
-#define GS_LAUNCH_ITEMS(T)                                                                     \
-    hipLaunchKernelGGL((k_items<MODE, T>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats, ctx, \
-                       woff, b0, R, ks, key_count, key_slot, cursor, cap_end, rows_out, sets,  \
-                       pool, overflow, tile_counters, next_counters, gd, dedup, saturate)
+    if (lc.m < 1 || lc.m >= GS_MAXE) return hipErrorInvalidValue;
+    const int pm = items_pm(lc.m);
+#define GS_LAUNCH_ITEMS(T, PM)                                                                 \
+    hipLaunchKernelGGL((k_items<MODE, T, PM>), dim3(lc.grid), dim3(GS_BLOCK), 0, s, g, pats,  \
+                       ctx, woff, b0, R, ks, key_count, key_slot, cursor, cap_end, rows_out,   \
+                       sets, pool, overflow, tile_counters, next_counters, gd, dedup, saturate)
+#define GS_LAUNCH_PM(T)                                  \
+    if (pm == 1) GS_LAUNCH_ITEMS(T, 1);                  \
+    else if (pm == 2) GS_LAUNCH_ITEMS(T, 2);             \
+    else GS_LAUNCH_ITEMS(T, GS_MAXE - 1)
     switch (lc.tile) {
-        case 256: GS_LAUNCH_ITEMS(256); break;
-        case 512: GS_LAUNCH_ITEMS(512); break;
-        case 1024: GS_LAUNCH_ITEMS(1024); break;
+        case 256: GS_LAUNCH_PM(256); break;
+        case 512: GS_LAUNCH_PM(512); break;
+        case 1024: GS_LAUNCH_PM(1024); break;
         default: return hipErrorInvalidValue;
     }
+#undef GS_LAUNCH_PM
 #undef GS_LAUNCH_ITEMS
     const hipError_t e = hipGetLastError();
     if (e == hipSuccess) tq.cur = 1 - tq.cur;
@@ -1838,24 +1878,35 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                                        dedup, saturate ? 1 : 0, s);
 }
 
-int items_blocks_per_cu(int mode, int tile) {
-    int per_cu = 0;
-    hipError_t e = hipErrorInvalidValue;
-#define GS_OCC(M, T) \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_items<M, T>, GS_BLOCK, 0)
-#define GS_OCC_TILES(M)                          \
-    if (tile == 256) GS_OCC(M, 256);             \
-    else if (tile == 512) GS_OCC(M, 512);        \
-    else if (tile == 1024) GS_OCC(M, 1024)
-    if (mode == GS_ITEMS_COUNT) {
-        GS_OCC_TILES(GS_ITEMS_COUNT);
-    } else if (mode == GS_ITEMS_SETS) {
-        GS_OCC_TILES(GS_ITEMS_SETS);
-    } else {
-        GS_OCC_TILES(GS_ITEMS_ROWS);
+template <int MODE, int TILE>
+hipError_t occupancy_of(int pm, int* per_cu) {
+    if (pm == 1)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_items<MODE, TILE, 1>,
+                                                            GS_BLOCK, 0);
+    if (pm == 2)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_items<MODE, TILE, 2>,
+                                                            GS_BLOCK, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_items<MODE, TILE, GS_MAXE - 1>,
+                                                        GS_BLOCK, 0);
+}
+
+template <int MODE>
+hipError_t occupancy_tiles(int tile, int pm, int* per_cu) {
+    switch (tile) {
+        case 256: return occupancy_of<MODE, 256>(pm, per_cu);
+        case 512: return occupancy_of<MODE, 512>(pm, per_cu);
+        case 1024: return occupancy_of<MODE, 1024>(pm, per_cu);
+        default: return hipErrorInvalidValue;
     }
-#undef GS_OCC_TILES
-#undef GS_OCC
+}
+
+int items_blocks_per_cu(int mode, int tile, int m) {
+    int per_cu = 0;
+    const int pm = items_pm(m);
+    hipError_t e;
+    if (mode == GS_ITEMS_COUNT) e = occupancy_tiles<GS_ITEMS_COUNT>(tile, pm, &per_cu);
+    else if (mode == GS_ITEMS_SETS) e = occupancy_tiles<GS_ITEMS_SETS>(tile, pm, &per_cu);
+    else e = occupancy_tiles<GS_ITEMS_ROWS>(tile, pm, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     return per_cu;
 }

@@ -19,19 +19,25 @@ class LaunchTuner {
    public:
     static constexpr long long kMinRows = 4096;  // smaller launches are not timed for tuning
 
+    // shapes per kernel mode and parent size class (the specialised kernels differ in
+    // registers, so in the workgroups that fit a CU)
     void init(int cus, bool enabled, int forced_tile) {
         enabled_ = enabled && forced_tile == 0;
         for (int f = 0; f < GS_ITEMS_MODES; ++f) {
-            cfgs_[f].clear();
-            for (int tile : {512, 1024, 256}) {  // the first shape is the default
-                if (forced_tile && tile != forced_tile) continue;
-                const int occ = gsk::items_blocks_per_cu(f, tile);
-                for (int blocks : {occ, occ / 2}) {
-                    if (blocks < 1) continue;
-                    bool dup = false;
-                    for (auto& c : cfgs_[f]) dup |= c.tile == tile && c.grid == blocks * cus;
-                    if (!dup) cfgs_[f].push_back(gsk::ItemsLaunch{tile, blocks * cus});
-                    if (!enabled_) break;  // fixed shape: full occupancy only
+            for (int pc = 0; pc < GS_PM_CLASSES; ++pc) {
+                auto& cs = cfgs_[f][pc];
+                const int m = pc == 2 ? 3 : pc + 1;  // a parent size of the class
+                cs.clear();
+                for (int tile : {512, 1024, 256}) {  // the first shape is the default
+                    if (forced_tile && tile != forced_tile) continue;
+                    const int occ = gsk::items_blocks_per_cu(f, tile, m);
+                    for (int blocks : {occ, occ / 2}) {
+                        if (blocks < 1) continue;
+                        bool dup = false;
+                        for (auto& c : cs) dup |= c.tile == tile && c.grid == blocks * cus;
+                        if (!dup) cs.push_back(gsk::ItemsLaunch{tile, blocks * cus, m});
+                        if (!enabled_) break;  // fixed shape: full occupancy only
+                    }
                 }
             }
         }
@@ -40,17 +46,21 @@ class LaunchTuner {
     // Shape for a launch over R parent rows in a kernel mode (GS_ITEMS_*); *bucket / *cfg
     // identify it for record() (-1 when the launch is not part of the search).
     gsk::ItemsLaunch choose(int mode, bool single, int m, long long R, int* bucket, int* cfg) {
-        const auto& cs = cfgs_[mode];
+        const auto& cs = cfgs_[mode][gsk::items_pm_class(m)];
         *bucket = -1;
         *cfg = 0;
-        if (!enabled_) return cs[0];
+        auto with_m = [m](gsk::ItemsLaunch lc) {
+            lc.m = m;
+            return lc;
+        };
+        if (!enabled_) return with_m(cs[0]);
         const int b = mode * 1000 + (single ? 100 : 0) + m;
         Bucket& bk = buckets_[b];
         if (bk.ms.empty()) bk.resize(cs.size());
         int best = best_of(bk);
         if (R < kMinRows) {
             *cfg = best;
-            return cs[best];
+            return with_m(cs[best]);
         }
         for (size_t i = 0; i < cs.size(); ++i)
             if (bk.n[i] == 0) {
@@ -59,7 +69,7 @@ class LaunchTuner {
             }
         *bucket = b;
         *cfg = best;
-        return cs[best];
+        return with_m(cs[best]);
     }
 
     void record(int bucket, int cfg, double ms, long long items) {
@@ -78,7 +88,7 @@ class LaunchTuner {
             const int b = kv.first;
             const Bucket& bk = kv.second;
             const int mode = b / 1000;
-            const auto& cs = cfgs_[mode];
+            const auto& cs = cfgs_[mode][gsk::items_pm_class(b % 100)];
             if (!first) s += ", ";
             first = false;
             static const char* const names[GS_ITEMS_MODES] = {"count", "sets", "rows"};
@@ -125,6 +135,6 @@ class LaunchTuner {
     }
 
     bool enabled_ = false;
-    std::vector<gsk::ItemsLaunch> cfgs_[GS_ITEMS_MODES];
+    std::vector<gsk::ItemsLaunch> cfgs_[GS_ITEMS_MODES][GS_PM_CLASSES];
     std::map<int, Bucket> buckets_;
 };
