@@ -229,6 +229,14 @@ struct DevGraph {
     const int* lidx;
     const int* adjlx;
     const int* lcnt;
+    // runs: maximal ranges of a vertex's adjl entries with one (neighbour label, edge
+    // label); run k covers adjl[rstart[k], rstart[k + 1]) (rstart[nruns] = nnz), rkey[k] =
+    // (label << 16) | edge label, runid[i] = the run of adjl entry i.  Within a run the
+    // entries ascend by neighbour id (the CSR order).
+    const int* rstart;
+    const int* rkey;
+    const int* runid;
+    int nruns;
 };
 
 // MLP parameters on the device: w32 holds every parameter in fp32 (W1[32x128], b1, W2[128x128],
@@ -254,9 +262,25 @@ hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int
 // work[row] >= 1 items per row; *work_full (nullable) += the rows' items without the
 // label filter, (r - 1) + sum of segment degrees (the reference scans every neighbour);
 // partial: ceil(R / GS_BLOCK) elements of scratch for that sum
+// runs: a row's forward work counts runs (COUNT / SETS modes), else neighbours
 hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
-                LayerPtrs L, RowCtx* ctx, int* work, unsigned long long* partial,
+                LayerPtrs L, bool runs, RowCtx* ctx, int* work, unsigned long long* partial,
                 unsigned long long* work_full, int* guard_err, hipStream_t s);
+// Groups of a chunk's rows for NEWV (after prep of the chunk, whose contexts give every row's
+// images): each (row, forward segment j <= r) pair with candidate neighbours joins the group
+// of (pattern, j, segment vertex), found in an open-addressing table of `slots` 64-bit keys
+// (a power of two >= 2 x pairs).  A group's context (gctx, ordered by the rows that claimed
+// their groups) holds in phi only the vertices that every row of the group has among its
+// images (the others -1): a neighbour that one row excludes as its own vertex is still a
+// child of another row of the group unless every row excludes it.  gwork[g] = candidate
+// neighbours of the group's segment vertex.  Scratch: pairs = R x nseg entries of pslot,
+// pflag, pidx (long long, pairs + 1) and the scan's; hits: 8 ints per pair.  The group count
+// G is left in pidx[pairs].
+hipError_t build_groups(const DevGraph& g, const PatDev* pats, const RowCtx* ctx, long long R,
+                        int nseg, unsigned long long* tkeys, int* trep, int* tcnt, int* tgrp,
+                        long long slots, int* pslot, int* pflag, long long* pidx,
+                        long long* scan_tmp, long long scan_tmp_elems, int* hits, RowCtx* gctx,
+                        int* gwork, int* guard_err, hipStream_t s);
 hipError_t scan_exclusive(const int* in, long long* out, long long n, long long* tmp,
                           long long tmp_elems, hipStream_t s);
 long long scan_tmp_elems(long long n);
@@ -273,13 +297,20 @@ inline int items_pm(int m) { return m <= 1 ? 1 : m == 2 ? 2 : GS_MAXE - 1; }
 inline int items_pm_class(int m) { return m <= 1 ? 0 : m == 2 ? 1 : 2; }
 #define GS_PM_CLASSES 3
 
-// Modes of the extension kernel: count children per key; fill the MNI image sets of the
-// children (and count them, in the single-pass last level) without writing rows; fill
-// child rows and sets.
+// Modes of the extension kernel.  COUNT and SETS take one work item per backward probe and
+// per forward *run* of a parent row (a run's children share a key, so a run adds its length
+// less the row's own vertices among its neighbours): COUNT counts children per key, SETS
+// (the last level, no rows) also inserts each parent row's images into the sets of the
+// keys it has children in (and counts, in the single-pass last level).  ROWS takes one
+// item per candidate neighbour and writes child rows and every image (levels below the
+// last).  NEWV takes one item per neighbour of a *group* -- the rows of one pattern whose
+// forward segment j starts at the same vertex -- and inserts the new vertex's image once
+// per group instead of once per row (the last level).
 #define GS_ITEMS_COUNT 0
 #define GS_ITEMS_SETS 1
 #define GS_ITEMS_ROWS 2
-#define GS_ITEMS_MODES 3
+#define GS_ITEMS_NEWV 3
+#define GS_ITEMS_MODES 4
 
 // Tile queues of the item kernels: two sets of GS_TILE_QUEUE ints (one counter per XCD,
 // GS_XCD_STRIDE apart) used in turn.  A launch takes set `cur` and zeroes the other one for
@@ -308,6 +339,12 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
                       int nlive, bool saturate, const ItemGuard& gd, hipStream_t s);
 // resident workgroups per CU of the extension kernel in a mode (GS_ITEMS_*) at a tile size
+// NEWV over G groups (contexts and scanned work from build_groups; trailing groups of no
+// work are allowed)
+hipError_t items_newv(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                      const RowCtx* gctx, const long long* gwoff, long long G, KeySpace ks,
+                      const int* key_slot, const SetDesc* sets, uint32_t* pool, TileQueues& tq,
+                      const ItemGuard& gd, hipStream_t s);
 int items_blocks_per_cu(int mode, int tile, int m);
 
 // Keys with a non-zero count, in ascending key order, with their counts (flags / pos: K

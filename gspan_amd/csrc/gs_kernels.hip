@@ -16,6 +16,7 @@
 // locate their rows by a search over the scanned offsets staged in LDS.  Output slots
 // come from one atomic per (tile, distinct key): keys are aggregated per tile in LDS.
 #include <climits>
+#include <type_traits>
 
 #include "gs_internal.h"
 
@@ -202,6 +203,7 @@ struct ItemOut {
     int eid;
     int newv;  // label-class rank of the new vertex's image (forward; NEWX) or -1
     int nv;    // vertices of the parent pattern
+    int w;     // children the item stands for (1, or a run's children)
     long long pbase;  // parent row in its level = pbase + (row in the batch)
 };
 
@@ -209,62 +211,76 @@ __device__ __forceinline__ bool in_range(long long x, long long n) {
     return (unsigned long long)x < (unsigned long long)n;
 }
 
-// One work item of a parent row: item q < r-1 is the backward probe of rmpath index
-// q+1 (gspan.cu:428-445); the rest walk the neighbours of u_n (first forward,
-// gspan.cu:447-462) and of every rightmost-path source (other forward, gspan.cu:464-483).
-// c is the row's context in registers.  Vertex ids and CSR entries are checked against
-// the graph before they are used as indices (GS_GUARD_GRAPH).  NEWX: a forward child also
-// returns the label-class rank of its new vertex (for its MNI set).
-template <bool UNI, bool NEWX, int PM>
-__device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P, const RowRegs& c,
+__device__ __forceinline__ ItemOut no_item(const PatDev& P) {
+    return ItemOut{false, 0, 0, -1, P.nv, 1, P.row_begin - P.brow_begin};
+}
+
+// Backward probe q < r-1 of a parent row: rmpath index q+1 (gspan.cu:428-445).  c is the
+// row's context in registers.  Vertex ids are checked against the graph before they are
+// used as indices (GS_GUARD_GRAPH).
+template <bool UNI, int PM>
+__device__ __forceinline__ ItemOut eval_back(const DevGraph& g, const PatDev& P, const RowRegs& c,
                                              int q, KeySpace ks, int* gerr) {
     constexpr int NV = PM + 1;  // parent vertices; rightmost path <= PM edges
-    ItemOut o{false, 0, 0, -1, P.nv, P.row_begin - P.brow_begin};
-    const int r = P.r;
-    if (q < r - 1) {
-        const int i = q + 1;
-        const int u_n = pickv<NV>(c.phi, P.v_n);
-        const int target = pickv<NV>(c.phi, pget<UNI, PM>(P.bw_v, i));
-        const int e_el = pget<UNI, PM>(P.bw_el, i);
-        const bool le = pget<UNI, PM>(P.bw_le, i) != 0;
-        if (!in_range(u_n, g.n)) {
-            atomicOr(gerr, GS_GUARD_GRAPH);
-            return o;
-        }
-        int lo = g.offsets[u_n];  // all neighbours of u_n, ordered by (to, label)
-        const int end = g.offsets[u_n + 1];
-        int hi = end;
-        while (lo < hi) {  // first entry with to >= target; adjacency sorted by (to, label)
-            const int mid = (lo + hi) >> 1;
-            if ((int)g.adj[mid].x < target) lo = mid + 1;
-            else hi = mid;
-        }
-        for (int e = lo; e < end; ++e) {
-            const uint2 a = g.adj[e];
-            if ((int)a.x != target) break;
-            const int el = (int)(a.y >> 16);
-            if (row_has_edge<UNI, PM>(P, c, u_n, target, el)) continue;
-            if (e_el < el || (e_el == el && le)) {
-                o.valid = true;
-                o.eid = e;
-                o.key = P.key_base + (i - 1) * ks.EL + el;
-                break;
-            }
-        }
+    ItemOut o = no_item(P);
+    const int i = q + 1;
+    const int u_n = pickv<NV>(c.phi, P.v_n);
+    const int target = pickv<NV>(c.phi, pget<UNI, PM>(P.bw_v, i));
+    const int e_el = pget<UNI, PM>(P.bw_el, i);
+    const bool le = pget<UNI, PM>(P.bw_le, i) != 0;
+    if (!in_range(u_n, g.n)) {
+        atomicOr(gerr, GS_GUARD_GRAPH);
         return o;
     }
-    const int f = q - (r - 1);
-    if (f >= pickv<NV>(c.fcum, r)) return o;  // the empty item of a row with no forward item
-    int j = 0;  // segment: fcum is non-decreasing
+    int lo = g.offsets[u_n];  // all neighbours of u_n, ordered by (to, label)
+    const int end = g.offsets[u_n + 1];
+    int hi = end;
+    while (lo < hi) {  // first entry with to >= target; adjacency sorted by (to, label)
+        const int mid = (lo + hi) >> 1;
+        if ((int)g.adj[mid].x < target) lo = mid + 1;
+        else hi = mid;
+    }
+    for (int e = lo; e < end; ++e) {
+        const uint2 a = g.adj[e];
+        if ((int)a.x != target) break;
+        const int el = (int)(a.y >> 16);
+        if (row_has_edge<UNI, PM>(P, c, u_n, target, el)) continue;
+        if (e_el < el || (e_el == el && le)) {
+            o.valid = true;
+            o.eid = e;
+            o.key = P.key_base + (i - 1) * ks.EL + el;
+            break;
+        }
+    }
+    return o;
+}
+
+// Segment of forward item (or run) f: fcum is non-decreasing.
+template <int PM>
+__device__ __forceinline__ int segment_of(const RowRegs& c, int r, int f) {
+    int j = 0;
 #pragma unroll
     for (int s = 0; s < PM; ++s) j += (s < r && f >= c.fcum[s]) ? 1 : 0;
+    return j;
+}
+
+// Forward item f of a parent row: one neighbour of u_n (first forward, gspan.cu:447-462) or
+// of a rightmost-path source (other forward, gspan.cu:464-483).  NEWX: a child also returns
+// the label-class rank of its new vertex (for its MNI set).
+template <bool UNI, bool NEWX, int PM>
+__device__ __forceinline__ ItemOut eval_fwd(const DevGraph& g, const PatDev& P, const RowRegs& c,
+                                            int f, KeySpace ks, int* gerr) {
+    constexpr int NV = PM + 1;
+    ItemOut o = no_item(P);
+    const int r = P.r;
+    if (f >= pickv<NV>(c.fcum, r)) return o;  // the empty item of a row with no forward item
+    const int j = segment_of<PM>(c, r, f);
     const int el_idx = pickv<NV>(c.base, j) + f;  // in adjl: neighbours by label from min_label
     if (!in_range(el_idx, g.nnz)) {
         atomicOr(gerr, GS_GUARD_GRAPH);
         return o;
     }
     const uint2 a = g.adjl[el_idx];
-    const int e = g.eidl[el_idx];  // the entry's CSR index
     const int to = (int)a.x, el = (int)(a.y >> 16), tl = (int)(a.y & 0xffffu);
     if (tl < P.min_label) return o;
     bool seen = false;
@@ -274,8 +290,60 @@ __device__ __forceinline__ ItemOut eval_item(const DevGraph& g, const PatDev& P,
     const int e_el = pget<UNI, NV>(P.seg_el, j);
     if (!(e_el < el || (e_el == el && pget<UNI, NV>(P.seg_tl, j) <= tl))) return o;
     o.valid = true;
-    o.eid = e;
+    o.eid = g.eidl[el_idx];  // the entry's CSR index
     o.newv = NEWX ? g.adjlx[el_idx] : 0;
+    o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
+    return o;
+}
+
+// Forward run f of a parent row: the neighbours of one segment vertex u with one (label,
+// edge label), all children of one key.  The run passes or fails the label and rule tests
+// as a whole (gspan.cu:455, 475-477); of its entries, those whose neighbour is one of the
+// row's own vertices are not children (gspan.cu:453, 473) -- found by binary search, as a
+// run ascends by neighbour, and only for the row's vertices of the run's label.  o.w = the
+// run's children.
+template <bool UNI, int PM>
+__device__ __forceinline__ ItemOut eval_run(const DevGraph& g, const PatDev& P, const RowRegs& c,
+                                            int f, KeySpace ks, int* gerr) {
+    constexpr int NV = PM + 1;
+    ItemOut o = no_item(P);
+    const int r = P.r;
+    if (f >= pickv<NV>(c.fcum, r)) return o;  // the empty item of a row with no forward run
+    const int j = segment_of<PM>(c, r, f);
+    const int ri = pickv<NV>(c.base, j) + f;
+    if (!in_range(ri, g.nruns)) {
+        atomicOr(gerr, GS_GUARD_GRAPH);
+        return o;
+    }
+    const int rk = g.rkey[ri];
+    const int tl = rk >> 16, el = rk & 0xffff;
+    if (tl < P.min_label) return o;
+    const int e_el = pget<UNI, NV>(P.seg_el, j);
+    if (!(e_el < el || (e_el == el && pget<UNI, NV>(P.seg_tl, j) <= tl))) return o;
+    const int st = g.rstart[ri], en = g.rstart[ri + 1];
+    if (st < 0 || en > g.nnz || st >= en) {
+        atomicOr(gerr, GS_GUARD_GRAPH);
+        return o;
+    }
+    int excl = 0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        if (v >= P.nv || pget<UNI, NV>(P.vlab, v) != tl) continue;
+        const int x = c.phi[v];
+        int lo = st, hi = en;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)g.adjl[mid].x < x) lo = mid + 1;
+            else hi = mid;
+        }
+        while (lo < en && (int)g.adjl[lo].x == x) {
+            ++excl;
+            ++lo;
+        }
+    }
+    if (en - st - excl <= 0) return o;
+    o.valid = true;
+    o.w = en - st - excl;
     o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
     return o;
 }
@@ -395,7 +463,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_seed(DevGraph g, int EL, int VL, i
 // the row's items without the label filter.
 __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
                                                        const PatDev* __restrict__ pats, int npats,
-                                                       long long b, const LayerPtrs& L,
+                                                       long long b, const LayerPtrs& L, bool runs,
                                                        RowCtx* __restrict__ out_ctx,
                                                        int* __restrict__ out_work, int* gerr) {
     int lo = 0, hi = npats - 1;
@@ -457,8 +525,15 @@ __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
                     else hi = mid;
                 }
             }
-            bj = first - fc;
-            fc += o1 - first;
+            if (runs) {  // the runs from `first` on: runid ascends along adjl
+                const int rf = first < o1 ? g.runid[first] : 0;
+                const int rc = first < o1 ? g.runid[o1 - 1] - rf + 1 : 0;
+                bj = rf - fc;
+                fc += rc;
+            } else {
+                bj = first - fc;
+                fc += o1 - first;
+            }
             deg += o1 - o0;
         }
         base[j] = bj;
@@ -481,13 +556,14 @@ __device__ __forceinline__ unsigned long long prep_row(const DevGraph& g,
 // k_sum_partials (atomics on one counter from every block would serialise)
 __global__ __launch_bounds__(GS_BLOCK) void k_prep(DevGraph g, const PatDev* __restrict__ pats,
                                                    int npats, long long b0, long long R, LayerPtrs L,
-                                                   RowCtx* __restrict__ ctx, int* __restrict__ work,
+                                                   int runs, RowCtx* __restrict__ ctx,
+                                                   int* __restrict__ work,
                                                    unsigned long long* __restrict__ partial,
                                                    int* gerr) {
     __shared__ unsigned long long s_w[GS_BLOCK / 64];
     const long long lb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long full = 0;
-    if (lb < R) full = prep_row(g, pats, npats, b0 + lb, L, ctx + lb, work + lb, gerr);
+    if (lb < R) full = prep_row(g, pats, npats, b0 + lb, L, runs != 0, ctx + lb, work + lb, gerr);
     if (partial) {
         for (int off = 32; off > 0; off >>= 1) full += __shfl_xor(full, off);
         if (lane_id() == 0) s_w[threadIdx.x / 64] = full;
@@ -652,6 +728,15 @@ __device__ __forceinline__ void count_add(int* count, int c) {
     if (old < 0 || old > INT_MAX - c) atomicExch(count, INT_MIN);
 }
 
+// Adds a workgroup's count c (64-bit in the run modes) to a child count: a c past INT_MAX
+// saturates the count at once; otherwise as count_add (saturate) or a plain add.
+template <class CNT>
+__device__ __forceinline__ void count_flush(int* count, CNT c, bool saturate) {
+    if ((unsigned long long)c > (unsigned long long)INT_MAX) atomicExch(count, INT_MIN);
+    else if (saturate) count_add(count, (int)c);
+    else atomicAdd(count, (int)c);
+}
+
 // First row whose scanned offset is <= item, found by wave 0 with 64-way probes of woff
 // (four rounds of one load per lane for 16M rows).  woff is non-decreasing and
 // woff[0] = 0 <= item, so the probes that pass form a prefix of the lanes.
@@ -687,25 +772,28 @@ __device__ __forceinline__ int wave_find_lds(const int* off, int n, int item) {
     return lo;
 }
 
-// Persistent extension kernel, in one of three modes (GS_ITEMS_COUNT / _SETS / _ROWS).
-// Items of all parent rows are flattened by the scan of per-row work (woff); workgroups
-// take TILE-item tiles from per-XCD queues.  Per tile:
+// Persistent extension kernel in one of four modes (GS_ITEMS_*).  Items of all parent rows
+// (or groups, NEWV) are flattened by the scan of per-row work (woff); workgroups take
+// TILE-item tiles from per-XCD queues.  Per tile:
 //   phase 1  every thread evaluates its items (row by a search over the tile's staged row
-//            offsets, row context, adjacency entry), aggregates their child keys in an LDS
-//            table and -- in the fill modes -- inserts the children's vertex images into
-//            their MNI sets right away (the set of a key is known once the key is);
-//            meanwhile wave 0 claims the next tile and finds its first row
-//   phase 2  one output reservation per distinct key (rows mode); child counts go to the
+//            offsets, row context, run or adjacency entry), aggregates their child keys in
+//            an LDS table and inserts the children's vertex images into their MNI sets
+//            right away (the set of a key is known once the key is): the parent's images
+//            (SETS, ROWS), the new vertex's image (ROWS, NEWV); meanwhile wave 0 claims
+//            the next tile and finds its first row
+//   phase 2  one output reservation per distinct key (ROWS); child counts go to the
 //            workgroup's persistent count table; the next tile's row offsets are loaded
-//   phase 3  (rows mode) child rows at their reserved positions
+//   phase 3  (ROWS) child rows at their reserved positions
 // So a tile costs three or four barriers, and the claim, row search and offset loads of
-// the next tile overlap the work of this one.
+// the next tile overlap the work of this one.  NEWV counts nothing: phase 2 only stages.
 //
 // Child counts (key_count) are summed per workgroup in a second LDS table that lives
 // across the tiles the workgroup takes, and added to global memory when it is more than
-// half full, after 2^14 tiles (so no LDS count passes 2^24) and when the workgroup ends:
-// the counters of a level's hot keys see one atomic per workgroup instead of one per tile
-// (device-scope atomics are performed at the memory side and serialise on one address).
+// half full, after 2^14 tiles and when the workgroup ends: the counters of a level's hot
+// keys see one atomic per workgroup instead of one per tile (device-scope atomics are
+// performed at the memory side and serialise on one address).  The run modes count in
+// 64 bits (a run adds up to a vertex degree); ROWS adds at most TILE per tile, so its
+// 32-bit LDS counts stay below 2^14 * TILE <= 2^24.
 template <int MODE, int TILE, int PM>
 __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __restrict__ pats,
                                                     const RowCtx* __restrict__ ctx,
@@ -719,11 +807,18 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                                                     uint32_t* pool, int* overflow,
                                                     int* tile_counters, int* next_counters,
                                                     ItemGuard gd, int dedup, int saturate) {
-    constexpr bool FILL = MODE != GS_ITEMS_COUNT;
+    constexpr bool RUNS = MODE == GS_ITEMS_COUNT || MODE == GS_ITEMS_SETS;
+    constexpr bool FILL = MODE != GS_ITEMS_COUNT;  // inserts into MNI sets
     constexpr bool ROWS = MODE == GS_ITEMS_ROWS;
+    constexpr bool NEWV = MODE == GS_ITEMS_NEWV;
+    constexpr bool PARENT = MODE == GS_ITEMS_SETS || ROWS;  // inserts the parent's images
+    constexpr bool NEWX = ROWS || NEWV;                      // inserts the new vertex's image
+    constexpr bool AGG = !NEWV;                              // aggregates keys per tile
+    using cnt_t = typename std::conditional<RUNS, unsigned long long, int>::type;
     constexpr int CV = PM + 2;  // vertices of a child (sets of a child key)
     static_assert(PM >= 1 && PM < GS_MAXE, "parent edges");
     constexpr int LDS_SLOTS = 2 * TILE;
+    constexpr int AGG_SLOTS = AGG ? LDS_SLOTS : 1;
     constexpr int IPT = TILE / GS_BLOCK;
     constexpr int STAGE = (TILE + 1 + GS_BLOCK - 1) / GS_BLOCK;  // offsets per thread
     static_assert(TILE % GS_BLOCK == 0 && (TILE & (TILE - 1)) == 0, "tile: power of two");
@@ -733,31 +828,33 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     // scanned offsets of the staged tiles' rows relative to the tile's first item, two
     // buffers: the current tile and the next (row 0 may start before the tile: <= 0)
     __shared__ int s_off[2][TILE + 1];
-    __shared__ int s_key[LDS_SLOTS];
-    __shared__ int s_cnt[LDS_SLOTS];
+    __shared__ int s_key[AGG_SLOTS];
+    __shared__ cnt_t s_cnt[AGG_SLOTS];
     __shared__ int s_base[ROWS ? LDS_SLOTS : 1];
-    // (live slot, row in tile) pairs already seen: only their first item inserts the
-    // parent's vertex images, which are the same for every item of the pair
-    __shared__ int s_pair[FILL ? LDS_SLOTS : 1];
+    // (live slot, row in tile) pairs already seen (ROWS): only their first item inserts the
+    // parent's vertex images, which are the same for every item of the pair (a run item is
+    // the only one of its (key, row) pair)
+    __shared__ int s_pair[ROWS ? LDS_SLOTS : 1];
     // persistent count table: LDS_SLOTS slots, flushed once more than TILE are used, so a
     // tile (at most TILE new keys) always finds room and probing ends
-    __shared__ int s_pkey[LDS_SLOTS];
-    __shared__ int s_pcnt[LDS_SLOTS];
+    __shared__ int s_pkey[AGG_SLOTS];
+    __shared__ cnt_t s_pcnt[AGG_SLOTS];
     // s_pfill: used slots of the persistent table (low 16 bits, at most LDS_SLOTS) and
     // tiles since its last flush (high bits)
     __shared__ int s_pfill;
     __shared__ long long s_tile[2], s_rlo[2];
     __shared__ int s_nrows[2];
-    // every tile adds at most TILE to a count, so no LDS count passes 2^14 * TILE <= 2^24
     constexpr int FLUSH_TILES = GS_FLUSH_TILES;
     static_assert(FLUSH_TILES >= 1 && FLUSH_TILES <= (1 << 14), "tile bits of s_pfill");
     if (R <= 0) return;
-    for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
-        s_key[h] = -1;
-        s_cnt[h] = 0;
-        if (FILL) s_pair[h] = -1;
-        s_pkey[h] = -1;
-        s_pcnt[h] = 0;
+    if (AGG) {
+        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+            s_key[h] = -1;
+            s_cnt[h] = 0;
+            if (ROWS) s_pair[h] = -1;
+            s_pkey[h] = -1;
+            s_pcnt[h] = 0;
+        }
     }
     if (threadIdx.x == 0) s_pfill = 0;
     const long long W = woff[R];
@@ -844,7 +941,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             const int it = q * GS_BLOCK + threadIdx.x;  // item offset in the tile
             hs[q] = -1;
             rank[q] = 0;
-            ItemOut o{false, 0, 0, -1, 0, 0};
+            ItemOut o{false, 0, 0, -1, 0, 1, 0};
             RowRegs c;
             int p = -1, f = 0, row = 0;
             if (it < tn) {
@@ -862,13 +959,23 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             // rows of one pattern are contiguous, so a wave's items mostly share it: then
             // the pattern's tables are read once per wave with scalar loads
             const int p0 = __builtin_amdgcn_readfirstlane(p);
+            auto eval = [&](const PatDev& P, auto uni) -> ItemOut {
+                constexpr bool U = decltype(uni)::value;
+                if constexpr (NEWV) {
+                    return eval_fwd<U, true, PM>(g, P, c, f, ks, gd.err);
+                } else {
+                    if (f < P.r - 1) return eval_back<U, PM>(g, P, c, f, ks, gd.err);
+                    if constexpr (RUNS) return eval_run<U, PM>(g, P, c, f - (P.r - 1), ks, gd.err);
+                    else return eval_fwd<U, NEWX, PM>(g, P, c, f - (P.r - 1), ks, gd.err);
+                }
+            };
             if (__all(p == p0 || p < 0)) {
-                if (it < tn) o = eval_item<true, FILL, PM>(g, pats[p0], c, f, ks, gd.err);
+                if (it < tn) o = eval(pats[p0], std::true_type{});
             } else if (it < tn) {
-                o = eval_item<false, FILL, PM>(g, pats[p], c, f, ks, gd.err);
+                o = eval(pats[p], std::false_type{});
             }
-            // aggregation key: child key (count, single pass) or live slot (fill); a child
-            // that is not written (not canonical) has no slot
+            // aggregation key: child key (counts) or live slot (fills); a child that is not
+            // written (not canonical) has no slot
             bool act = false;
             int agg = -1;
             if (o.valid) {
@@ -890,54 +997,64 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     act = false;
                 }
             }
-            // Wave-level aggregation: consecutive items mostly share a key (a row's
-            // neighbours are ordered by (label, edge label)), so the lanes of one key take
-            // their tile counts with one LDS atomic by their first lane; keys left after a
-            // few rounds take one atomic per lane.  lead: this lane inserts the parent's
-            // images for its (slot, row) pair -- the first item of the pair in the tile.
-            bool lead = true;
-            unsigned long long pend = __ballot(act);
-            for (int round = 0; round < 4 && pend; ++round) {
-                // the leader is wave-uniform: values move by readlane, not by LDS shuffles
-                const int leader = __ffsll((long long)pend) - 1;
-                const int lkey = __builtin_amdgcn_readlane(agg, leader);
-                const int lrow = __builtin_amdgcn_readlane(row, leader);
-                const bool mine = act && agg == lkey;
-                const unsigned long long m = __ballot(mine);
-                int h = 0, base = 0, first = 0;
-                if (lane == leader) {
-                    h = lds_slot<LDS_SLOTS>(s_key, lkey);
-                    base = atomicAdd(&s_cnt[h], (int)__popcll(m));
-                    // a pair is named by the key's slot in this tile's table and the row:
-                    // below 2 * TILE * TILE, so distinct pairs never share a name
-                    if (FILL) first = (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + lrow));
-                }
-                h = __builtin_amdgcn_readlane(h, leader);
-                base = __builtin_amdgcn_readlane(base, leader);
-                if (FILL) first = __builtin_amdgcn_readlane(first, leader);
-                if (mine) {
-                    hs[q] = h;
-                    rank[q] = base + (int)mbcnt64(m);
-                    if (FILL) {
-                        lead = row == lrow ? (lane == leader && first != 0)
-                                           : (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + row));
+            // lead: this item inserts the parent's images for its (slot, row) pair;
+            // ins: this item inserts images at all
+            bool lead = true, ins = false;
+            if (ROWS) {
+                // Wave-level aggregation: consecutive items mostly share a key (a row's
+                // neighbours are ordered by (label, edge label)), so the lanes of one key take
+                // their tile counts with one LDS atomic by their first lane; keys left after a
+                // few rounds take one atomic per lane.  lead: the pair's first item in the tile.
+                unsigned long long pend = __ballot(act);
+                for (int round = 0; round < 4 && pend; ++round) {
+                    // the leader is wave-uniform: values move by readlane, not LDS shuffles
+                    const int leader = __ffsll((long long)pend) - 1;
+                    const int lkey = __builtin_amdgcn_readlane(agg, leader);
+                    const int lrow = __builtin_amdgcn_readlane(row, leader);
+                    const bool mine = act && agg == lkey;
+                    const unsigned long long m = __ballot(mine);
+                    int h = 0, base = 0, first = 0;
+                    if (lane == leader) {
+                        h = lds_slot<LDS_SLOTS>(s_key, lkey);
+                        base = atomicAdd(&s_cnt[h], (int)__popcll(m));
+                        // a pair is named by the key's slot in this tile's table and the row:
+                        // below 2 * TILE * TILE, so distinct pairs never share a name
+                        first = !dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + lrow);
                     }
-                    act = false;
+                    h = __builtin_amdgcn_readlane(h, leader);
+                    base = __builtin_amdgcn_readlane(base, leader);
+                    first = __builtin_amdgcn_readlane(first, leader);
+                    if (mine) {
+                        hs[q] = h;
+                        rank[q] = base + (int)mbcnt64(m);
+                        lead = row == lrow
+                                   ? (lane == leader && first != 0)
+                                   : (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + row));
+                        act = false;
+                    }
+                    pend &= ~m;
                 }
-                pend &= ~m;
+                if (act) {
+                    hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
+                    rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
+                    lead = !dedup || lds_first<LDS_SLOTS>(s_pair, hs[q] * TILE + row);
+                }
+                ins = hs[q] >= 0;
+            } else if (act) {
+                // run modes: each item is its row's only one of its key; weighted counts
+                ins = true;
+                if (RUNS && key_count) {
+                    hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
+                    atomicAdd(&s_cnt[hs[q]], (cnt_t)o.w);
+                }
             }
-            if (act) {
-                hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
-                rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
-                if (FILL) lead = !dedup || lds_first<LDS_SLOTS>(s_pair, hs[q] * TILE + row);
-            }
-            if (FILL && hs[q] >= 0) {
+            if (FILL && ins) {
                 if (ROWS) {
                     eid[q] = o.eid;
                     prow[q] = (int)(o.pbase + b0 + rlo + row);
                 }
                 const int nv = o.nv;
-                if (lead) {
+                if (PARENT && lead) {
 #pragma unroll
                     for (int v = 0; v < PM + 1; ++v) {
                         if (v >= nv) continue;
@@ -947,7 +1064,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                         else atomicOr(gd.err, GS_GUARD_GRAPH);
                     }
                 }
-                if (o.newv >= 0) set_insert<CV>(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
+                if (NEWX && o.newv >= 0)
+                    set_insert<CV>(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
             }
         }
         // wave 0: the next tile's first row -- from this tile's offsets when it follows
@@ -971,35 +1089,37 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             const int i = threadIdx.x + k * GS_BLOCK;
             stage[k] = (ntl >= 0 && i <= nnr) ? woff[nrlo + i] : 0;
         }
-        if (key_count && threadIdx.x == 0) atomicAdd(&s_pfill, 1 << 16);  // a tile counted
-        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
-            const int key = s_key[h];
-            if (key < 0) continue;
-            if (ROWS) {
-                // 64-bit cursors cannot wrap however far a key overruns a predicted
-                // capacity; a reservation that does not fit writes nothing
-                const unsigned long long cnt = (unsigned long long)s_cnt[h];
-                const unsigned long long at = atomicAdd(&cursor[key], cnt);
-                const unsigned long long cap = cap_end ? (unsigned long long)cap_end[key]
-                                                       : (unsigned long long)gd.rows_cap;
-                const bool fits = at + cnt <= cap && at + cnt <= (unsigned long long)gd.rows_cap;
-                s_base[h] = fits ? (int)at : -1;
-                if (!fits) atomicOr(cap_end ? overflow : gd.err, cap_end ? 1 : GS_GUARD_ROW);
-            }
-            if (key_count) {
-                const int ph = lds_slot_fill<LDS_SLOTS>(s_pkey, key, &s_pfill);
-                atomicAdd(&s_pcnt[ph], s_cnt[h]);
+        if (AGG) {
+            if (key_count && threadIdx.x == 0) atomicAdd(&s_pfill, 1 << 16);  // a tile counted
+            for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+                const int key = s_key[h];
+                if (key < 0) continue;
+                if (ROWS) {
+                    // 64-bit cursors cannot wrap however far a key overruns a predicted
+                    // capacity; a reservation that does not fit writes nothing
+                    const unsigned long long cnt = (unsigned long long)s_cnt[h];
+                    const unsigned long long at = atomicAdd(&cursor[key], cnt);
+                    const unsigned long long cap = cap_end ? (unsigned long long)cap_end[key]
+                                                           : (unsigned long long)gd.rows_cap;
+                    const bool fits =
+                        at + cnt <= cap && at + cnt <= (unsigned long long)gd.rows_cap;
+                    s_base[h] = fits ? (int)at : -1;
+                    if (!fits) atomicOr(cap_end ? overflow : gd.err, cap_end ? 1 : GS_GUARD_ROW);
+                }
+                if (key_count) {
+                    const int ph = lds_slot_fill<LDS_SLOTS>(s_pkey, key, &s_pfill);
+                    atomicAdd(&s_pcnt[ph], s_cnt[h]);
+                }
             }
         }
         __syncthreads();
         // Flush the persistent table once more than TILE slots are used (a tile adds at
         // most TILE keys, so the next tile still finds room) or after FLUSH_TILES tiles.
         // Uniform: s_pfill was last written before the barrier above.
-        if (key_count && ((s_pfill & 0xffff) > TILE || (s_pfill >> 16) >= FLUSH_TILES)) {
+        if (AGG && key_count && ((s_pfill & 0xffff) > TILE || (s_pfill >> 16) >= FLUSH_TILES)) {
             for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
                 if (s_pkey[h] >= 0) {
-                    if (saturate) count_add(&key_count[s_pkey[h]], s_pcnt[h]);
-                    else atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
+                    count_flush(&key_count[s_pkey[h]], s_pcnt[h], saturate != 0);
                     s_pkey[h] = -1;
                     s_pcnt[h] = 0;
                 }
@@ -1017,12 +1137,14 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             }
             __syncthreads();
         }
-        for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
-            if (s_key[h] >= 0) {
-                s_key[h] = -1;
-                s_cnt[h] = 0;
+        if (AGG) {
+            for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
+                if (s_key[h] >= 0) {
+                    s_key[h] = -1;
+                    s_cnt[h] = 0;
+                }
+                if (ROWS) s_pair[h] = -1;
             }
-            if (FILL) s_pair[h] = -1;
         }
 #pragma unroll
         for (int k = 0; k < STAGE; ++k) {
@@ -1033,12 +1155,182 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
         buf = nb;
     }
     // the loop ends after a barrier; the persistent table was last written before it
-    if (key_count) {
+    if (AGG && key_count) {
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
             if (s_pkey[h] < 0) continue;
-            if (saturate) count_add(&key_count[s_pkey[h]], s_pcnt[h]);
-            else atomicAdd(&key_count[s_pkey[h]], s_pcnt[h]);
+            count_flush(&key_count[s_pkey[h]], s_pcnt[h], saturate != 0);
         }
+    }
+}
+
+// ------------------------------------------------------------------ groups (NEWV)
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+// First candidate neighbour (label >= min_label) of segment vertex u and their count.
+__device__ __forceinline__ void seg_candidates(const DevGraph& g, int u, int min_label,
+                                               int* first, int* cnt) {
+    const int o0 = g.offsets[u], o1 = g.offsets[u + 1];
+    int f;
+    if (g.lstart) {
+        f = g.lstart[(long long)u * g.lstride + min_label];
+    } else {
+        f = o0;
+        int hi = o1;
+        while (f < hi) {
+            const int mid = (f + hi) >> 1;
+            if ((int)(g.adjl[mid].y & 0xffffu) < min_label) f = mid + 1;
+            else hi = mid;
+        }
+    }
+    *first = f;
+    *cnt = o1 - f;
+}
+
+// The pair (row b, segment j) of a chunk: its pattern, segment vertex and candidates.
+__device__ __forceinline__ bool pair_of(const DevGraph& g, const PatDev* __restrict__ pats,
+                                        const RowCtx* __restrict__ ctx, long long b, int j,
+                                        int* p, int* u, int* first, int* cnt, int* gerr) {
+    const RowCtx& c = ctx[b];
+    *p = c.p;
+    const PatDev& P = pats[*p];
+    if (j > P.r) return false;
+    *u = c.phi[P.seg_v[j]];
+    if (!in_range(*u, g.n)) {
+        atomicOr(gerr, GS_GUARD_GRAPH);
+        return false;
+    }
+    seg_candidates(g, *u, P.min_label, first, cnt);
+    return *cnt > 0;
+}
+
+__device__ __forceinline__ unsigned long long group_key(int p, int j, int u) {
+    return ((unsigned long long)((unsigned)p * 8u + (unsigned)j) << 32) | (unsigned)u;
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    return x;
+}
+
+// pslot[pair] = table slot of the pair's group (-1: no candidates); tcnt = rows per group;
+// trep = the row that claimed the slot.  The table has at least twice as many slots as
+// pairs, so probing ends.
+__global__ void k_group_claim(DevGraph g, const PatDev* __restrict__ pats,
+                              const RowCtx* __restrict__ ctx, long long R, int nseg,
+                              unsigned long long* tkeys, int* trep, int* tcnt, long long slots,
+                              int* pslot, int* gerr) {
+    const long long npairs = R * nseg;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long pi = (long long)blockIdx.x * blockDim.x + threadIdx.x; pi < npairs;
+         pi += stride) {
+        const long long b = pi / nseg;
+        const int j = (int)(pi % nseg);
+        int p, u, first, cnt;
+        int slot = -1;
+        if (pair_of(g, pats, ctx, b, j, &p, &u, &first, &cnt, gerr)) {
+            const unsigned long long key = group_key(p, j, u);
+            const unsigned long long mask = (unsigned long long)slots - 1;
+            unsigned long long h = mix64(key) & mask;
+            for (long long probe = 0; probe < slots; ++probe) {
+                const unsigned long long prev = atomicCAS(&tkeys[h], kEmptyKey, key);
+                if (prev == kEmptyKey) {
+                    trep[h] = (int)b;
+                    slot = (int)h;
+                    break;
+                }
+                if (prev == key) {
+                    slot = (int)h;
+                    break;
+                }
+                h = (h + 1) & mask;
+            }
+            if (slot >= 0) atomicAdd(&tcnt[slot], 1);
+            else atomicOr(gerr, GS_GUARD_SLOT);
+        }
+        pslot[pi] = slot;
+    }
+}
+
+// pflag[pair] = 1 for the pair that claimed its group's slot
+__global__ void k_group_flags(long long npairs, int nseg, const int* __restrict__ pslot,
+                              const int* __restrict__ trep, int* pflag) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long pi = (long long)blockIdx.x * blockDim.x + threadIdx.x; pi < npairs;
+         pi += stride) {
+        const int h = pslot[pi];
+        pflag[pi] = (h >= 0 && trep[h] == (int)(pi / nseg)) ? 1 : 0;
+    }
+}
+
+// tgrp[slot] = group index (the claiming pair's rank among claiming pairs)
+__global__ void k_group_index(long long npairs, const int* __restrict__ pslot,
+                              const int* __restrict__ pflag, const long long* __restrict__ pidx,
+                              int* tgrp) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long pi = (long long)blockIdx.x * blockDim.x + threadIdx.x; pi < npairs;
+         pi += stride)
+        if (pflag[pi]) tgrp[pslot[pi]] = (int)pidx[pi];
+}
+
+// hits[8 g + v]: rows of group g, other than its claiming row, that have the claiming row's
+// image of pattern vertex v among their own images
+__global__ void k_group_hits(const PatDev* __restrict__ pats, const RowCtx* __restrict__ ctx,
+                             long long npairs, int nseg, const int* __restrict__ pslot,
+                             const int* __restrict__ pflag, const int* __restrict__ trep,
+                             const int* __restrict__ tgrp, int* hits) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long pi = (long long)blockIdx.x * blockDim.x + threadIdx.x; pi < npairs;
+         pi += stride) {
+        const int h = pslot[pi];
+        if (h < 0 || pflag[pi]) continue;
+        const RowCtx& me = ctx[pi / nseg];
+        const RowCtx& rep = ctx[trep[h]];
+        const int nv = pats[me.p].nv;
+        const int gi = tgrp[h];
+        for (int v = 0; v < GS_MAXV; ++v) {
+            if (v >= nv) break;
+            const int w = rep.phi[v];
+            bool in = false;
+            for (int x = 0; x < GS_MAXV; ++x) in |= x < nv && me.phi[x] == w;
+            if (in) atomicAdd(&hits[(long long)gi * GS_MAXV + v], 1);
+        }
+    }
+}
+
+// The context and candidate count of every group, from its claiming pair.
+__global__ void k_group_ctx(DevGraph g, const PatDev* __restrict__ pats,
+                            const RowCtx* __restrict__ ctx, long long npairs, int nseg,
+                            const int* __restrict__ pslot, const int* __restrict__ pflag,
+                            const long long* __restrict__ pidx, const int* __restrict__ tcnt,
+                            const int* __restrict__ hits, RowCtx* gctx, int* gwork, int* gerr) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long pi = (long long)blockIdx.x * blockDim.x + threadIdx.x; pi < npairs;
+         pi += stride) {
+        if (!pflag[pi]) continue;
+        const long long b = pi / nseg;
+        const int j = (int)(pi % nseg);
+        const long long gi = pidx[pi];
+        int p, u, first, cnt;
+        if (!pair_of(g, pats, ctx, b, j, &p, &u, &first, &cnt, gerr)) continue;
+        const PatDev& P = pats[p];
+        const RowCtx& rep = ctx[b];
+        const int others = tcnt[pslot[pi]] - 1;
+        RowCtx o;
+        for (int v = 0; v < GS_MAXV; ++v) {
+            const bool common = v < P.nv && (v == P.seg_v[j] || hits[gi * GS_MAXV + v] == others);
+            o.phi[v] = common ? rep.phi[v] : -1;
+        }
+        for (int s2 = 0; s2 < GS_MAXE; ++s2) {
+            o.fcum[s2] = s2 < j ? 0 : cnt;  // every item falls in segment j
+            o.base[s2] = first;
+        }
+        o.p = p;
+        o.pad = 0;
+        gctx[gi] = o;
+        gwork[gi] = cnt;
     }
 }
 
@@ -1109,6 +1401,11 @@ __global__ void k_scatter_slots(int* key_slot, const int* keys, int nkeys) {
 }
 
 __global__ void k_fill_i32(int* p, int v, long long n) {
+    long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, long long n) {
     long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
@@ -1789,13 +2086,13 @@ hipError_t seed_fill(const DevGraph& g, int EL, int VL, const int* key_slot, int
 }
 
 hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, long long R,
-                LayerPtrs L, RowCtx* ctx, int* work, unsigned long long* partial,
+                LayerPtrs L, bool runs, RowCtx* ctx, int* work, unsigned long long* partial,
                 unsigned long long* work_full, int* guard_err, hipStream_t s) {
     if (R <= 0) return hipSuccess;
     long long grid = (R + GS_BLOCK - 1) / GS_BLOCK;
     if (!grid_fits(grid, GS_BLOCK)) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_prep, dim3((unsigned)grid), dim3(GS_BLOCK), 0, s, g, pats, npats, b0, R,
-                       L, ctx, work, work_full ? partial : nullptr, guard_err);
+                       L, runs ? 1 : 0, ctx, work, work_full ? partial : nullptr, guard_err);
     if (work_full)
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(GS_BLOCK), 0, s, partial, grid,
                            work_full);
@@ -1803,6 +2100,34 @@ hipError_t prep(const DevGraph& g, const PatDev* pats, int npats, long long b0, 
 }
 
 long long scan_tmp_elems(long long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+hipError_t build_groups(const DevGraph& g, const PatDev* pats, const RowCtx* ctx, long long R,
+                        int nseg, unsigned long long* tkeys, int* trep, int* tcnt, int* tgrp,
+                        long long slots, int* pslot, int* pflag, long long* pidx,
+                        long long* scan_tmp, long long scan_tmp_elems, int* hits, RowCtx* gctx,
+                        int* gwork, int* guard_err, hipStream_t s) {
+    const long long npairs = R * nseg;
+    if (npairs <= 0) return hipSuccess;
+    if (nseg < 1 || nseg > GS_MAXE || slots < 2 * npairs || (slots & (slots - 1)) != 0 ||
+        slots > (1LL << 31))
+        return hipErrorInvalidValue;
+    const int gp = grid_for(npairs, 256, 16384), gs = grid_for(slots, 256, 16384);
+    hipLaunchKernelGGL(k_fill_u64, dim3(gs), dim3(256), 0, s, tkeys, kEmptyKey, slots);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gs), dim3(256), 0, s, tcnt, 0, slots);
+    hipLaunchKernelGGL(k_group_claim, dim3(gp), dim3(256), 0, s, g, pats, ctx, R, nseg, tkeys,
+                       trep, tcnt, slots, pslot, guard_err);
+    hipLaunchKernelGGL(k_group_flags, dim3(gp), dim3(256), 0, s, npairs, nseg, pslot, trep, pflag);
+    hipError_t e = scan_exclusive(pflag, pidx, npairs, scan_tmp, scan_tmp_elems, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_group_index, dim3(gp), dim3(256), 0, s, npairs, pslot, pflag, pidx, tgrp);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(npairs * GS_MAXV, 256, 16384)), dim3(256), 0, s,
+                       hits, 0, npairs * GS_MAXV);
+    hipLaunchKernelGGL(k_group_hits, dim3(gp), dim3(256), 0, s, pats, ctx, npairs, nseg, pslot,
+                       pflag, trep, tgrp, hits);
+    hipLaunchKernelGGL(k_group_ctx, dim3(gp), dim3(256), 0, s, g, pats, ctx, npairs, nseg, pslot,
+                       pflag, pidx, tcnt, hits, gctx, gwork, guard_err);
+    return hipGetLastError();
+}
 
 hipError_t scan_exclusive(const int* in, long long* out, long long n, long long* tmp,
                           long long tmp_elems, hipStream_t s) {
@@ -1875,7 +2200,7 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                                            gd, dedup, saturate ? 1 : 0, s);
     return launch_items<GS_ITEMS_SETS>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot,
                                        cursor, cap_end, nullptr, sets, pool, overflow, tq, gd,
-                                       dedup, saturate ? 1 : 0, s);
+                                       0, saturate ? 1 : 0, s);
 }
 
 template <int MODE, int TILE>
@@ -1900,13 +2225,24 @@ hipError_t occupancy_tiles(int tile, int pm, int* per_cu) {
     }
 }
 
+hipError_t items_newv(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
+                      const RowCtx* gctx, const long long* gwoff, long long G, KeySpace ks,
+                      const int* key_slot, const SetDesc* sets, uint32_t* pool, TileQueues& tq,
+                      const ItemGuard& gd, hipStream_t s) {
+    if (G <= 0) return hipSuccess;
+    return launch_items<GS_ITEMS_NEWV>(lc, g, pats, gctx, gwoff, 0, G, ks, nullptr, key_slot,
+                                       nullptr, nullptr, nullptr, sets, pool, nullptr, tq, gd, 0,
+                                       0, s);
+}
+
 int items_blocks_per_cu(int mode, int tile, int m) {
     int per_cu = 0;
     const int pm = items_pm(m);
     hipError_t e;
     if (mode == GS_ITEMS_COUNT) e = occupancy_tiles<GS_ITEMS_COUNT>(tile, pm, &per_cu);
     else if (mode == GS_ITEMS_SETS) e = occupancy_tiles<GS_ITEMS_SETS>(tile, pm, &per_cu);
-    else e = occupancy_tiles<GS_ITEMS_ROWS>(tile, pm, &per_cu);
+    else if (mode == GS_ITEMS_ROWS) e = occupancy_tiles<GS_ITEMS_ROWS>(tile, pm, &per_cu);
+    else e = occupancy_tiles<GS_ITEMS_NEWV>(tile, pm, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     return per_cu;
 }

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -294,6 +295,8 @@ struct gs_ctx {
     // vertices of each label rank (lcnt on the host too: set planning)
     DBuf d_lidx, d_adjlx, d_lcnt;
     std::vector<int> lcnt;
+    // runs of equal (neighbour label, edge label) in adjl
+    DBuf d_rstart, d_rkey, d_runid;
     DevGraph dg{};
     // [0] guard word, [1] n_active, [2] err, [3] overflow, [4..5] int64 total,
     // [6..7] items without the label filter, [8..11] two 64-bit step totals
@@ -1006,18 +1009,38 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     long long W_total = 0;
     unsigned long long W_full = 0;
     std::vector<long long> Wc_host(nchunks, 0);
+    // work items of later launches, read at the step's next synchronisation (the tuner's
+    // time per item); a deque keeps the addresses of earlier entries
+    std::deque<long long> launch_items;
+    auto items_at = [&](const long long* d_total) -> const long long* {
+        launch_items.push_back(0);
+        d2h(c, &launch_items.back(), d_total, 1);
+        return &launch_items.back();
+    };
     HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->work_full64()), 0, 2, c->stream));
-    auto prep_chunk = [&](int ci, bool first_pass) -> long long {
+    // Contexts of a chunk's rows, for the run modes (COUNT, SETS: work counts runs) or the
+    // neighbour modes (ROWS); the chunk and mode last prepared are remembered.
+    int prepped_ci = -1;
+    bool prepped_runs = false;
+    auto prep_chunk = [&](int ci, bool first_pass, bool runs) -> long long {
         const long long b0 = (long long)ci * chunk;
         const long long Rc = std::min(chunk, R - b0);
-        HIPCHECK(gsk::prep(c->dg, d_pats.as<PatDev>(), B, b0, Rc, L, d_ctx.as<RowCtx>(),
+        HIPCHECK(gsk::prep(c->dg, d_pats.as<PatDev>(), B, b0, Rc, L, runs, d_ctx.as<RowCtx>(),
                            d_work.as<int>(), d_partial.as<unsigned long long>(),
                            first_pass ? c->work_full64() : nullptr, c->guard_word(),
                            c->stream));
         HIPCHECK(gsk::scan_exclusive(d_work.as<int>(), d_woff.as<long long>(), Rc,
                                      d_tmp.as<long long>(), tmp_elems, c->stream));
+        prepped_ci = ci;
+        prepped_runs = runs;
         return Rc;
     };
+    // the chunk's contexts in a mode, prepared again only when the last preparation differs
+    auto chunk_ready = [&](int ci, bool runs) -> long long {
+        if (prepped_ci == ci && prepped_runs == runs) return std::min(chunk, R - (long long)ci * chunk);
+        return prep_chunk(ci, false, runs);
+    };
+
     // launch shape from the tuner, and events around the launch when they are wanted
     int t_bucket = -1, t_cfg = 0;
     auto shape = [&](int mode, bool single_pass, long long Rc) {
@@ -1042,6 +1065,46 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].second, c->stream));
         c->ev_pending.push_back({(int)c->ev_used, kind, t_bucket, t_cfg, items});
         ++c->ev_used;
+    };
+
+    // Last level, after a SETS launch over a chunk: the new vertices' images, inserted once
+    // per group of rows (pattern, forward segment, segment vertex) by NEWV.  Rows go to the
+    // groups in sub-chunks of at most kGroupPairs (row, segment) pairs: a union of groups
+    // over any split of the rows gives the same sets.
+    int nseg = 1;
+    for (int i = 0; i < B; ++i) nseg = std::max(nseg, pd[i].r + 1);
+    auto newv_pass = [&](long long Rc, const int* key_slot, long long nslots, const SetDesc* sets,
+                         uint32_t* pool, long long pool_words) {
+        constexpr long long kGroupPairs = 1LL << 24;
+        const long long sub = std::max(1LL, kGroupPairs / nseg);
+        const long long maxp = std::min(Rc, sub) * nseg;
+        long long slots = 1;
+        while (slots < 2 * maxp) slots <<= 1;
+        const long long te = gsk::scan_tmp_elems(maxp);
+        DBuf tkeys(P, 8 * slots), trep(P, 4 * slots), tcnt(P, 4 * slots), tgrp(P, 4 * slots);
+        DBuf pslot(P, 4 * maxp), pflag(P, 4 * maxp), pidx(P, 8 * (maxp + 1)), ptmp(P, 8 * te);
+        DBuf hits(P, 4 * GS_MAXV * maxp), gctx(P, sizeof(RowCtx) * maxp), gwork(P, 4 * maxp);
+        DBuf gwoff(P, 8 * (maxp + 1));
+        for (long long r0 = 0; r0 < Rc; r0 += sub) {
+            const long long Rs = std::min(sub, Rc - r0), np = Rs * nseg;
+            HIPCHECK(gsk::fill_i32(gwork.as<int>(), 0, np, c->stream));
+            HIPCHECK(gsk::build_groups(c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>() + r0, Rs,
+                                       nseg, tkeys.as<unsigned long long>(), trep.as<int>(),
+                                       tcnt.as<int>(), tgrp.as<int>(), slots, pslot.as<int>(),
+                                       pflag.as<int>(), pidx.as<long long>(),
+                                       ptmp.as<long long>(), te, hits.as<int>(),
+                                       gctx.as<RowCtx>(), gwork.as<int>(), c->guard_word(),
+                                       c->stream));
+            HIPCHECK(gsk::scan_exclusive(gwork.as<int>(), gwoff.as<long long>(), np,
+                                         ptmp.as<long long>(), te, c->stream));
+            const gsk::ItemsLaunch lc = shape(GS_ITEMS_NEWV, false, np);
+            time_begin();
+            HIPCHECK(gsk::items_newv(lc, c->dg, d_pats.as<PatDev>(), gctx.as<RowCtx>(),
+                                     gwoff.as<long long>(), np, ks, key_slot, sets, pool, c->tq,
+                                     guard_of(nslots, 0, pool_words), c->stream));
+            time_end(1, items_at(gwoff.as<long long>() + np));
+            c->st.fill_launches++;
+        }
     };
 
     // ---- learned sizes: one fill pass below the last level from predicted child counts
@@ -1109,7 +1172,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             std::vector<long long> Wc_p(nchunks, 0);
             for (int ci = 0; ci < nchunks; ++ci) {
                 const long long b0 = (long long)ci * chunk;
-                const long long Rc = prep_chunk(ci, true);
+                const long long Rc = prep_chunk(ci, true, false);
                 d2h(c, &Wc_p[ci], d_woff.as<long long>() + Rc, 1);
                 const gsk::ItemsLaunch lc = shape(GS_ITEMS_ROWS, false, Rc);
                 time_begin();
@@ -1217,7 +1280,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
     for (int ci = 0; ci < nchunks; ++ci) {
         const long long b0 = (long long)ci * chunk;
-        const long long Rc = prep_chunk(ci, true);
+        const long long Rc = prep_chunk(ci, true, true);
         d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
         const gsk::ItemsLaunch lc = shape(single ? GS_ITEMS_SETS : GS_ITEMS_COUNT, single, Rc);
         time_begin();
@@ -1231,6 +1294,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                      c->stream));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
+            newv_pass(Rc, nullptr, K, d_all_sets.as<SetDesc>(), d_all_pool.as<uint32_t>(),
+                      std::max<long long>(all_words, 4));
         } else {
             HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                       d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
@@ -1354,8 +1419,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         if (!single) {  // pass 2: child rows and MNI sets
             for (int ci = 0; ci < nchunks; ++ci) {
                 const long long b0 = (long long)ci * chunk;
-                const long long Rc =
-                    nchunks > 1 ? prep_chunk(ci, false) : std::min(chunk, R - b0);
+                const long long Rc = chunk_ready(ci, !write_rows);
                 const gsk::ItemsLaunch lc =
                     shape(write_rows ? GS_ITEMS_ROWS : GS_ITEMS_SETS, false, Rc);
                 time_begin();
@@ -1366,8 +1430,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                     d_pool.as<uint32_t>(), c->overflow_flag(), c->tq, A, false,
                     guard_of(A, write_rows ? rows_total : 0, std::max<long long>(pool_words, 4)),
                     c->stream));
-                time_end(1, &Wc_host[ci]);
+                time_end(1, items_at(d_woff.as<long long>() + Rc));
                 c->st.fill_launches++;
+                if (!write_rows)
+                    newv_pass(Rc, d_slot.as<int>(), A, d_sets.as<SetDesc>(),
+                              d_pool.as<uint32_t>(), std::max<long long>(pool_words, 4));
             }
         }
         // supports, then the frequent children in key order
@@ -1553,7 +1620,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
         for (int ci = 0; ci < nchunks; ++ci) {
             const long long b0 = (long long)ci * chunk;
-            const long long Rc = nchunks > 1 ? prep_chunk(ci, false) : std::min(chunk, R - b0);
+            const long long Rc = chunk_ready(ci, !write_rows);
             const gsk::ItemsLaunch lc =
                 shape(write_rows ? GS_ITEMS_ROWS : GS_ITEMS_SETS, false, Rc);
             time_begin();
@@ -1566,8 +1633,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                      guard_of(NL, write_rows ? rows_total : 0,
                                               std::max<long long>(pool_words, 4)),
                                      c->stream));
-            time_end(1, &Wc_host[ci]);
+            time_end(1, items_at(d_woff.as<long long>() + Rc));
             c->st.fill_launches++;
+            if (!write_rows)
+                newv_pass(Rc, d_slot.as<int>(), NL, d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
+                          std::max<long long>(pool_words, 4));
         }
     }
     account_bytes(emitted);
@@ -1966,6 +2036,21 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         std::vector<int> lidx(n), adjlx(nnz);
         for (int v = 0; v < n; ++v) lidx[v] = x->lcnt[vr[v]]++;
         for (int i = 0; i < nnz; ++i) adjlx[i] = lidx[adjl[i].x];
+        // runs: maximal ranges of a vertex's adjl entries with one (label, edge label)
+        std::vector<int> rstart, rkey, runid(nnz);
+        for (int v = 0; v < n; ++v) {
+            const int o0 = offsets[v], o1 = offsets[v + 1];
+            for (int i = o0; i < o1; ++i) {
+                const int k = (int)(((adjl[i].y & 0xffffu) << 16) | (adjl[i].y >> 16));
+                if (i == o0 || k != rkey.back()) {
+                    rstart.push_back(i);
+                    rkey.push_back(k);
+                }
+                runid[i] = (int)rkey.size() - 1;
+            }
+        }
+        const int nruns = (int)rkey.size();
+        rstart.push_back(nnz);
         // label start table, when it is at most 1 GiB
         const long long lstride = (long long)x->VL + 1;
         std::vector<int> lstart;
@@ -1993,6 +2078,9 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->d_lidx = DBuf(P, sizeof(int) * std::max(n, 1));
         x->d_adjlx = DBuf(P, sizeof(int) * std::max(nnz, 1));
         x->d_lcnt = DBuf(P, sizeof(int) * x->VL);
+        x->d_rstart = DBuf(P, sizeof(int) * (nruns + 1));
+        x->d_rkey = DBuf(P, sizeof(int) * std::max(nruns, 1));
+        x->d_runid = DBuf(P, sizeof(int) * std::max(nnz, 1));
         x->d_misc = DBuf(P, 64);
         HIPCHECK(gsk::fill_i32(x->d_misc.as<int>(), 0, 16, x->stream));
         x->d_tiles = DBuf(P, sizeof(int) * 2 * GS_TILE_QUEUE);
@@ -2006,6 +2094,9 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         h2d(x, x->d_lidx.as<int>(), lidx.data(), n);
         h2d(x, x->d_adjlx.as<int>(), adjlx.data(), nnz);
         h2d(x, x->d_lcnt.as<int>(), x->lcnt.data(), x->VL);
+        h2d(x, x->d_rstart.as<int>(), rstart.data(), nruns + 1);
+        h2d(x, x->d_rkey.as<int>(), rkey.data(), nruns);
+        h2d(x, x->d_runid.as<int>(), runid.data(), nnz);
         init_label_stats(x, vr, src, adj);
         mlp_init(x);
         sync(x);
@@ -2022,6 +2113,10 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->dg.lidx = x->d_lidx.as<int>();
         x->dg.adjlx = x->d_adjlx.as<int>();
         x->dg.lcnt = x->d_lcnt.as<int>();
+        x->dg.rstart = x->d_rstart.as<int>();
+        x->dg.rkey = x->d_rkey.as<int>();
+        x->dg.runid = x->d_runid.as<int>();
+        x->dg.nruns = nruns;
     });
     if (rc != GS_OK) {
         std::fprintf(stderr, "gs_create: %s\n", c->err.c_str());
@@ -2432,6 +2527,9 @@ int gs_destroy(gs_ctx* c) {
     c->d_lidx.reset();
     c->d_adjlx.reset();
     c->d_lcnt.reset();
+    c->d_rstart.reset();
+    c->d_rkey.reset();
+    c->d_runid.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();

@@ -91,7 +91,7 @@ class LaunchTuner {
             const auto& cs = cfgs_[mode][gsk::items_pm_class(b % 100)];
             if (!first) s += ", ";
             first = false;
-            static const char* const names[GS_ITEMS_MODES] = {"count", "sets", "rows"};
+            static const char* const names[GS_ITEMS_MODES] = {"count", "sets", "rows", "newv"};
             s += "\"" + std::string(names[mode]) + ((b % 1000) >= 100 ? " single" : "") +
                  " m=" + std::to_string(b % 100) + "\": ";
             const int best = best_of(bk);
