@@ -32,6 +32,9 @@ GS_MNI_OMP_LITERAL = 1
 GS_SETS_AUTO = 0
 GS_SETS_BITS = 1
 GS_SETS_BYTES = 2
+GS_GROUPS_AUTO = 0
+GS_GROUPS_ON = 1
+GS_GROUPS_OFF = 2
 GS_MAX_PATTERN_EDGES = 7
 GS_NOALLOC_MAX_CODES = 8192
 
@@ -48,6 +51,7 @@ class GsOpts(ctypes.Structure):
         ("set_kind", ctypes.c_int),
         ("autotune", ctypes.c_int),
         ("tile", ctypes.c_int),
+        ("groups", ctypes.c_int),
     ]
 
 

@@ -331,13 +331,16 @@ hipError_t items_count(const ItemsLaunch& lc, const DevGraph& g, const PatDev* p
 // cursor[s]: next row of slot s (64-bit, so an overrun cannot wrap).  cap_end (nullable):
 // rows of slot s may only be written below cap_end[s]; a child that does not fit, or a hash
 // set that fills up, sets *overflow instead and writes nothing (predicted sizes).
-// rows_out == nullptr: sets only (GS_ITEMS_SETS), else rows and sets (GS_ITEMS_ROWS).
+// rows_out == nullptr: sets only -- by runs (GS_ITEMS_SETS: parent images; the new vertices'
+// come from items_newv) or, by_items, by neighbours (GS_ITEMS_ROWS writing no rows: every
+// image) -- else rows and sets (GS_ITEMS_ROWS).
 hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pats,
                       const RowCtx* ctx, const long long* woff, long long b0, long long R,
                       KeySpace ks, int* key_count, const int* key_slot,
                       unsigned long long* cursor, const int* cap_end, int2* rows_out,
                       const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
-                      int nlive, bool saturate, const ItemGuard& gd, hipStream_t s);
+                      int nlive, bool saturate, const ItemGuard& gd, hipStream_t s,
+                      bool by_items = false);
 // resident workgroups per CU of the extension kernel in a mode (GS_ITEMS_*) at a tile size
 // NEWV over G groups (contexts and scanned work from build_groups; trailing groups of no
 // work are allowed)

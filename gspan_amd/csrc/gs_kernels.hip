@@ -1094,7 +1094,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
                 const int key = s_key[h];
                 if (key < 0) continue;
-                if (ROWS) {
+                if (ROWS && rows_out) {
                     // 64-bit cursors cannot wrap however far a key overruns a predicted
                     // capacity; a reservation that does not fit writes nothing
                     const unsigned long long cnt = (unsigned long long)s_cnt[h];
@@ -1128,7 +1128,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             if (threadIdx.x == 0) s_pfill = 0;  // read again only after later barriers
         }
         // phase 3 (rows): child rows at their reserved positions
-        if (ROWS) {
+        if (ROWS && rows_out) {
 #pragma unroll
             for (int q = 0; q < IPT; ++q) {
                 if (hs[q] < 0) continue;
@@ -2190,11 +2190,12 @@ hipError_t items_fill(const ItemsLaunch& lc, const DevGraph& g, const PatDev* pa
                       KeySpace ks, int* key_count, const int* key_slot,
                       unsigned long long* cursor, const int* cap_end, int2* rows_out,
                       const SetDesc* sets, uint32_t* pool, int* overflow, TileQueues& tq,
-                      int nlive, bool saturate, const ItemGuard& gd, hipStream_t s) {
+                      int nlive, bool saturate, const ItemGuard& gd, hipStream_t s,
+                      bool by_items) {
     if (R <= 0) return hipSuccess;
-    // pair keys slot * TILE + row must stay below 2^31
-    const int dedup = (long long)nlive * lc.tile < (1LL << 31) ? 1 : 0;
-    if (rows_out)
+    const int dedup = 1;
+    (void)nlive;
+    if (rows_out || by_items)
         return launch_items<GS_ITEMS_ROWS>(lc, g, pats, ctx, woff, b0, R, ks, key_count, key_slot,
                                            cursor, cap_end, rows_out, sets, pool, overflow, tq,
                                            gd, dedup, saturate ? 1 : 0, s);

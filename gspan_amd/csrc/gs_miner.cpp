@@ -28,6 +28,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <thread>
 #include <vector>
 
 #include "gs_internal.h"
@@ -297,6 +298,12 @@ struct gs_ctx {
     std::vector<int> lcnt;
     // runs of equal (neighbour label, edge label) in adjl
     DBuf d_rstart, d_rkey, d_runid;
+    // neighbours per run as a row scan sees them: a vertex is scanned about once per
+    // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
+    // runs make the run modes and per-group new-vertex inserts pay at the last level
+    // (measured: 2.15 on the ER workload, where they cost 13%; 2.7 and 5.3 on the R-MAT
+    // scale-14 and scale-18 workloads with 100 labels, where they saved 24% and 38%)
+    double run_gain = 1.0;
     DevGraph dg{};
     // [0] guard word, [1] n_active, [2] err, [3] overflow, [4..5] int64 total,
     // [6..7] items without the label filter, [8..11] two 64-bit step totals
@@ -429,6 +436,60 @@ void d2h(gs_ctx* c, T* dst, const T* src, size_t count) {
     } else {
         HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     }
+}
+
+// Distinct labels of count values p[0], p[stride], ... in ascending order (rank -> label),
+// and each label's rank: a direct table over the labels' range when it spans at most 2^24
+// values (two passes over the values, no sort), else a sort of the values.
+struct LabelRanks {
+    std::vector<int> vals;
+    long long lo = 0;
+    std::vector<int> table;  // label - lo -> rank (direct table), else empty
+    void build(const int* p, long long count, long long stride) {
+        vals.clear();
+        table.clear();
+        if (count == 0) return;
+        int mn = p[0], mx = p[0];
+        for (long long i = 1; i < count; ++i) {
+            const int v = p[i * stride];
+            mn = std::min(mn, v);
+            mx = std::max(mx, v);
+        }
+        lo = mn;
+        const long long span = (long long)mx - mn + 1;
+        if (span <= (1LL << 24)) {
+            table.assign((size_t)span, -1);
+            for (long long i = 0; i < count; ++i) table[(size_t)((long long)p[i * stride] - lo)] = 0;
+            for (long long k = 0; k < span; ++k)
+                if (table[(size_t)k] == 0) {
+                    table[(size_t)k] = (int)vals.size();
+                    vals.push_back((int)(lo + k));
+                }
+        } else {
+            vals.resize((size_t)count);
+            for (long long i = 0; i < count; ++i) vals[(size_t)i] = p[i * stride];
+            std::sort(vals.begin(), vals.end());
+            vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+        }
+    }
+    int rank(int label) const {
+        if (!table.empty()) return table[(size_t)((long long)label - lo)];
+        return int(std::lower_bound(vals.begin(), vals.end(), label) - vals.begin());
+    }
+};
+
+// fn(begin, end) over [0, n) split among the host's hardware threads (at most 16)
+void parallel_ranges(long long n, const std::function<void(long long, long long)>& fn) {
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = (int)std::min<long long>(hw, std::max(1LL, n / (1 << 16)));
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < nt; ++t)
+        ts.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+    for (auto& th : ts) th.join();
 }
 
 int vl_rank(const gs_ctx* c, int label) {
@@ -1073,6 +1134,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // over any split of the rows gives the same sets.
     int nseg = 1;
     for (int i = 0; i < B; ++i) nseg = std::max(nseg, pd[i].r + 1);
+    // last level by runs and groups, or by every neighbour of every row
+    constexpr double kGroupGain = 2.5;
+    const bool grouped = c->opts.groups == GS_GROUPS_ON ||
+                         (c->opts.groups != GS_GROUPS_OFF && c->run_gain >= kGroupGain);
     auto newv_pass = [&](long long Rc, const int* key_slot, long long nslots, const SetDesc* sets,
                          uint32_t* pool, long long pool_words) {
         constexpr long long kGroupPairs = 1LL << 24;
@@ -1280,9 +1345,11 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
     for (int ci = 0; ci < nchunks; ++ci) {
         const long long b0 = (long long)ci * chunk;
-        const long long Rc = prep_chunk(ci, true, true);
+        const bool by_items = single && !grouped;  // every neighbour of every row
+        const long long Rc = prep_chunk(ci, true, !by_items);
         d2h(c, &Wc_host[ci], d_woff.as<long long>() + Rc, 1);
-        const gsk::ItemsLaunch lc = shape(single ? GS_ITEMS_SETS : GS_ITEMS_COUNT, single, Rc);
+        const gsk::ItemsLaunch lc =
+            shape(single ? (by_items ? GS_ITEMS_ROWS : GS_ITEMS_SETS) : GS_ITEMS_COUNT, single, Rc);
         time_begin();
         if (single) {
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
@@ -1291,11 +1358,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                      d_all_pool.as<uint32_t>(), c->overflow_flag(),
                                      c->tq, (int)K, saturate,
                                      guard_of(K, K, std::max<long long>(all_words, 4)),
-                                     c->stream));
+                                     c->stream, by_items));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
-            newv_pass(Rc, nullptr, K, d_all_sets.as<SetDesc>(), d_all_pool.as<uint32_t>(),
-                      std::max<long long>(all_words, 4));
+            if (!by_items)
+                newv_pass(Rc, nullptr, K, d_all_sets.as<SetDesc>(), d_all_pool.as<uint32_t>(),
+                          std::max<long long>(all_words, 4));
         } else {
             HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                       d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(),
@@ -1419,9 +1487,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         if (!single) {  // pass 2: child rows and MNI sets
             for (int ci = 0; ci < nchunks; ++ci) {
                 const long long b0 = (long long)ci * chunk;
-                const long long Rc = chunk_ready(ci, !write_rows);
-                const gsk::ItemsLaunch lc =
-                    shape(write_rows ? GS_ITEMS_ROWS : GS_ITEMS_SETS, false, Rc);
+                const bool runs = !write_rows && grouped;
+                const long long Rc = chunk_ready(ci, runs);
+                const gsk::ItemsLaunch lc = shape(runs ? GS_ITEMS_SETS : GS_ITEMS_ROWS, false, Rc);
                 time_begin();
                 HIPCHECK(gsk::items_fill(
                     lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(), d_woff.as<long long>(),
@@ -1429,10 +1497,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                     nullptr, write_rows ? out.rows.ptr() : nullptr, d_sets.as<SetDesc>(),
                     d_pool.as<uint32_t>(), c->overflow_flag(), c->tq, A, false,
                     guard_of(A, write_rows ? rows_total : 0, std::max<long long>(pool_words, 4)),
-                    c->stream));
+                    c->stream, !runs));
                 time_end(1, items_at(d_woff.as<long long>() + Rc));
                 c->st.fill_launches++;
-                if (!write_rows)
+                if (runs)
                     newv_pass(Rc, d_slot.as<int>(), A, d_sets.as<SetDesc>(),
                               d_pool.as<uint32_t>(), std::max<long long>(pool_words, 4));
             }
@@ -1620,9 +1688,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
         for (int ci = 0; ci < nchunks; ++ci) {
             const long long b0 = (long long)ci * chunk;
-            const long long Rc = chunk_ready(ci, !write_rows);
-            const gsk::ItemsLaunch lc =
-                shape(write_rows ? GS_ITEMS_ROWS : GS_ITEMS_SETS, false, Rc);
+            const bool runs = !write_rows && grouped;
+            const long long Rc = chunk_ready(ci, runs);
+            const gsk::ItemsLaunch lc = shape(runs ? GS_ITEMS_SETS : GS_ITEMS_ROWS, false, Rc);
             time_begin();
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, nullptr,
@@ -1632,10 +1700,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                      c->overflow_flag(), c->tq, NL, false,
                                      guard_of(NL, write_rows ? rows_total : 0,
                                               std::max<long long>(pool_words, 4)),
-                                     c->stream));
+                                     c->stream, !runs));
             time_end(1, items_at(d_woff.as<long long>() + Rc));
             c->st.fill_launches++;
-            if (!write_rows)
+            if (runs)
                 newv_pass(Rc, d_slot.as<int>(), NL, d_sets.as<SetDesc>(), d_pool.as<uint32_t>(),
                           std::max<long long>(pool_words, 4));
         }
@@ -1976,13 +2044,11 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->n = n;
         x->nnz = nnz;
         for (int v = 0; v < n; ++v) x->max_deg = std::max(x->max_deg, offsets[v + 1] - offsets[v]);
-        x->vl_vals.assign(vlabels, vlabels + n);
-        std::sort(x->vl_vals.begin(), x->vl_vals.end());
-        x->vl_vals.erase(std::unique(x->vl_vals.begin(), x->vl_vals.end()), x->vl_vals.end());
-        x->el_vals.clear();
-        for (int i = 0; i < nnz; ++i) x->el_vals.push_back(edges3[3 * (size_t)i + 2]);
-        std::sort(x->el_vals.begin(), x->el_vals.end());
-        x->el_vals.erase(std::unique(x->el_vals.begin(), x->el_vals.end()), x->el_vals.end());
+        LabelRanks vranks, eranks;
+        vranks.build(vlabels, n, 1);
+        eranks.build(edges3 + 2, nnz, 3);
+        x->vl_vals = vranks.vals;
+        x->el_vals = eranks.vals;
         if (x->vl_vals.empty()) x->vl_vals.push_back(0);
         if (x->el_vals.empty()) x->el_vals.push_back(0);
         if (x->vl_vals.size() > 65535 || x->el_vals.size() > 65535)
@@ -2009,28 +2075,31 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
 
         std::vector<int> vr(n), src(nnz);
         std::vector<uint2> adj(nnz);
-        std::unordered_map<int, int> vmap, emap;
-        for (int i = 0; i < x->VL; ++i) vmap[x->vl_vals[i]] = i;
-        for (int i = 0; i < x->EL; ++i) emap[x->el_vals[i]] = i;
-        for (int v = 0; v < n; ++v) vr[v] = vmap[vlabels[v]];
-        for (int i = 0; i < nnz; ++i) {
-            const int* e = edges3 + 3 * (size_t)i;
-            src[i] = e[0];
-            adj[i].x = (unsigned)e[1];
-            adj[i].y = ((unsigned)emap[e[2]] << 16) | (unsigned)vr[e[1]];
-        }
+        parallel_ranges(n, [&](long long a, long long b) {
+            for (long long v = a; v < b; ++v) vr[v] = vranks.rank(vlabels[v]);
+        });
+        parallel_ranges(nnz, [&](long long a, long long b) {
+            for (long long i = a; i < b; ++i) {
+                const int* e = edges3 + 3 * (size_t)i;
+                src[i] = e[0];
+                adj[i].x = (unsigned)e[1];
+                adj[i].y = ((unsigned)eranks.rank(e[2]) << 16) | (unsigned)vr[e[1]];
+            }
+        });
         // each vertex's entries ordered by (neighbour label, edge label), with their CSR
         // indices: the entries of one child key are then consecutive work items
         std::vector<uint2> adjl(nnz);
         std::vector<int> eidl(nnz);
         auto lab_el = [&](int i) { return ((adj[i].y & 0xffffu) << 16) | (adj[i].y >> 16); };
-        for (int v = 0; v < n; ++v) {
-            const int o0 = offsets[v], o1 = offsets[v + 1];
-            for (int i = o0; i < o1; ++i) eidl[i] = i;
-            std::stable_sort(eidl.begin() + o0, eidl.begin() + o1,
-                             [&](int a, int b) { return lab_el(a) < lab_el(b); });
-            for (int i = o0; i < o1; ++i) adjl[i] = adj[eidl[i]];
-        }
+        parallel_ranges(n, [&](long long va, long long vb) {
+            for (long long v = va; v < vb; ++v) {
+                const int o0 = offsets[v], o1 = offsets[v + 1];
+                for (int i = o0; i < o1; ++i) eidl[i] = i;
+                std::stable_sort(eidl.begin() + o0, eidl.begin() + o1,
+                                 [&](int a, int b) { return lab_el(a) < lab_el(b); });
+                for (int i = o0; i < o1; ++i) adjl[i] = adj[eidl[i]];
+            }
+        });
         // label classes: each vertex's rank among the vertices of its label (ascending id)
         x->lcnt.assign(x->VL, 0);
         std::vector<int> lidx(n), adjlx(nnz);
@@ -2051,6 +2120,17 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         }
         const int nruns = (int)rkey.size();
         rstart.push_back(nnz);
+        {
+            double num = 0, den = 0;
+            for (int v = 0; v < n; ++v) {
+                const int o0 = offsets[v], o1 = offsets[v + 1];
+                if (o1 == o0) continue;
+                const double d = o1 - o0;
+                num += d * d;
+                den += d * (runid[o1 - 1] - runid[o0] + 1);
+            }
+            x->run_gain = den > 0 ? num / den : 1.0;
+        }
         // label start table, when it is at most 1 GiB
         const long long lstride = (long long)x->VL + 1;
         std::vector<int> lstart;

@@ -43,7 +43,7 @@ class Miner:
     def __init__(self, g: CSRGraph, mode: str = "batched", mni: str = "gpu", device: int = 0,
                  timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0,
                  two_pass: bool = False, sets: str = "auto", autotune: bool = True,
-                 tile: int = 0):
+                 tile: int = 0, groups: str = "auto"):
         self._lib = L.lib()
         opts = L.GsOpts()
         self._lib.gs_default_opts(ctypes.byref(opts))
@@ -59,6 +59,8 @@ class Miner:
                          "bytes": L.GS_SETS_BYTES}[sets]
         opts.autotune = 1 if autotune else 0
         opts.tile = tile
+        opts.groups = {"auto": L.GS_GROUPS_AUTO, "on": L.GS_GROUPS_ON,
+                       "off": L.GS_GROUPS_OFF}[groups]
         self.mni_mode = opts.mni_mode
         self.graph = g
         h = ctypes.c_void_p()

@@ -67,11 +67,19 @@ typedef struct gs_opts {
                                         size and workgroups per CU by timing launches  */
     int tile;                        /* 0, or a fixed tile size (256, 512, 1024) with
                                         full occupancy (turns the tuner off)           */
+    int groups;                      /* last level: GS_GROUPS_AUTO counts by neighbour
+                                        runs and inserts new vertices once per group of
+                                        rows sharing a segment vertex where the graph's
+                                        runs are long, else scans every neighbour of
+                                        every row; GS_GROUPS_ON / GS_GROUPS_OFF force it */
 } gs_opts;
 
 #define GS_SETS_AUTO 0
 #define GS_SETS_BITS 1
 #define GS_SETS_BYTES 2
+#define GS_GROUPS_AUTO 0
+#define GS_GROUPS_ON 1
+#define GS_GROUPS_OFF 2
 
 typedef struct gs_stats {
     long long extended;     /* child rows emitted by extension (gspan.cu:708,717,730);   */

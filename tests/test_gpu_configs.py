@@ -52,8 +52,9 @@ def _predicted_weights(g, k, ms):
 
 
 def _all_modes_match(oracle_mod, g, k, ms, want, st):
-    for mode in ("batched", "plain", "predicted"):
-        with Miner(g, mode=mode) as mn:
+    for mode, groups in (("batched", "on"), ("batched", "off"), ("plain", "auto"),
+                         ("predicted", "auto")):
+        with Miner(g, mode=mode, groups=groups) as mn:
             if mode == "predicted":
                 w = _predicted_weights(g, k, ms)
                 if w is not None:

@@ -378,8 +378,9 @@ def test_large_key_space(oracle_mod):
 @pytest.mark.parametrize("seed", range(12))
 def test_random_graphs_all_modes(oracle_mod, seed):
     """Seeded ER and small R-MAT graphs with random label counts, k, threshold and MNI
-    semantics: batched, plain and bitmap-set runs all give the oracle's report and
-    `extended` total."""
+    semantics: batched, plain and bitmap-set runs, with the last level by neighbour runs and
+    row groups or by every neighbour, single-pass or two-pass, all give the oracle's report
+    and `extended` total."""
     rng = np.random.default_rng(7000 + seed)
     if seed % 3:
         nv = int(rng.integers(30, 400))
@@ -395,7 +396,12 @@ def test_random_graphs_all_modes(oracle_mod, seed):
     mni = int(rng.integers(0, 2))
     _announce(f"random graph seed={seed} k={k} ms={ms} mni={mni}")
     want, st = oracle_mod.fsm(g, k, ms, mni_mode=mni)
-    for mode, sets in (("batched", "auto"), ("plain", "auto"), ("batched", "bits")):
-        with Miner(g, mode=mode, mni=("gpu", "omp-literal")[mni], sets=sets) as mn:
-            assert mn.mine(k, ms) == want, (mode, sets)
-            assert mn.stats()["extended"] == st["extended"], (mode, sets)
+    for mode, sets, groups, two in (("batched", "auto", "auto", False),
+                                    ("plain", "auto", "on", False),
+                                    ("batched", "bits", "off", False),
+                                    ("batched", "auto", "on", True),
+                                    ("plain", "auto", "off", True)):
+        with Miner(g, mode=mode, mni=("gpu", "omp-literal")[mni], sets=sets, groups=groups,
+                   two_pass=two) as mn:
+            assert mn.mine(k, ms) == want, (mode, sets, groups, two)
+            assert mn.stats()["extended"] == st["extended"], (mode, sets, groups, two)
