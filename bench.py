@@ -166,6 +166,12 @@ def main() -> int:
     ap.add_argument("--min-sup", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0,
                     help="fixed tile size of the extension kernels (turns the tuner off)")
+    ap.add_argument("--timed-mode", default="batched", choices=("batched", "plain"),
+                    help="method of the timed steps (the metric is quoted for batched)")
+    ap.add_argument("--groups", default="auto", choices=("auto", "on", "off"),
+                    help="last level by neighbour runs and row groups, or by every neighbour")
+    ap.add_argument("--two-pass", action="store_true",
+                    help="count every level before filling it (no single-pass last level)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-predicted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -213,7 +219,8 @@ def main() -> int:
     gstats = g.stats()
     log(f"graph {args.workload}: {gstats} ({time.perf_counter() - t0:.1f}s)")
 
-    with Miner(g, mode="batched", timing=True, tile=args.tile) as mn:
+    with Miner(g, mode=args.timed_mode, timing=True, tile=args.tile, groups=args.groups,
+               two_pass=args.two_pass) as mn:
         for i in range(args.warmup):
             t = time.perf_counter()
             res = mn.mine_raw(k, ms)
@@ -256,7 +263,7 @@ def main() -> int:
 
     plain = None
     if not args.no_plain:
-        with Miner(g, mode="plain") as mp:
+        with Miner(g, mode="plain", groups=args.groups) as mp:
             t = time.perf_counter()
             pres = mp.mine_raw(k, ms)
             pdt = time.perf_counter() - t
@@ -281,7 +288,7 @@ def main() -> int:
             loss = ml.predictor_train(epochs=300, lr=3e-3)
             train_s = time.perf_counter() - t
             weights = ml.predictor_weights()
-        with Miner(g, mode="predicted", timing=True) as mq:
+        with Miner(g, mode="predicted", timing=True, groups=args.groups) as mq:
             mq.set_predictor_weights(weights)
             for _ in range(args.warmup):
                 mq.mine_raw(k, ms)
@@ -332,7 +339,8 @@ def main() -> int:
                         + (f" Zipf({w['vlabel_zipf']})" if w["vlabel_zipf"] else " uniform")
                         + f", {w['n_elabels']} elabels, min_sup {ms}, k {k}",
             "downsized_from": w.get("of"),
-            "k": k, "min_support": ms, "mode": "batched", "mni": "gpu",
+            "k": k, "min_support": ms, "mode": args.timed_mode, "mni": "gpu",
+            "groups": args.groups, "two_pass": args.two_pass,
             "graph": gstats,
         },
         "patterns": len(pats),

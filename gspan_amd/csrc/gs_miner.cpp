@@ -217,11 +217,14 @@ struct OrderKey {
     }
 };
 
+// One reported pattern: its last code edge and its parent's report (the parent is reported
+// before it), so a report is 40 bytes however long the code; codes are rebuilt along a
+// pre-order walk of the report tree (for_each_report).
 struct Report {
     int m;
-    int code5[5 * GS_MAXE];  // original labels, reference placeholder form
-    uint64_t edge_key;       // order word of the last code edge (see order_key)
-    int parent;              // report of the parent pattern, -1 for a single edge
+    int e5[5];          // last code edge, original labels, reference placeholder form
+    uint64_t edge_key;  // order word of the last code edge (see order_key)
+    int parent;         // report of the parent pattern, -1 for a single edge
     int support;
 };
 
@@ -329,6 +332,7 @@ struct gs_ctx {
     // mining state
     int k = 0, min_sup = 0, mni_mode = 0;
     std::vector<Report> reports;
+    std::vector<int> report_order;  // reports in the reference's order (order_reports)
     // child-count predictor: label statistics, parameters, Adam state, logged samples
     DBuf d_H, d_vfreq, d_efreq, d_dmean;
     DBuf d_w32, d_w16, d_adam_m, d_adam_v, d_grads;
@@ -631,29 +635,74 @@ void order_reports(gs_ctx* c) {
         for (int i = start[x + 2] - 1; i >= start[x + 1]; --i) stack.push_back(kids[i]);
     }
     if ((int)order.size() != n) throw GsError(GS_ERR_INTERNAL, "report tree is not connected");
-    std::vector<Report> sorted;
-    sorted.reserve(n);
-    for (int i : order) sorted.push_back(std::move(r[i]));
-    r.swap(sorted);
+    c->report_order.swap(order);
+}
+
+// fn(code5, m, support) for every report in the reference's order: the order is a pre-order
+// of the report tree, so the codes of a report's ancestors are the prefix of the path built
+// so far.
+template <class Fn>
+void for_each_report(const gs_ctx* c, Fn&& fn) {
+    int path[5 * GS_MAXE];
+    for (int i : c->report_order) {
+        const Report& r = c->reports[i];
+        std::memcpy(path + 5 * (r.m - 1), r.e5, sizeof(r.e5));
+        fn(path, r.m, r.support);
+    }
 }
 
 // Appends the report of h; returns its index.
 int add_report(gs_ctx* c, const HPat& h) {
     Report r;
     r.m = h.m;
-    for (int j = 0; j < h.m; ++j) {
-        r.code5[5 * j + 0] = h.f[j];
-        r.code5[5 * j + 1] = h.t[j];
-        r.code5[5 * j + 2] = j == 0 ? c->vl_vals[h.fl0] : -1;
-        r.code5[5 * j + 3] = c->el_vals[h.el[j]];
-        r.code5[5 * j + 4] = h.tl[j] < 0 ? -1 : c->vl_vals[h.tl[j]];
-    }
-    r.edge_key = edge_word(h, h.m - 1);
+    const int j = h.m - 1;
+    r.e5[0] = h.f[j];
+    r.e5[1] = h.t[j];
+    r.e5[2] = j == 0 ? c->vl_vals[h.fl0] : -1;
+    r.e5[3] = c->el_vals[h.el[j]];
+    r.e5[4] = h.tl[j] < 0 ? -1 : c->vl_vals[h.tl[j]];
+    r.edge_key = edge_word(h, j);
     r.parent = h.parent_report;
     r.support = h.support;
-    c->reports.push_back(std::move(r));
+    c->reports.push_back(r);
     c->st.reported++;
     return (int)c->reports.size() - 1;
+}
+
+// Appends the report of the child of parent pattern p (device pattern d) at key slot `slot`
+// with its support, straight from the key: the last edge alone is decoded (as in
+// decode_child_into), for children that are reported and not extended.
+void report_child(gs_ctx* c, const HPat& p, const PatDev& d, int slot, int support) {
+    const int EL = c->EL, VL = c->VL;
+    const int nb = (d.r - 1) * EL;
+    int f, t, el, tl;
+    if (slot < nb) {
+        f = d.v_n;
+        t = p.f[d.rm[slot / EL + 1]];
+        el = slot % EL;
+        tl = -1;
+    } else {
+        const int s2 = slot - nb, seg = s2 / (EL * VL);
+        f = seg == 0 ? d.v_n : p.f[d.rm[seg - 1]];
+        t = d.v_n + 1;
+        el = (s2 / VL) % EL;
+        tl = s2 % VL;
+    }
+    Report r;
+    r.m = p.m + 1;
+    r.e5[0] = f;
+    r.e5[1] = t;
+    r.e5[2] = -1;
+    r.e5[3] = c->el_vals[el];
+    r.e5[4] = tl < 0 ? -1 : c->vl_vals[tl];
+    auto put = [](int a, int b, int cc, int dd) {
+        return (uint64_t)a << 48 | (uint64_t)b << 32 | (uint64_t)cc << 16 | (uint64_t)dd;
+    };
+    r.edge_key = f > t ? put(0, t + 8, el, 0) : put(1, 8 - f, el, tl);  // as edge_word
+    r.parent = p.report_id;
+    r.support = support;
+    c->reports.push_back(r);
+    c->st.reported++;
 }
 
 // Support from the per-vertex distinct-image counts (gspan.cu:282-288 / 367-373).
@@ -938,6 +987,7 @@ struct StepOut {
     LevelRows rows;                   // child rows of the written keys
     long long rows_total = 0;         // rows in `rows`
     long long emitted = 0;
+    bool reported_inline = false;     // the frequent children were reported by run_step
 };
 
 // error of a canonicality walk that outgrew its projection buffers (never expected: the
@@ -1542,8 +1592,19 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             c->child_bufs.pop_back();
             out.children.clear();
         }
-        out.children.reserve(F);
         int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
+        if (!write_rows) {  // the last level: frequent children are reported, not extended
+            c->reports.reserve(c->reports.size() + F);
+            for (int i = 0; i < F; ++i) {
+                const int key = fkey[i];
+                while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
+                if (fsup[i] >= c->min_sup)
+                    report_child(c, pats[p], pd[p], key - pd[p].key_base, fsup[i]);
+            }
+            F = 0;
+            out.reported_inline = true;
+        }
+        out.children.reserve(F);
         for (int i = 0; i < F; ++i) {
             const int key = fkey[i];
             while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
@@ -1755,7 +1816,7 @@ void expand(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) {
     c->st.canonical += so.canonical;
     std::vector<HPat> next;
     for (auto& ch : so.children) {  // canonical children (the others only count)
-        if (ch.support < c->min_sup) continue;
+        if (so.reported_inline || ch.support < c->min_sup) continue;
         ch.pat.report_id = add_report(c, ch.pat);
         if (!last) next.push_back(ch.pat);
     }
@@ -2220,6 +2281,7 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->min_sup = min_support;
         c->mni_mode = c->opts.mni_mode;
         c->reports.clear();
+        c->report_order.clear();
         clear_guard(c);
         c->reset_tile_queues();
         mine_seeds(c);
@@ -2232,9 +2294,11 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->st.t_mine_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         c->st.peak_device_bytes = c->pool.peak();
         if (cb) {
-            for (const Report& r : c->reports)
-                if (cb(r.code5, r.m, r.support, user) != 0)
-                    throw GsError(GS_ERR_CALLBACK, "report callback stopped mining");
+            bool stop = false;
+            for_each_report(c, [&](const int* code5, int m, int support) {
+                if (!stop && cb(code5, m, support, user) != 0) stop = true;
+            });
+            if (stop) throw GsError(GS_ERR_CALLBACK, "report callback stopped mining");
         }
     });
 }
@@ -2562,22 +2626,22 @@ int gs_tuner_report(const gs_ctx* c, char* buf, int cap, int* len) {
 int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
     if (!c || !count || !total_edges) return GS_ERR_INVALID;
     long long e = 0;
-    for (const Report& r : c->reports) e += r.m;
-    *count = (long long)c->reports.size();
+    for (int i : c->report_order) e += c->reports[i].m;
+    *count = (long long)c->report_order.size();
     *total_edges = e;
     return GS_OK;
 }
 
 int gs_result_copy(const gs_ctx* c, int* codes5, int* lens, int* supports) {
     if (!c || (!c->reports.empty() && (!codes5 || !lens || !supports))) return GS_ERR_INVALID;
-    long long at = 0;
-    for (size_t i = 0; i < c->reports.size(); ++i) {
-        const Report& r = c->reports[i];
-        std::memcpy(codes5 + at, r.code5, sizeof(int) * 5 * r.m);
-        at += 5LL * r.m;
-        lens[i] = r.m;
-        supports[i] = r.support;
-    }
+    long long at = 0, i = 0;
+    for_each_report(c, [&](const int* code5, int m, int support) {
+        std::memcpy(codes5 + at, code5, sizeof(int) * 5 * m);
+        at += 5LL * m;
+        lens[i] = m;
+        supports[i] = support;
+        ++i;
+    });
     return GS_OK;
 }
 
