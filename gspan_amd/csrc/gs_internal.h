@@ -117,7 +117,10 @@ struct SetDesc {
     int nv;               // vertex sets (vertices of the child pattern); 0 = no sets
     int kinds;            // kind of vertex set v (GS_SET_*) in bits [2v, 2v + 2)
     int vo[GS_MAXV + 1];  // vertex set v: words [off + vo[v], off + vo[v + 1]); vo[nv] = all
-    int pad[2];
+    // byte a bytemap insert stores, 1..255: a bytemap holds exactly the bytes equal to it,
+    // so a pool reused with a new tag needs no clearing (bytes of older tags read as absent)
+    int tag;
+    int pad;
 };
 static_assert(sizeof(SetDesc) == 64, "SetDesc is four 16-byte loads");
 
@@ -163,7 +166,8 @@ __host__ __device__ inline long long plan_key_sets(long long rows, int nv, const
             d->vo[v] = (int)w;
         }
     }
-    d->pad[0] = d->pad[1] = 0;
+    d->tag = 1;
+    d->pad = 0;
     return w;
 }
 
@@ -181,7 +185,7 @@ struct KeySpace {
 // pattern's keys start at P.set_base: its backward keys (the parent's vertex sets), then
 // per (segment, edge label) one key per label rank (the parent's sets plus the new vertex's).
 __host__ __device__ inline void all_key_set(const PatDev& P, int slot, KeySpace ks, const int* lw,
-                                            const int* lcum, int kind, SetDesc* d) {
+                                            const int* lcum, int kind, int tag, SetDesc* d) {
     const int nb = (P.r - 1) * ks.EL;
     const bool fwd = slot >= nb;
     const int tl = fwd ? (slot - nb) % ks.VL : 0;
@@ -205,7 +209,8 @@ __host__ __device__ inline void all_key_set(const PatDev& P, int slot, KeySpace 
                (long long)tl * pw + lcum[tl];
     }
     d->off = off;
-    d->pad[0] = d->pad[1] = 0;
+    d->tag = tag;
+    d->pad = 0;
 }
 
 struct DevGraph {
@@ -399,7 +404,7 @@ hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys
 // lcum[l] = lw[0] + ... + lw[l-1] (VL + 1 entries); a pattern's keys start at its set_base:
 // its backward keys (parent sets), then per (segment, edge label) one key per label rank.
 hipError_t all_key_sets(const PatDev* pats, int npats, KeySpace ks, long long K, const int* lw,
-                        const int* lcum, int kind, SetDesc* sets, hipStream_t s);
+                        const int* lcum, int kind, int tag, SetDesc* sets, hipStream_t s);
 hipError_t gather_frequent(const int* fidx, const int* nf, int max_f, const int* akeys,
                            const int* acnt, const long long* row_off, int* fkey, int* fcnt,
                            long long* frow, hipStream_t s);

@@ -97,6 +97,7 @@ struct SetRegs {
     int nv, kinds;
     int8v vo;  // vo[0 .. 7]
     int vo8;   // vo[8]
+    int tag;   // the byte a bytemap insert stores
 };
 
 __device__ __forceinline__ SetRegs load_set(const SetDesc* p) {
@@ -109,6 +110,7 @@ __device__ __forceinline__ SetRegs load_set(const SetDesc* p) {
     r.kinds = a.w;
     r.vo = int8v{b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
     r.vo8 = e.x;
+    r.tag = e.y;
     return r;
 }
 
@@ -144,7 +146,7 @@ __device__ __forceinline__ void set_insert(const SetRegs& d, uint32_t* pool, int
     uint32_t* base = pool + d.off + v0;
     const uint32_t words = (uint32_t)(vo_at<NV>(d, v + 1) - v0);
     if (kind == GS_SET_BYTEMAP) {
-        if (x < 4u * words) reinterpret_cast<unsigned char*>(base)[x] = 1;
+        if (x < 4u * words) reinterpret_cast<unsigned char*>(base)[x] = (unsigned char)d.tag;
         else atomicOr(gerr, GS_GUARD_POOL);
     } else if (kind == GS_SET_BITMAP) {
         if ((x >> 5) < words) {
@@ -204,6 +206,7 @@ struct ItemOut {
     int newv;  // label-class rank of the new vertex's image (forward; NEWX) or -1
     int nv;    // vertices of the parent pattern
     int w;     // children the item stands for (1, or a run's children)
+    int sv;    // pattern vertex the forward segment starts at (-1: backward)
     long long pbase;  // parent row in its level = pbase + (row in the batch)
 };
 
@@ -212,7 +215,7 @@ __device__ __forceinline__ bool in_range(long long x, long long n) {
 }
 
 __device__ __forceinline__ ItemOut no_item(const PatDev& P) {
-    return ItemOut{false, 0, 0, -1, P.nv, 1, P.row_begin - P.brow_begin};
+    return ItemOut{false, 0, 0, -1, P.nv, 1, -1, P.row_begin - P.brow_begin};
 }
 
 // Backward probe q < r-1 of a parent row: rmpath index q+1 (gspan.cu:428-445).  c is the
@@ -292,6 +295,7 @@ __device__ __forceinline__ ItemOut eval_fwd(const DevGraph& g, const PatDev& P, 
     o.valid = true;
     o.eid = g.eidl[el_idx];  // the entry's CSR index
     o.newv = NEWX ? g.adjlx[el_idx] : 0;
+    o.sv = pget<UNI, NV>(P.seg_v, j);
     o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
     return o;
 }
@@ -344,6 +348,7 @@ __device__ __forceinline__ ItemOut eval_run(const DevGraph& g, const PatDev& P, 
     if (en - st - excl <= 0) return o;
     o.valid = true;
     o.w = en - st - excl;
+    o.sv = pget<UNI, NV>(P.seg_v, j);
     o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
     return o;
 }
@@ -941,7 +946,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             const int it = q * GS_BLOCK + threadIdx.x;  // item offset in the tile
             hs[q] = -1;
             rank[q] = 0;
-            ItemOut o{false, 0, 0, -1, 0, 1, 0};
+            ItemOut o{false, 0, 0, -1, 0, 1, -1, 0};
             RowRegs c;
             int p = -1, f = 0, row = 0;
             if (it < tn) {
@@ -1054,10 +1059,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                     prow[q] = (int)(o.pbase + b0 + rlo + row);
                 }
                 const int nv = o.nv;
+                // SETS leaves the image of a forward run's segment vertex to NEWV: it is
+                // the group's vertex u, the same for every row of the group, and the group's
+                // key has a child exactly when one of NEWV's items for it does
                 if (PARENT && lead) {
 #pragma unroll
                     for (int v = 0; v < PM + 1; ++v) {
-                        if (v >= nv) continue;
+                        if (v >= nv || (MODE == GS_ITEMS_SETS && v == o.sv)) continue;
                         const int x = c.phi[v];
                         if (in_range(x, g.n))
                             set_insert<CV>(d, pool, v, (uint32_t)g.lidx[x], gd.err, overflow);
@@ -1066,6 +1074,12 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 }
                 if (NEWX && o.newv >= 0)
                     set_insert<CV>(d, pool, nv, (uint32_t)o.newv, gd.err, overflow);
+                if (NEWV && o.sv >= 0) {
+                    const int x = pickv<PM + 1>(c.phi, o.sv);
+                    if (in_range(x, g.n))
+                        set_insert<CV>(d, pool, o.sv, (uint32_t)g.lidx[x], gd.err, overflow);
+                    else atomicOr(gd.err, GS_GUARD_GRAPH);
+                }
             }
         }
         // wave 0: the next tile's first row -- from this tile's offsets when it follows
@@ -1440,7 +1454,18 @@ __global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets, lo
             if (ok && v < d.nv) {
                 const uint4* base = reinterpret_cast<const uint4*>(pool + d.off + d.vo[v]);
                 const int n4 = (d.vo[v + 1] - d.vo[v]) / 4;
-                if (set_kind(d, v) != GS_SET_HASH) {
+                const int kind = set_kind(d, v);
+                if (kind == GS_SET_BYTEMAP) {  // bytes equal to the tag
+                    const uint32_t t4 = (uint32_t)(d.tag & 0xff) * 0x01010101u;
+                    auto eq = [t4](uint32_t w) {  // bytes of w equal to the tag byte
+                        const uint32_t z = w ^ t4;
+                        return __popc(~(((z & 0x7f7f7f7fu) + 0x7f7f7f7fu) | z | 0x7f7f7f7fu));
+                    };
+                    for (int i = lane; i < n4; i += 64) {
+                        const uint4 w = base[i];
+                        c += eq(w.x) + eq(w.y) + eq(w.z) + eq(w.w);
+                    }
+                } else if (kind == GS_SET_BITMAP) {
                     for (int i = lane; i < n4; i += 64) {
                         const uint4 w = base[i];
                         c += __popc(w.x) + __popc(w.y) + __popc(w.z) + __popc(w.w);
@@ -2025,13 +2050,13 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
 // Descriptor of every potential child key k < K (see gsk::all_key_sets).
 __global__ void k_all_key_sets(const PatDev* __restrict__ pats, int npats, KeySpace ks,
                                long long K, const int* __restrict__ lw,
-                               const int* __restrict__ lcum, int kind, SetDesc* sets) {
+                               const int* __restrict__ lcum, int kind, int tag, SetDesc* sets) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long key = (long long)blockIdx.x * blockDim.x + threadIdx.x; key < K;
          key += stride) {
         const PatDev& P = pats[owner_of(pats, npats, (int)key)];
         SetDesc d;
-        all_key_set(P, (int)(key - P.key_base), ks, lw, lcum, kind, &d);
+        all_key_set(P, (int)(key - P.key_base), ks, lw, lcum, kind, tag, &d);
         sets[key] = d;
     }
 }
@@ -2357,10 +2382,11 @@ hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys
 }
 
 hipError_t all_key_sets(const PatDev* pats, int npats, KeySpace ks, long long K, const int* lw,
-                        const int* lcum, int kind, SetDesc* sets, hipStream_t s) {
+                        const int* lcum, int kind, int tag, SetDesc* sets, hipStream_t s) {
     if (K <= 0) return hipSuccess;
+    if (tag < 1 || tag > 255) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_all_key_sets, dim3(grid_for(K, 256, 16384)), dim3(256), 0, s, pats,
-                       npats, ks, K, lw, lcum, kind, sets);
+                       npats, ks, K, lw, lcum, kind, tag, sets);
     return hipGetLastError();
 }
 

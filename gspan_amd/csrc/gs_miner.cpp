@@ -301,6 +301,11 @@ struct gs_ctx {
     std::vector<int> lcnt;
     // runs of equal (neighbour label, edge label) in adjl
     DBuf d_rstart, d_rkey, d_runid;
+    // bytemap pool of the single-pass last level, kept between steps: tagged_words words,
+    // bytes of the current step equal `tag`
+    DBuf d_tagged;
+    long long tagged_words = 0;
+    int tag = 0;
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
     // runs make the run modes and per-group new-vertex inserts pay at the last level
@@ -1095,7 +1100,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
         const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
         long long tw = 0;
-        if (want_bytes(c) && (size_t)(tw = total_words(true)) * 4 <= avail / 4) {
+        if (want_bytes(c) && ((tw = total_words(true)) <= c->tagged_words ||
+                              (size_t)tw * 4 <= avail / 4)) {
             single = single_bytes = true;
         } else if (c->opts.set_kind != GS_SETS_BYTES &&
                    (size_t)(tw = total_words(false)) * 4 <= avail / 4) {
@@ -1103,18 +1109,42 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         }
         if (single) all_words = tw;
     }
+    // The bytemaps of every key live in the context's tagged pool: each single-pass step
+    // stores a new tag byte (1..255), so the pool is cleared only when it grows or the tag
+    // wraps (clearing 25 GB cost 4.9 ms per run on the R-MAT scale-18 workload).  Bitmaps
+    // take a pool of their own, cleared.
     DBuf d_all_sets, d_all_pool;
+    uint32_t* all_pool = nullptr;
+    int all_tag = 1;
     if (single) {
         DBuf d_lw(P, sizeof(int) * c->VL), d_lcum(P, sizeof(int) * (c->VL + 1));
         h2d(c, d_pats.as<PatDev>(), pd.data(), B);  // with set_base
         h2d(c, d_lw.as<int>(), lw.data(), c->VL);
         h2d(c, d_lcum.as<int>(), lcum.data(), c->VL + 1);
+        if (single_bytes) {
+            if (c->tagged_words < all_words) {
+                c->d_tagged = DBuf();
+                c->d_tagged = DBuf(P, sizeof(uint32_t) * std::max<long long>(all_words, 4));
+                c->tagged_words = all_words;
+                HIPCHECK(gsk::fill_u32(c->d_tagged.as<uint32_t>(), 0u, all_words, c->stream));
+                c->tag = 0;
+            }
+            if (++c->tag > 255) {
+                HIPCHECK(gsk::fill_u32(c->d_tagged.as<uint32_t>(), 0u, c->tagged_words,
+                                       c->stream));
+                c->tag = 1;
+            }
+            all_pool = c->d_tagged.as<uint32_t>();
+            all_tag = c->tag;
+        } else {
+            d_all_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(all_words, 4));
+            HIPCHECK(gsk::fill_u32(d_all_pool.as<uint32_t>(), 0u, all_words, c->stream));
+            all_pool = d_all_pool.as<uint32_t>();
+        }
         d_all_sets = DBuf(P, sizeof(SetDesc) * K);
-        d_all_pool = DBuf(P, sizeof(uint32_t) * std::max<long long>(all_words, 4));
         HIPCHECK(gsk::all_key_sets(d_pats.as<PatDev>(), B, ks, K, d_lw.as<int>(),
                                    d_lcum.as<int>(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP,
-                                   d_all_sets.as<SetDesc>(), c->stream));
-        HIPCHECK(gsk::fill_u32(d_all_pool.as<uint32_t>(), 0u, all_words, c->stream));
+                                   all_tag, d_all_sets.as<SetDesc>(), c->stream));
     }
 
     long long W_total = 0;
@@ -1405,14 +1435,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             HIPCHECK(gsk::items_fill(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
                                      d_woff.as<long long>(), b0, Rc, ks, d_kc.as<int>(), nullptr,
                                      nullptr, nullptr, nullptr, d_all_sets.as<SetDesc>(),
-                                     d_all_pool.as<uint32_t>(), c->overflow_flag(),
+                                     all_pool, c->overflow_flag(),
                                      c->tq, (int)K, saturate,
                                      guard_of(K, K, std::max<long long>(all_words, 4)),
                                      c->stream, by_items));
             time_end(1, &Wc_host[ci]);
             c->st.fill_launches++;
             if (!by_items)
-                newv_pass(Rc, nullptr, K, d_all_sets.as<SetDesc>(), d_all_pool.as<uint32_t>(),
+                newv_pass(Rc, nullptr, K, d_all_sets.as<SetDesc>(), all_pool,
                           std::max<long long>(all_words, 4));
         } else {
             HIPCHECK(gsk::items_count(lc, c->dg, d_pats.as<PatDev>(), d_ctx.as<RowCtx>(),
@@ -1558,7 +1588,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         // supports, then the frequent children in key order
         DBuf d_counts(P, sizeof(int) * (size_t)A * GS_MAXV), d_sup(P, sizeof(int) * A);
         HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), A,
-                                 single ? d_all_pool.as<uint32_t>() : d_pool.as<uint32_t>(),
+                                 single ? all_pool : d_pool.as<uint32_t>(),
                                  std::max<long long>(single ? all_words : pool_words, 4),
                                  d_counts.as<int>(), c->guard_word(), c->stream));
         HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
@@ -1690,7 +1720,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                 const int key = live_keys[li];
                 const int p = owner_of(pd, key);
                 all_key_set(pd[p], key - pd[p].key_base, KeySpace{c->EL, c->VL}, lw.data(),
-                            lcum.data(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP, &d);
+                            lcum.data(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP, all_tag,
+                            &d);
             } else {
                 int lab[GS_MAXV];
                 ch.pat.labels(lab);
@@ -1771,7 +1802,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     }
     account_bytes(emitted);
     HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
-                             single ? d_all_pool.as<uint32_t>() : d_pool.as<uint32_t>(),
+                             single ? all_pool : d_pool.as<uint32_t>(),
                              std::max<long long>(single ? all_words : pool_words, 4),
                              d_counts.as<int>(), c->guard_word(), c->stream));
     std::vector<int> counts((size_t)NL * GS_MAXV);
@@ -2674,6 +2705,7 @@ int gs_destroy(gs_ctx* c) {
     c->d_rstart.reset();
     c->d_rkey.reset();
     c->d_runid.reset();
+    c->d_tagged.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();
