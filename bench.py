@@ -172,6 +172,9 @@ def main() -> int:
                     help="last level by neighbour runs and row groups, or by every neighbour")
     ap.add_argument("--two-pass", action="store_true",
                     help="count every level before filling it (no single-pass last level)")
+    ap.add_argument("--sets", default="auto", choices=("auto", "bits", "bytes"),
+                    help="MNI image sets of the timed method: byte per vertex when it fits, "
+                         "bitmaps, or bytemaps only")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-predicted", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -220,7 +223,7 @@ def main() -> int:
     log(f"graph {args.workload}: {gstats} ({time.perf_counter() - t0:.1f}s)")
 
     with Miner(g, mode=args.timed_mode, timing=True, tile=args.tile, groups=args.groups,
-               two_pass=args.two_pass) as mn:
+               two_pass=args.two_pass, sets=args.sets) as mn:
         for i in range(args.warmup):
             t = time.perf_counter()
             res = mn.mine_raw(k, ms)
@@ -340,7 +343,7 @@ def main() -> int:
                         + f", {w['n_elabels']} elabels, min_sup {ms}, k {k}",
             "downsized_from": w.get("of"),
             "k": k, "min_support": ms, "mode": args.timed_mode, "mni": "gpu",
-            "groups": args.groups, "two_pass": args.two_pass,
+            "groups": args.groups, "two_pass": args.two_pass, "sets": args.sets,
             "graph": gstats,
         },
         "patterns": len(pats),

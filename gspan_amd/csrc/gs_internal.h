@@ -412,8 +412,12 @@ hipError_t gather_frequent(const int* fidx, const int* nf, int max_f, const int*
 hipError_t fill_i32(int* p, int v, long long n, hipStream_t s);
 hipError_t fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t s);
+// rows (nullable): the child count of each set's key (k_items' counts); sets of keys with
+// fewer than min_rows children are not read (their counts read 0): a support never exceeds
+// the child count, so such a key is infrequent at min_sup <= min_rows.
 hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
-                      long long pool_words, int* counts, int* guard_err, hipStream_t s);
+                      long long pool_words, int* counts, int* guard_err, hipStream_t s,
+                      const int* rows = nullptr, int min_rows = 0);
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
                         int mni_mode, const SetDesc* set, uint32_t* pool,
                         long long pool_words, int* guard_err, hipStream_t s);

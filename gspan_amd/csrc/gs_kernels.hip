@@ -1434,19 +1434,27 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n) {
         p[i] = v;
 }
 
-// distinct images per (set, vertex): popcount of a bitmap / bytemap (bytes are 0 or 1), or
-// occupied hash slots.  One wave per set, sets taken grid-stride (a launch holds fewer than
-// 2^32 work-items); each vertex set is read with 16-byte loads.  counts[8 s + v] for all
-// v < 8 (0 past the set's vertices).
+// child count of a key from k_items' 32-bit count (a negative count: more than INT_MAX)
+__device__ __forceinline__ long long child_rows(int c) {
+    return c < 0 ? (long long)INT_MAX + 1 : (long long)c;
+}
+
+// distinct images per (set, vertex): bytes equal to the set's tag of a bytemap, popcount of
+// a bitmap, or occupied hash slots.  One wave per set, sets taken grid-stride (a launch
+// holds fewer than 2^32 work-items); each vertex set is read with 16-byte loads.
+// counts[8 s + v] for all v < 8 (0 past the set's vertices, and for every vertex of a set
+// whose key has fewer than min_rows children when rows is given).
 __global__ __launch_bounds__(GS_BLOCK) void k_set_counts(const SetDesc* sets, long long nsets,
                                                          const uint32_t* pool,
                                                          long long pool_words, int* counts,
-                                                         int* gerr) {
+                                                         int* gerr, const int* rows,
+                                                         int min_rows) {
     constexpr int WPB = GS_BLOCK / 64;  // waves per block
     const int lane = lane_id();
     const long long waves = (long long)gridDim.x * WPB;
     for (long long si = (long long)blockIdx.x * WPB + threadIdx.x / 64; si < nsets; si += waves) {
-        const SetDesc d = sets[si];
+        SetDesc d = sets[si];
+        if (rows && child_rows(rows[si]) < (long long)min_rows) d.nv = 0;
         const bool ok = d.nv == 0 || set_fits<GS_MAXV>(load_set(sets + si), 0, pool_words);
         if (!ok && lane == 0) atomicOr(gerr, GS_GUARD_POOL);
         for (int v = 0; v < GS_MAXV; ++v) {
@@ -1801,6 +1809,129 @@ __device__ int canon_wave(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, in
     }
 }
 
+// is_canonical of a two-edge code (gspan.cu:586-606 with projection_is_canonical,
+// gspan.cu:485-584), walked on the code's pattern graph held in registers: at most three
+// vertices and two edges, i.e. four directed entries.  The walk starts from every entry
+// equal to the least initial entry (label(from) <= label(to), ordered by (from label,
+// edge label, to label), gspan.cu:40-50, 410-426).  One edge is then left and the
+// backward search needs a rightmost path of two edges, so the minimal code's second edge
+// is the least (edge label, to label) first forward edge from an embedding's `to`
+// (gspan.cu:447-462), else the least other forward edge from its `from` under the
+// edge-label rule (gspan.cu:464-483); with neither the walk accepts (gspan.cu:583).
+// Labels are the code's own (vertex v: C.lab[v]).
+__device__ __forceinline__ int canon_two(const CanonCode& C) {
+    const int L0 = C.lab[0], L1 = C.lab[1], L2 = C.lab[2];
+    auto lab = [&](int v) { return v == 0 ? L0 : (v == 1 ? L1 : L2); };
+    auto less3 = [](int a, int e, int b, int xa, int xe, int xb) {
+        return a < xa || (a == xa && (e < xe || (e == xe && b < xb)));
+    };
+    int ef[4], et[4], ee[4];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        ef[2 * j] = C.cf[j];
+        et[2 * j] = C.ct[j];
+        ef[2 * j + 1] = C.ct[j];
+        et[2 * j + 1] = C.cf[j];
+        ee[2 * j] = ee[2 * j + 1] = C.cel[j];
+    }
+    int ba = INT_MAX, be = INT_MAX, bb = INT_MAX;  // least initial entry
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int a = lab(ef[i]), b = lab(et[i]);
+        if (a <= b && less3(a, ee[i], b, ba, be, bb)) {
+            ba = a;
+            be = ee[i];
+            bb = b;
+        }
+    }
+    const int min_label = ba;
+    bool ff = false, of = false;
+    int fe = INT_MAX, fl = INT_MAX;  // least (edge label, to label) found
+    auto take = [&](int el, int tl) {
+        if (el < fe || (el == fe && tl < fl)) {
+            fe = el;
+            fl = tl;
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // first forward edges from `to` of each embedding
+        if (lab(ef[i]) != ba || ee[i] != be || lab(et[i]) != bb) continue;
+        const int u = ef[i], v = et[i];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int w = et[k];
+            if (ef[k] == v && w != u && w != v && lab(w) >= min_label) {
+                ff = true;
+                take(ee[k], lab(w));
+            }
+        }
+    }
+    if (!ff) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {  // other forward edges from `from` of each embedding
+            if (lab(ef[i]) != ba || ee[i] != be || lab(et[i]) != bb) continue;
+            const int u = ef[i], v = et[i], eu = ee[i];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int w = et[k], tl = lab(w);
+                if (ef[k] == u && w != u && w != v && tl >= min_label &&
+                    (eu < ee[k] || (eu == ee[k] && lab(v) <= tl))) {
+                    of = true;
+                    take(ee[k], tl);
+                }
+            }
+        }
+    }
+    if (!ff && !of) return 1;
+    return C.cf[1] == (ff ? 1 : 0) && C.ct[1] == 2 && C.cel[1] == fe && C.ctl[1] == fl;
+}
+
+// Canonicality of the two-edge children of single-edge parents: one thread per active key
+// (the child of slot `slot` as decode_child writes it, for a parent of one edge).
+__global__ __launch_bounds__(GS_BLOCK) void k_canon_two(const PatDev* __restrict__ pats,
+                                                        int npats, KeySpace ks,
+                                                        const int* active_keys,
+                                                        const int* n_active, int* flags) {
+    const long long na = *n_active;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
+        const int key = active_keys[a];
+        int lo = 0, hi = npats - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pats[mid].key_base <= key) lo = mid;
+            else hi = mid - 1;
+        }
+        const PatDev& P = pats[lo];
+        const int slot = key - P.key_base;
+        CanonCode C;
+        C.m = 2;
+        C.cf[0] = P.cfrom[0];
+        C.ct[0] = P.cto[0];
+        C.cel[0] = P.cel[0];
+        C.ctl[0] = P.vlab[P.cto[0]];
+        C.lab[0] = P.vlab[0];
+        C.lab[1] = P.vlab[1];
+        C.lab[2] = 0;
+        const int nb = (P.r - 1) * ks.EL;
+        if (slot < nb) {  // backward: a second edge between the parent's two vertices
+            C.cf[1] = P.v_n;
+            C.ct[1] = P.cfrom[P.rm[slot / ks.EL + 1]];
+            C.cel[1] = slot % ks.EL;
+            C.ctl[1] = -1;
+        } else {
+            const int sl = slot - nb;
+            const int j = sl / (ks.EL * ks.VL), el = (sl / ks.VL) % ks.EL, tl = sl % ks.VL;
+            C.cf[1] = j == 0 ? P.v_n : P.cfrom[P.rm[j - 1]];
+            C.ct[1] = 2;
+            C.cel[1] = el;
+            C.ctl[1] = tl;
+            C.lab[2] = tl;
+        }
+        flags[a] = canon_two(C);
+    }
+}
+
 __device__ void decode_child(const PatDev& Pp, int slot, KeySpace ks, CanonCode& C) {
     C.m = Pp.m + 1;
     for (int j = 0; j < Pp.m; ++j) {
@@ -1884,7 +2015,9 @@ __global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes
         }
         __syncthreads();
         int e = 0;
-        int f = canon_wave<CAP>(C, bufA, bufB, mf, mt, mel, &e);
+        // C.m is uniform: two-edge codes take the register walk (thread 0)
+        const int f = C.m == 2 ? (threadIdx.x == 0 ? canon_two(C) : 0)
+                               : canon_wave<CAP>(C, bufA, bufB, mf, mt, mel, &e);
         if (threadIdx.x == 0) {
             flags[a] = f;
             if (e) atomicOr(err, e);
@@ -1930,10 +2063,6 @@ __device__ __forceinline__ void child_shape(const PatDev& P, int slot, KeySpace 
         if (P.nv < GS_MAXV) lab[P.nv] = (slot - nb) % ks.VL;  // the new vertex's label
     }
     *v0 = from0 || from == 0;
-}
-
-__device__ __forceinline__ long long child_rows(int c) {  // a negative count: > INT_MAX
-    return c < 0 ? (long long)INT_MAX + 1 : (long long)c;
 }
 
 // Per active key a < na: set words and rows of its child (0 unless canonical; no set words
@@ -2007,10 +2136,13 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
     unsigned long long rows = 0, canon = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
-        rows += (unsigned long long)child_rows(acnt[a]);
+        const long long cr = child_rows(acnt[a]);
+        rows += (unsigned long long)cr;
         int out = 0;
-        if (flags[a]) {
-            ++canon;
+        if (flags[a]) ++canon;
+        // a support never exceeds the child count (each child row adds at most one image
+        // to each vertex set)
+        if (flags[a] && cr >= (long long)min_sup) {
             const int key = akeys[a];
             const PatDev& P = pats[owner_of(pats, npats, key)];
             int nv, lab[GS_MAXV];
@@ -2297,6 +2429,11 @@ hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_k
                  const int* n_active, int max_active, int child_m, int* flags, int* err,
                  hipStream_t s) {
     if (max_active <= 0) return hipSuccess;
+    if (child_m == 2) {
+        hipLaunchKernelGGL(k_canon_two, dim3(grid_for(max_active, GS_BLOCK, 4096)), dim3(GS_BLOCK),
+                           0, s, pats, npats, ks, active_keys, n_active, flags);
+        return hipGetLastError();
+    }
     int grid = max_active < 16384 ? max_active : 16384;
     if (child_m > 6)
         hipLaunchKernelGGL(k_canon<GS_CANON_CAP7>, dim3(grid), dim3(64), 0, s, pats, npats, ks,
@@ -2341,10 +2478,12 @@ hipError_t scatter_slots(int* key_slot, const int* keys, int nkeys, hipStream_t 
 }
 
 hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
-                      long long pool_words, int* counts, int* guard_err, hipStream_t s) {
+                      long long pool_words, int* counts, int* guard_err, hipStream_t s,
+                      const int* rows, int min_rows) {
     if (nsets <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_set_counts, dim3(grid_for(nsets, GS_BLOCK / 64, 16384)), dim3(GS_BLOCK),
-                       0, s, sets, (long long)nsets, pool, pool_words, counts, guard_err);
+                       0, s, sets, (long long)nsets, pool, pool_words, counts, guard_err, rows,
+                       min_rows);
     return hipGetLastError();
 }
 

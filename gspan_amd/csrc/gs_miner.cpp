@@ -1590,7 +1590,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), A,
                                  single ? all_pool : d_pool.as<uint32_t>(),
                                  std::max<long long>(single ? all_words : pool_words, 4),
-                                 d_counts.as<int>(), c->guard_word(), c->stream));
+                                 d_counts.as<int>(), c->guard_word(), c->stream,
+                                 d_acnt.as<int>(), c->min_sup));
         HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
         HIPCHECK(gsk::supports(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), d_acnt.as<int>(),
                                d_flags.as<int>(), A, d_counts.as<int>(), c->mni_mode, c->min_sup,
@@ -1801,10 +1802,22 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         }
     }
     account_bytes(emitted);
+    // Mining reads only the sets of children with at least min_sup rows (a support never
+    // exceeds the child count); gs_extend (all_live) wants every child's support.
+    std::vector<int> live_rows;
+    DBuf d_lrows;
+    if (!all_live) {
+        live_rows.reserve(NL);
+        for (const auto& ch : out.children)
+            if (ch.canonical) live_rows.push_back((int)std::min<long long>(ch.count, INT_MAX));
+        d_lrows = DBuf(P, sizeof(int) * std::max(NL, 1));
+        h2d(c, d_lrows.as<int>(), live_rows.data(), NL);
+    }
     HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
                              single ? all_pool : d_pool.as<uint32_t>(),
                              std::max<long long>(single ? all_words : pool_words, 4),
-                             d_counts.as<int>(), c->guard_word(), c->stream));
+                             d_counts.as<int>(), c->guard_word(), c->stream,
+                             all_live ? nullptr : d_lrows.as<int>(), c->min_sup));
     std::vector<int> counts((size_t)NL * GS_MAXV);
     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
     sync(c);
@@ -1812,7 +1825,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     int li = 0;
     for (auto& ch : out.children) {
         if (!(ch.canonical || all_live)) continue;
-        ch.support = support_of(ch.pat, &counts[(size_t)li * GS_MAXV], c->mni_mode);
+        ch.support = !all_live && ch.count < c->min_sup
+                         ? 0
+                         : support_of(ch.pat, &counts[(size_t)li * GS_MAXV], c->mni_mode);
         ch.pat.support = ch.support;
         ch.pat.row_begin = ch.row_off;
         ch.pat.row_count = ch.count;

@@ -43,6 +43,29 @@ def test_canonical_batch_matches_oracle(oracle_mod):
     assert 50 < sum(want) < len(want)
 
 
+def test_canonical_two_edge_codes_exhaustive(oracle_mod):
+    """Every two-edge code over three vertex labels and two edge labels (first edges in
+    both label orders; forward second edges from either vertex, backward second edges)
+    against the oracle: the kernels walk two-edge codes in registers."""
+    _announce("canonical two-edge codes, exhaustive")
+    g = synthetic("er", n=60, m=150, n_vlabels=3, n_elabels=2, seed=2)
+    codes = []
+    for a in range(3):
+        for e0 in range(2):
+            for b in range(3):
+                first = (0, 1, a, e0, b)
+                for e in range(2):
+                    codes.append([first, (1, 0, -1, e, -1)])
+                    for s in range(2):
+                        for l in range(3):
+                            codes.append([first, (s, 2, -1, e, l)])
+    with Miner(g) as mn:
+        got = mn.is_canonical(codes)
+    want = [oracle_mod.is_canonical(c) == 1 for c in codes]
+    assert got == want
+    assert 20 < sum(want) < len(want)
+
+
 def test_canonical_star_projection_count(oracle_mod):
     # K_{1,6} with one label: its minimum code starts at a leaf and walks the other five
     # leaves from the centre; the walk carries up to 6! = 720 projections (cap 2048).
