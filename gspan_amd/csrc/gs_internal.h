@@ -242,6 +242,10 @@ struct DevGraph {
     const int* rkey;
     const int* runid;
     int nruns;
+    // 1 when no vertex has a self-loop or two edges to one neighbour (every adjacency
+    // ascends strictly by neighbour): the only edge between a row's two vertices joined by a
+    // pattern edge is that edge
+    int simple;
 };
 
 // MLP parameters on the device: w32 holds every parameter in fp32 (W1[32x128], b1, W2[128x128],

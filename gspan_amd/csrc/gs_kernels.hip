@@ -303,9 +303,11 @@ __device__ __forceinline__ ItemOut eval_fwd(const DevGraph& g, const PatDev& P, 
 // Forward run f of a parent row: the neighbours of one segment vertex u with one (label,
 // edge label), all children of one key.  The run passes or fails the label and rule tests
 // as a whole (gspan.cu:455, 475-477); of its entries, those whose neighbour is one of the
-// row's own vertices are not children (gspan.cu:453, 473) -- found by binary search, as a
-// run ascends by neighbour, and only for the row's vertices of the run's label.  o.w = the
-// run's children.
+// row's own vertices are not children (gspan.cu:453, 473).  Only the row's vertices of the
+// run's label can be among them, each counted once.  In a simple graph u has no self-loop,
+// and a row vertex joined to u by a pattern edge is in the run exactly when that edge's
+// label is the run's; other row vertices are looked up by binary search, as a run ascends
+// by neighbour.  o.w = the run's children.
 template <bool UNI, int PM>
 __device__ __forceinline__ ItemOut eval_run(const DevGraph& g, const PatDev& P, const RowRegs& c,
                                             int f, KeySpace ks, int* gerr) {
@@ -330,10 +332,31 @@ __device__ __forceinline__ ItemOut eval_run(const DevGraph& g, const PatDev& P, 
         return o;
     }
     int excl = 0;
+    const int sv = pget<UNI, NV>(P.seg_v, j);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         if (v >= P.nv || pget<UNI, NV>(P.vlab, v) != tl) continue;
+        if (g.simple) {
+            if (v == sv) continue;
+            int pel = -1;  // label of the pattern edge joining sv and v, if any
+#pragma unroll
+            for (int k = 0; k < PM; ++k) {
+                if (k >= P.m) continue;
+                const int a = pget<UNI, PM>(P.cfrom, k), b = pget<UNI, PM>(P.cto, k);
+                if ((a == sv && b == v) || (a == v && b == sv)) pel = pget<UNI, PM>(P.cel, k);
+            }
+            if (pel >= 0) {
+                excl += pel == el ? 1 : 0;
+                continue;
+            }
+        }
         const int x = c.phi[v];
+        // a self-loop row maps two pattern vertices to one graph vertex: its entries are
+        // excluded once (gspan.cu:453, 473 test the neighbour against the row's vertex set)
+        bool again = false;
+#pragma unroll
+        for (int w = 0; w < v; ++w) again |= c.phi[w] == x;
+        if (again) continue;
         int lo = st, hi = en;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -348,7 +371,7 @@ __device__ __forceinline__ ItemOut eval_run(const DevGraph& g, const PatDev& P, 
     if (en - st - excl <= 0) return o;
     o.valid = true;
     o.w = en - st - excl;
-    o.sv = pget<UNI, NV>(P.seg_v, j);
+    o.sv = sv;
     o.key = P.key_base + (r - 1) * ks.EL + (j * ks.EL + el) * ks.VL + tl;
     return o;
 }
@@ -792,10 +815,11 @@ __device__ __forceinline__ int wave_find_lds(const int* off, int n, int item) {
 // So a tile costs three or four barriers, and the claim, row search and offset loads of
 // the next tile overlap the work of this one.  NEWV counts nothing: phase 2 only stages.
 //
-// Child counts (key_count) are summed per workgroup in a second LDS table that lives
-// across the tiles the workgroup takes, and added to global memory when it is more than
-// half full, after 2^14 tiles and when the workgroup ends: the counters of a level's hot
-// keys see one atomic per workgroup instead of one per tile (device-scope atomics are
+// Child counts (key_count) are summed per workgroup in a persistent LDS table that lives
+// across the tiles the workgroup takes (ROWS adds its tile table into it in phase 2, the
+// run modes add to it directly in phase 1), and added to global memory when it is more
+// than half full, after 2^14 tiles and when the workgroup ends: the counters of a level's
+// hot keys see one atomic per workgroup instead of one per tile (device-scope atomics are
 // performed at the memory side and serialise on one address).  The run modes count in
 // 64 bits (a run adds up to a vertex degree); ROWS adds at most TILE per tile, so its
 // 32-bit LDS counts stay below 2^14 * TILE <= 2^24.
@@ -824,6 +848,10 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     static_assert(PM >= 1 && PM < GS_MAXE, "parent edges");
     constexpr int LDS_SLOTS = 2 * TILE;
     constexpr int AGG_SLOTS = AGG ? LDS_SLOTS : 1;
+    // ROWS keeps a per-tile key table (its output reservations are per tile); the run modes
+    // count straight into the persistent table, which leaves LDS for more resident waves
+    constexpr bool TTAB = ROWS;
+    constexpr int TT_SLOTS = TTAB ? LDS_SLOTS : 1;
     constexpr int IPT = TILE / GS_BLOCK;
     constexpr int STAGE = (TILE + 1 + GS_BLOCK - 1) / GS_BLOCK;  // offsets per thread
     static_assert(TILE % GS_BLOCK == 0 && (TILE & (TILE - 1)) == 0, "tile: power of two");
@@ -833,8 +861,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     // scanned offsets of the staged tiles' rows relative to the tile's first item, two
     // buffers: the current tile and the next (row 0 may start before the tile: <= 0)
     __shared__ int s_off[2][TILE + 1];
-    __shared__ int s_key[AGG_SLOTS];
-    __shared__ cnt_t s_cnt[AGG_SLOTS];
+    __shared__ int s_key[TT_SLOTS];
+    __shared__ cnt_t s_cnt[TT_SLOTS];
     __shared__ int s_base[ROWS ? LDS_SLOTS : 1];
     // (live slot, row in tile) pairs already seen (ROWS): only their first item inserts the
     // parent's vertex images, which are the same for every item of the pair (a run item is
@@ -854,9 +882,11 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
     if (R <= 0) return;
     if (AGG) {
         for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
-            s_key[h] = -1;
-            s_cnt[h] = 0;
-            if (ROWS) s_pair[h] = -1;
+            if constexpr (TTAB) {
+                s_key[h] = -1;
+                s_cnt[h] = 0;
+                s_pair[h] = -1;
+            }
             s_pkey[h] = -1;
             s_pcnt[h] = 0;
         }
@@ -1047,10 +1077,11 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
                 ins = hs[q] >= 0;
             } else if (act) {
                 // run modes: each item is its row's only one of its key; weighted counts
+                // go straight to the persistent table
                 ins = true;
                 if (RUNS && key_count) {
-                    hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
-                    atomicAdd(&s_cnt[hs[q]], (cnt_t)o.w);
+                    const int ph = lds_slot_fill<LDS_SLOTS>(s_pkey, agg, &s_pfill);
+                    atomicAdd(&s_pcnt[ph], (cnt_t)o.w);
                 }
             }
             if (FILL && ins) {
@@ -1103,8 +1134,8 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             const int i = threadIdx.x + k * GS_BLOCK;
             stage[k] = (ntl >= 0 && i <= nnr) ? woff[nrlo + i] : 0;
         }
-        if (AGG) {
-            if (key_count && threadIdx.x == 0) atomicAdd(&s_pfill, 1 << 16);  // a tile counted
+        if (AGG && key_count && threadIdx.x == 0) atomicAdd(&s_pfill, 1 << 16);  // a tile counted
+        if constexpr (TTAB) {
             for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
                 const int key = s_key[h];
                 if (key < 0) continue;
@@ -1151,13 +1182,13 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             }
             __syncthreads();
         }
-        if (AGG) {
+        if constexpr (TTAB) {
             for (int h = threadIdx.x; h < LDS_SLOTS; h += GS_BLOCK) {
                 if (s_key[h] >= 0) {
                     s_key[h] = -1;
                     s_cnt[h] = 0;
                 }
-                if (ROWS) s_pair[h] = -1;
+                s_pair[h] = -1;
             }
         }
 #pragma unroll

@@ -2151,6 +2151,18 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->n = n;
         x->nnz = nnz;
         for (int v = 0; v < n; ++v) x->max_deg = std::max(x->max_deg, offsets[v + 1] - offsets[v]);
+        // simple graph: each adjacency ascends strictly by neighbour and skips its vertex
+        x->dg.simple = 1;
+        for (int v = 0; v < n && x->dg.simple; ++v) {
+            for (int i = offsets[v], prev = -1; i < offsets[v + 1]; ++i) {
+                const int to = edges3[3 * (long long)i + 1];
+                if (to == v || to <= prev) {
+                    x->dg.simple = 0;
+                    break;
+                }
+                prev = to;
+            }
+        }
         LabelRanks vranks, eranks;
         vranks.build(vlabels, n, 1);
         eranks.build(edges3 + 2, nnz, 3);

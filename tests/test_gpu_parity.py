@@ -398,6 +398,35 @@ def test_large_key_space(oracle_mod):
             assert mn.stats()["extended"] == st["extended"], mode
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_multigraphs_all_modes(oracle_mod, seed):
+    """Edge lists with repeated pairs (same or other edge label) and self-loops, which the
+    reference's CSR keeps (graph.cc:32-36): such a graph is not simple, so a neighbour run's
+    row vertices are looked up in the run.  Every method against the oracle."""
+    rng = np.random.default_rng(7100 + seed)
+    n = int(rng.integers(20, 80))
+    m = int(rng.integers(2 * n, 5 * n))
+    loops = rng.integers(0, n, size=max(1, n // 10))
+    u = np.concatenate([rng.integers(0, n, size=m), loops])
+    v = np.concatenate([rng.integers(0, n, size=m), loops])
+    el = rng.integers(0, int(rng.integers(1, 3)), size=u.shape[0])
+    vl = rng.integers(0, int(rng.integers(1, 4)), size=n)
+    g = csr_from_undirected(n, u, v, el, vl)
+    k = int(rng.integers(2, 4))
+    ms = int(rng.integers(2, 8))
+    mni = int(rng.integers(0, 2))
+    _announce(f"multigraph seed={seed} n={n} m={u.shape[0]} k={k} ms={ms} mni={mni}")
+    want, st = oracle_mod.fsm(g, k, ms, mni_mode=mni)
+    for mode, sets, groups, two in (("batched", "auto", "on", False),
+                                    ("batched", "auto", "off", False),
+                                    ("plain", "auto", "on", False),
+                                    ("batched", "bits", "on", True)):
+        with Miner(g, mode=mode, mni=("gpu", "omp-literal")[mni], sets=sets, groups=groups,
+                   two_pass=two) as mn:
+            assert mn.mine(k, ms) == want, (mode, sets, groups, two)
+            assert mn.stats()["extended"] == st["extended"], (mode, sets, groups, two)
+
+
 @pytest.mark.parametrize("seed", range(12))
 def test_random_graphs_all_modes(oracle_mod, seed):
     """Seeded ER and small R-MAT graphs with random label counts, k, threshold and MNI
