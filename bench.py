@@ -172,6 +172,9 @@ def main() -> int:
                     help="last level by neighbour runs and row groups, or by every neighbour")
     ap.add_argument("--two-pass", action="store_true",
                     help="count every level before filling it (no single-pass last level)")
+    ap.add_argument("--two-edge", default="auto", choices=("auto", "off"),
+                    help="two-edge last level from per-vertex neighbour runs (simple graphs), "
+                         "or by extending the parent rows")
     ap.add_argument("--sets", default="auto", choices=("auto", "bits", "bytes"),
                     help="MNI image sets of the timed method: byte per vertex when it fits, "
                          "bitmaps, or bytemaps only")
@@ -223,7 +226,7 @@ def main() -> int:
     log(f"graph {args.workload}: {gstats} ({time.perf_counter() - t0:.1f}s)")
 
     with Miner(g, mode=args.timed_mode, timing=True, tile=args.tile, groups=args.groups,
-               two_pass=args.two_pass, sets=args.sets) as mn:
+               two_pass=args.two_pass, sets=args.sets, two_edge=args.two_edge) as mn:
         for i in range(args.warmup):
             t = time.perf_counter()
             res = mn.mine_raw(k, ms)
@@ -344,6 +347,7 @@ def main() -> int:
             "downsized_from": w.get("of"),
             "k": k, "min_support": ms, "mode": args.timed_mode, "mni": "gpu",
             "groups": args.groups, "two_pass": args.two_pass, "sets": args.sets,
+            "two_edge": args.two_edge,
             "graph": gstats,
         },
         "patterns": len(pats),

@@ -35,6 +35,8 @@ GS_SETS_BYTES = 2
 GS_GROUPS_AUTO = 0
 GS_GROUPS_ON = 1
 GS_GROUPS_OFF = 2
+GS_TWO_EDGE_AUTO = 0
+GS_TWO_EDGE_OFF = 1
 GS_MAX_PATTERN_EDGES = 7
 GS_NOALLOC_MAX_CODES = 8192
 
@@ -52,6 +54,7 @@ class GsOpts(ctypes.Structure):
         ("autotune", ctypes.c_int),
         ("tile", ctypes.c_int),
         ("groups", ctypes.c_int),
+        ("two_edge", ctypes.c_int),
     ]
 
 

@@ -47,6 +47,7 @@ def build_library(quiet: bool = False) -> str:
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, "gs_kernels.hip"), os.path.join(CSRC, "gs_mlp.hip"),
             os.path.join(CSRC, "gs_ingest.hip"), os.path.join(CSRC, "gs_noalloc.hip"),
+            os.path.join(CSRC, "gs_twoedge.hip"),
             os.path.join(CSRC, "gs_miner.cpp")]
     _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", *srcs, "-o", tmp], quiet)
     os.replace(tmp, out)

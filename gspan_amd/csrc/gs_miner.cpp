@@ -306,6 +306,10 @@ struct gs_ctx {
     DBuf d_tagged;
     long long tagged_words = 0;
     int tag = 0;
+    // two-edge last level: run masks M1 / M2 (te_W words per vertex) and the (vertex, run)
+    // pairs ordered by class, built on first use (they depend on the graph only)
+    DBuf d_te_m1, d_te_m2, d_te_pairs;
+    bool te_ready = false;
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
     // runs make the run modes and per-group new-vertex inserts pay at the last level
@@ -1097,7 +1101,16 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         }
         return tw;
     };
-    if (!write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
+    // Two-edge last level: children of single-edge parents on a simple graph take their
+    // child rows and image counts from per-vertex neighbour runs (gsk::te_level), with no
+    // pass over the parent rows and no image sets.  The three count tables hold
+    // VL * (VL * EL)^2 entries of 16 bytes, at most 1 GiB.
+    const long long te_nb = (long long)c->VL * c->EL;
+    const bool two_edge = !write_rows && !all_live && m == 1 && c->opts.mode != GS_MODE_PLAIN &&
+                          !c->opts.two_pass && c->opts.two_edge != GS_TWO_EDGE_OFF &&
+                          c->dg.simple && te_nb <= 1024 &&
+                          (long long)c->VL * te_nb * te_nb <= (1LL << 26);
+    if (!two_edge && !write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
         const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
         long long tw = 0;
         if (want_bytes(c) && ((tw = total_words(true)) <= c->tagged_words ||
@@ -1146,6 +1159,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                    d_lcum.as<int>(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP,
                                    all_tag, d_all_sets.as<SetDesc>(), c->stream));
     }
+    if (two_edge) single = true;  // one pass, no fill pass below; supports from d_te_img
+    DBuf d_te_img;                  // image counts of every key, 8 per key
+    DBuf d_te_work;                 // work of the run pass (runs visited, mask words read)
+    unsigned long long te_work[2] = {0, 0};
 
     long long W_total = 0;
     unsigned long long W_full = 0;
@@ -1423,7 +1440,44 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     }
 
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
-    for (int ci = 0; ci < nchunks; ++ci) {
+    if (two_edge) {
+        const int NB = (int)te_nb, W = (NB + 31) / 32;
+        if (!c->te_ready) {
+            const long long NC = (long long)c->VL * NB;
+            c->d_te_m1 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
+            c->d_te_m2 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
+            c->d_te_pairs = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
+            const long long te = gsk::scan_tmp_elems(NC);
+            DBuf hist(P, 4 * NC), cst(P, 8 * (NC + 1)), tmp(P, 8 * te);
+            HIPCHECK(gsk::te_prepare(c->dg, c->EL, NB, W, c->d_te_m1.as<uint32_t>(),
+                                     c->d_te_m2.as<uint32_t>(), hist.as<int>(), cst.as<long long>(),
+                                     tmp.as<long long>(), te, c->d_te_pairs.as<int2>(),
+                                     c->stream));
+            c->te_ready = true;
+        }
+        std::vector<int> p_of((size_t)c->VL * c->EL * c->VL, -1);
+        for (int i = 0; i < B; ++i)
+            p_of[((size_t)pd[i].vlab[0] * c->EL + pd[i].cel[0]) * c->VL + pd[i].vlab[1]] = i;
+        DBuf d_pof(P, sizeof(int) * p_of.size());
+        h2d(c, d_pof.as<int>(), p_of.data(), p_of.size());
+        const long long rows = (long long)c->VL * NB * NB;
+        DBuf d_cnt(P, 8 * rows), d_cen(P, 4 * rows), d_end(P, 4 * rows), d_q(P, 4);
+        d_te_img = DBuf(P, sizeof(int) * 8 * K);
+        d_te_work = DBuf(P, 2 * sizeof(unsigned long long));
+        HIPCHECK(gsk::fill_i32(d_te_work.as<int>(), 0, 4, c->stream));
+        Wc_host[0] = c->dg.nruns;  // (vertex, run) pairs
+        time_begin();
+        HIPCHECK(gsk::te_level(c->dg, ks, d_pats.as<PatDev>(), B, K, d_pof.as<int>(),
+                               c->d_te_m1.as<uint32_t>(), c->d_te_m2.as<uint32_t>(),
+                               c->d_te_pairs.as<int2>(), d_q.as<int>(),
+                               d_cnt.as<unsigned long long>(), d_cen.as<int>(), d_end.as<int>(),
+                               d_kc.as<int>(), d_te_img.as<int>(),
+                               d_te_work.as<unsigned long long>(), c->stream));
+        time_end(1, &Wc_host[0]);
+        c->st.fill_launches++;
+        d2h(c, te_work, d_te_work.as<unsigned long long>(), 2);
+    }
+    for (int ci = 0; ci < (two_edge ? 0 : nchunks); ++ci) {
         const long long b0 = (long long)ci * chunk;
         const bool by_items = single && !grouped;  // every neighbour of every row
         const long long Rc = prep_chunk(ci, true, !by_items);
@@ -1490,6 +1544,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
     auto account_bytes = [&](long long emitted) {
+        if (two_edge) {  // per pair: its record, run and label (24 B); per run visited 12 B;
+                         // per mask word 4 B
+            c->st.alg_bytes_fill += 24.0 * (double)c->dg.nruns + 12.0 * (double)te_work[0] +
+                                    4.0 * (double)te_work[1];
+            return;
+        }
         double bytes = 16.0 * (double)W_full;
         for (int i = 0; i < B; ++i)
             bytes += (double)pats[i].row_count *
@@ -1587,11 +1647,15 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         }
         // supports, then the frequent children in key order
         DBuf d_counts(P, sizeof(int) * (size_t)A * GS_MAXV), d_sup(P, sizeof(int) * A);
-        HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), A,
-                                 single ? all_pool : d_pool.as<uint32_t>(),
-                                 std::max<long long>(single ? all_words : pool_words, 4),
-                                 d_counts.as<int>(), c->guard_word(), c->stream,
-                                 d_acnt.as<int>(), c->min_sup));
+        if (two_edge)
+            HIPCHECK(gsk::te_gather(d_te_img.as<int>(), d_active.as<int>(), c->n_active(), A,
+                                    d_counts.as<int>(), c->stream));
+        else
+            HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), A,
+                                     single ? all_pool : d_pool.as<uint32_t>(),
+                                     std::max<long long>(single ? all_words : pool_words, 4),
+                                     d_counts.as<int>(), c->guard_word(), c->stream,
+                                     d_acnt.as<int>(), c->min_sup));
         HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
         HIPCHECK(gsk::supports(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), d_acnt.as<int>(),
                                d_flags.as<int>(), A, d_counts.as<int>(), c->mni_mode, c->min_sup,
@@ -1717,7 +1781,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         for (int li = 0; li < NL; ++li) {
             SetDesc d;
             const ChildInfo& ch = out.children[live_child[li]];
-            if (single) {
+            if (two_edge) {
+                d = SetDesc{};  // no sets: image counts come from the run tables
+            } else if (single) {
                 const int key = live_keys[li];
                 const int p = owner_of(pd, key);
                 all_key_set(pd[p], key - pd[p].key_base, KeySpace{c->EL, c->VL}, lw.data(),
@@ -1813,11 +1879,19 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         d_lrows = DBuf(P, sizeof(int) * std::max(NL, 1));
         h2d(c, d_lrows.as<int>(), live_rows.data(), NL);
     }
-    HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
-                             single ? all_pool : d_pool.as<uint32_t>(),
-                             std::max<long long>(single ? all_words : pool_words, 4),
-                             d_counts.as<int>(), c->guard_word(), c->stream,
-                             all_live ? nullptr : d_lrows.as<int>(), c->min_sup));
+    DBuf d_lkeys;
+    if (two_edge) {
+        d_lkeys = DBuf(P, sizeof(int) * std::max(NL, 1));
+        h2d(c, d_lkeys.as<int>(), live_keys.data(), NL);
+        HIPCHECK(gsk::te_gather(d_te_img.as<int>(), d_lkeys.as<int>(), nullptr, NL,
+                                d_counts.as<int>(), c->stream));
+    } else {
+        HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
+                                 single ? all_pool : d_pool.as<uint32_t>(),
+                                 std::max<long long>(single ? all_words : pool_words, 4),
+                                 d_counts.as<int>(), c->guard_word(), c->stream,
+                                 all_live ? nullptr : d_lrows.as<int>(), c->min_sup));
+    }
     std::vector<int> counts((size_t)NL * GS_MAXV);
     d2h(c, counts.data(), d_counts.as<int>(), counts.size());
     sync(c);
@@ -2733,6 +2807,9 @@ int gs_destroy(gs_ctx* c) {
     c->d_rkey.reset();
     c->d_runid.reset();
     c->d_tagged.reset();
+    c->d_te_m1.reset();
+    c->d_te_m2.reset();
+    c->d_te_pairs.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();

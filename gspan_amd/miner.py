@@ -43,7 +43,7 @@ class Miner:
     def __init__(self, g: CSRGraph, mode: str = "batched", mni: str = "gpu", device: int = 0,
                  timing: bool = False, budget_bytes: int = 0, chunk_rows: int = 0,
                  two_pass: bool = False, sets: str = "auto", autotune: bool = True,
-                 tile: int = 0, groups: str = "auto"):
+                 tile: int = 0, groups: str = "auto", two_edge: str = "auto"):
         self._lib = L.lib()
         opts = L.GsOpts()
         self._lib.gs_default_opts(ctypes.byref(opts))
@@ -61,6 +61,7 @@ class Miner:
         opts.tile = tile
         opts.groups = {"auto": L.GS_GROUPS_AUTO, "on": L.GS_GROUPS_ON,
                        "off": L.GS_GROUPS_OFF}[groups]
+        opts.two_edge = {"auto": L.GS_TWO_EDGE_AUTO, "off": L.GS_TWO_EDGE_OFF}[two_edge]
         self.mni_mode = opts.mni_mode
         self.graph = g
         h = ctypes.c_void_p()

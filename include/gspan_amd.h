@@ -72,6 +72,12 @@ typedef struct gs_opts {
                                         rows sharing a segment vertex where the graph's
                                         runs are long, else scans every neighbour of
                                         every row; GS_GROUPS_ON / GS_GROUPS_OFF force it */
+    int two_edge;                    /* last level of two-edge children (k = 2): on a
+                                        graph with no self-loops or repeated neighbours,
+                                        GS_TWO_EDGE_AUTO takes child rows and image counts
+                                        from per-vertex neighbour runs (batched and
+                                        predicted methods); GS_TWO_EDGE_OFF extends the
+                                        parent rows as every other level                */
 } gs_opts;
 
 #define GS_SETS_AUTO 0
@@ -80,6 +86,8 @@ typedef struct gs_opts {
 #define GS_GROUPS_AUTO 0
 #define GS_GROUPS_ON 1
 #define GS_GROUPS_OFF 2
+#define GS_TWO_EDGE_AUTO 0
+#define GS_TWO_EDGE_OFF 1
 
 typedef struct gs_stats {
     long long extended;     /* child rows emitted by extension (gspan.cu:708,717,730);   */

@@ -129,12 +129,13 @@ def test_scale16_k2_closed_form(cfg, ms):
     _announce(f"{cfg} at R-MAT scale 16, k=2, min_sup {ms}: closed form")
     ext, seed_sup = _closed_form_extended_k2(g, ms)
     res = {}
-    for mode in ("batched", "plain"):
-        with Miner(g, mode=mode) as mn:
-            res[mode] = (mn.mine(2, ms), mn.stats())
-    pats, st = res["batched"]
-    assert pats == res["plain"][0]
-    assert st["extended"] == ext == res["plain"][1]["extended"]
+    for mode, te in (("batched", "auto"), ("batched", "off"), ("plain", "auto")):
+        with Miner(g, mode=mode, two_edge=te) as mn:
+            res[mode, te] = (mn.mine(2, ms), mn.stats())
+    pats, st = res["batched", "auto"]
+    for r in res.values():
+        assert r[0] == pats
+        assert r[1]["extended"] == ext
     nl, _, EL, _, _ = SPECS[cfg]
     label_count = np.bincount(g.vlabels, minlength=nl)
     parent = {}

@@ -448,12 +448,13 @@ def test_random_graphs_all_modes(oracle_mod, seed):
     mni = int(rng.integers(0, 2))
     _announce(f"random graph seed={seed} k={k} ms={ms} mni={mni}")
     want, st = oracle_mod.fsm(g, k, ms, mni_mode=mni)
-    for mode, sets, groups, two in (("batched", "auto", "auto", False),
-                                    ("plain", "auto", "on", False),
-                                    ("batched", "bits", "off", False),
-                                    ("batched", "auto", "on", True),
-                                    ("plain", "auto", "off", True)):
+    for mode, sets, groups, two, te in (("batched", "auto", "auto", False, "auto"),
+                                        ("batched", "auto", "auto", False, "off"),
+                                        ("plain", "auto", "on", False, "auto"),
+                                        ("batched", "bits", "off", False, "off"),
+                                        ("batched", "auto", "on", True, "auto"),
+                                        ("plain", "auto", "off", True, "auto")):
         with Miner(g, mode=mode, mni=("gpu", "omp-literal")[mni], sets=sets, groups=groups,
-                   two_pass=two) as mn:
-            assert mn.mine(k, ms) == want, (mode, sets, groups, two)
-            assert mn.stats()["extended"] == st["extended"], (mode, sets, groups, two)
+                   two_pass=two, two_edge=te) as mn:
+            assert mn.mine(k, ms) == want, (mode, sets, groups, two, te)
+            assert mn.stats()["extended"] == st["extended"], (mode, sets, groups, two, te)
