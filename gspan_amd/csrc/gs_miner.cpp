@@ -310,6 +310,9 @@ struct gs_ctx {
     // pairs ordered by class, built on first use (they depend on the graph only)
     DBuf d_te_m1, d_te_m2, d_te_pairs;
     bool te_ready = false;
+    // per class (vertex label * VL * EL + run bit): first pair (te_cstart, VL * VL * EL + 1
+    // entries) and CSR entries of its runs (te_clen), on the host
+    std::vector<long long> te_cstart, te_clen;
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
     // runs make the run modes and per-group new-vertex inserts pay at the last level
@@ -1018,6 +1021,40 @@ long long predicted_capacity(float y) {
     return (long long)std::min(std::ceil(1.25 * pred) + 16.0, kMax);
 }
 
+// The graph and options allow the two-edge last level (gs_twoedge.hip): a simple graph,
+// the batched or predicted method, no forced count-then-fill, and count tables of
+// VL * (VL * EL)^2 entries (16 bytes each) of at most 1 GiB.
+bool two_edge_graph(const gs_ctx* c) {
+    const long long nb = (long long)c->VL * c->EL;
+    return c->opts.mode != GS_MODE_PLAIN && !c->opts.two_pass &&
+           c->opts.two_edge != GS_TWO_EDGE_OFF && c->dg.simple && nb <= 1024 &&
+           (long long)c->VL * nb * nb <= (1LL << 26);
+}
+
+// Run masks, class-ordered (vertex, run) pairs and the per-class pair starts and entry
+// counts of the graph, built once per context.
+void te_build(gs_ctx* c) {
+    if (c->te_ready) return;
+    DevPool* P = &c->pool;
+    const int NB = c->VL * c->EL, W = (NB + 31) / 32;
+    const long long NC = (long long)c->VL * NB;
+    c->d_te_m1 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
+    c->d_te_m2 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
+    c->d_te_pairs = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
+    const long long te = gsk::scan_tmp_elems(NC);
+    DBuf hist(P, 4 * NC), cst(P, 8 * (NC + 1)), clen(P, 8 * NC), tmp(P, 8 * te);
+    HIPCHECK(gsk::te_prepare(c->dg, c->EL, NB, W, c->d_te_m1.as<uint32_t>(),
+                             c->d_te_m2.as<uint32_t>(), hist.as<int>(), cst.as<long long>(),
+                             clen.as<unsigned long long>(), tmp.as<long long>(), te,
+                             c->d_te_pairs.as<int2>(), c->stream));
+    c->te_cstart.resize(NC + 1);
+    c->te_clen.resize(NC);
+    d2h(c, c->te_cstart.data(), cst.as<long long>(), NC + 1);
+    d2h(c, c->te_clen.data(), clen.as<long long>(), NC);
+    sync(c);
+    c->te_ready = true;
+}
+
 // One extension step of a batch of size-m patterns.  write_rows: keep child rows (needed
 // below the last level); all_live: write rows and sets of non-canonical children too.
 int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>& layers,
@@ -1050,10 +1087,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     }
     const KeySpace ks{c->EL, c->VL};
 
+    // Two-edge last level: children of single-edge parents take their child rows and image
+    // counts from per-vertex neighbour runs (gsk::te_level), with no pass over the parent
+    // rows and no image sets.
+    const bool two_edge = !write_rows && !all_live && m == 1 && two_edge_graph(c);
     // parent rows per pass (one prep thread per row: launches stay far below 2^32 items)
     const long long chunk =
         c->opts.chunk_rows > 0 ? std::min(c->opts.chunk_rows, 1LL << 26) : (16LL << 20);
-    const long long crow = std::min(R, chunk);
+    const long long crow = two_edge ? 1 : std::min(R, chunk);
     const int nchunks = (int)((R + chunk - 1) / chunk);
     DBuf d_ctx(P, sizeof(RowCtx) * std::max<long long>(crow, 1));
     DBuf d_work(P, sizeof(int) * std::max<long long>(crow, 1));
@@ -1101,15 +1142,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         }
         return tw;
     };
-    // Two-edge last level: children of single-edge parents on a simple graph take their
-    // child rows and image counts from per-vertex neighbour runs (gsk::te_level), with no
-    // pass over the parent rows and no image sets.  The three count tables hold
-    // VL * (VL * EL)^2 entries of 16 bytes, at most 1 GiB.
     const long long te_nb = (long long)c->VL * c->EL;
-    const bool two_edge = !write_rows && !all_live && m == 1 && c->opts.mode != GS_MODE_PLAIN &&
-                          !c->opts.two_pass && c->opts.two_edge != GS_TWO_EDGE_OFF &&
-                          c->dg.simple && te_nb <= 1024 &&
-                          (long long)c->VL * te_nb * te_nb <= (1LL << 26);
     if (!two_edge && !write_rows && !all_live && !c->opts.two_pass && K < (1 << 21)) {
         const size_t avail = c->budget > c->pool.in_use() ? c->budget - c->pool.in_use() : 0;
         long long tw = 0;
@@ -1441,20 +1474,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
 
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
     if (two_edge) {
-        const int NB = (int)te_nb, W = (NB + 31) / 32;
-        if (!c->te_ready) {
-            const long long NC = (long long)c->VL * NB;
-            c->d_te_m1 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
-            c->d_te_m2 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
-            c->d_te_pairs = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
-            const long long te = gsk::scan_tmp_elems(NC);
-            DBuf hist(P, 4 * NC), cst(P, 8 * (NC + 1)), tmp(P, 8 * te);
-            HIPCHECK(gsk::te_prepare(c->dg, c->EL, NB, W, c->d_te_m1.as<uint32_t>(),
-                                     c->d_te_m2.as<uint32_t>(), hist.as<int>(), cst.as<long long>(),
-                                     tmp.as<long long>(), te, c->d_te_pairs.as<int2>(),
-                                     c->stream));
-            c->te_ready = true;
-        }
+        const int NB = (int)te_nb;
+        te_build(c);
         std::vector<int> p_of((size_t)c->VL * c->EL * c->VL, -1);
         for (int i = 0; i < B; ++i)
             p_of[((size_t)pd[i].vlab[0] * c->EL + pd[i].cel[0]) * c->VL + pd[i].vlab[1]] = i;
@@ -1971,7 +1992,53 @@ void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) 
 
 // Single-edge codes (gspan.cu:1176-1190): counted, sized and filled in two passes over
 // the CSR entries; every code of size 1 is canonical (gspan.cu:592-593).
+// Mining to k = 2 with the two-edge last level reads no single-edge rows, and each
+// single-edge code's rows and images follow from the graph's run classes: code (a, e0, b),
+// a <= b, has as many rows as CSR entries from label a to label b over e0 (gspan.cu:410-426),
+// the vertices of label a with a run (b, e0) as vertex-0 images and those of label b with a
+// run (a, e0) as vertex-1 images.  Codes in key order, as mine_seeds reports them.
+void seeds_from_runs(gs_ctx* c) {
+    te_build(c);
+    const int VL = c->VL, EL = c->EL, NB = VL * EL;
+    auto pairs_of = [&](long long cl) { return c->te_cstart[cl + 1] - c->te_cstart[cl]; };
+    std::vector<HPat> freq;
+    long long codes = 0, rows = 0;
+    for (int a = 0; a < VL; ++a) {
+        for (int e0 = 0; e0 < EL; ++e0) {
+            for (int b = a; b < VL; ++b) {
+                const long long cl = (long long)a * NB + (long long)b * EL + e0;
+                const long long cnt = c->te_clen[cl];
+                if (cnt == 0) continue;
+                ++codes;
+                HPat h;
+                h.m = 1;
+                h.f[0] = 0;
+                h.t[0] = 1;
+                h.fl0 = a;
+                h.el[0] = e0;
+                h.tl[0] = b;
+                const long long n1 = pairs_of((long long)b * NB + (long long)a * EL + e0);
+                int counts[GS_MAXV] = {(int)pairs_of(cl), (int)n1};
+                h.support = support_of(h, counts, c->mni_mode);
+                if (h.support < c->min_sup) continue;
+                h.row_begin = rows;
+                h.row_count = cnt;
+                rows += cnt;
+                h.report_id = add_report(c, h);
+                freq.push_back(h);
+            }
+        }
+    }
+    c->st.candidates += codes;
+    c->st.canonical += codes;
+    if (!freq.empty()) {
+        std::vector<Layer> layers{Layer{nullptr, rows}};
+        mine_level(c, freq, layers);
+    }
+}
+
 void mine_seeds(gs_ctx* c) {
+    if (c->k == 2 && two_edge_graph(c)) return seeds_from_runs(c);
     DevPool* P = &c->pool;
     const long long K = (long long)c->VL * c->EL * c->VL;
     if (K > (1LL << 30)) throw GsError(GS_ERR_LIMIT, "too many labels for the seed key space");

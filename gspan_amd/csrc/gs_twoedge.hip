@@ -68,10 +68,11 @@ __global__ void k_te_masks(DevGraph g, int EL, int W, uint32_t* __restrict__ m1,
     }
 }
 
-// pairs per class (hist), then the pairs at their class positions (cursor = class starts,
-// advanced); one thread per vertex over its runs
+// pairs and CSR entries per class (hist, clen), then the pairs at their class positions
+// (cursor = class starts, advanced); one thread per vertex over its runs
 template <bool SCATTER>
 __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or_cursor,
+                           unsigned long long* __restrict__ clen,
                            const long long* __restrict__ cstart, int2* __restrict__ pairs) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += stride) {
@@ -83,6 +84,7 @@ __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or
             const long long c = cbase + run_bit(g.rkey[k], EL);
             if (!SCATTER) {
                 atomicAdd(&hist_or_cursor[c], 1);
+                atomicAdd(&clen[c], (unsigned long long)(g.rstart[k + 1] - g.rstart[k]));
             } else {
                 const long long at = cstart[c] + atomicAdd(&hist_or_cursor[c], 1);
                 pairs[at] = make_int2((int)v, k);
@@ -257,21 +259,22 @@ namespace gsk {
 static_assert(GS_MAXV == 8, "image counts are 8 per key");
 
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
-                      int* hist, long long* cstart, long long* scan_tmp, long long scan_elems,
-                      int2* pairs, hipStream_t s) {
+                      int* hist, long long* cstart, unsigned long long* clen,
+                      long long* scan_tmp, long long scan_elems, int2* pairs, hipStream_t s) {
     const long long NC = (long long)(NB / EL) * NB;  // VL * NB classes
     hipError_t e;
     if ((e = fill_u32(m1, 0u, (long long)g.n * W, s)) != hipSuccess) return e;
     if ((e = fill_u32(m2, 0u, (long long)g.n * W, s)) != hipSuccess) return e;
     if ((e = fill_i32(hist, 0, NC, s)) != hipSuccess) return e;
+    if ((e = fill_u32(reinterpret_cast<uint32_t*>(clen), 0u, 2 * NC, s)) != hipSuccess) return e;
     const int grid = te_grid(g.n, 256, 8192);
     hipLaunchKernelGGL(k_te_masks, dim3(grid), dim3(256), 0, s, g, EL, W, m1, m2);
-    hipLaunchKernelGGL(k_te_pairs<false>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist,
+    hipLaunchKernelGGL(k_te_pairs<false>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist, clen,
                        (const long long*)nullptr, (int2*)nullptr);
     if ((e = scan_exclusive(hist, cstart, NC, scan_tmp, scan_elems, s)) != hipSuccess) return e;
     if ((e = fill_i32(hist, 0, NC, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_te_pairs<true>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist, cstart,
-                       pairs);
+    hipLaunchKernelGGL(k_te_pairs<true>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist,
+                       (unsigned long long*)nullptr, cstart, pairs);
     return hipGetLastError();
 }
 
