@@ -225,8 +225,12 @@ def main() -> int:
     gstats = g.stats()
     log(f"graph {args.workload}: {gstats} ({time.perf_counter() - t0:.1f}s)")
 
+    t_create = time.perf_counter()
     with Miner(g, mode=args.timed_mode, timing=True, tile=args.tile, groups=args.groups,
                two_pass=args.two_pass, sets=args.sets, two_edge=args.two_edge) as mn:
+        # gs_create: device copy and indexes of the graph (label-ordered adjacency, runs,
+        # two-edge run masks and pairs), before any step
+        create_ms = (time.perf_counter() - t_create) * 1e3
         for i in range(args.warmup):
             t = time.perf_counter()
             res = mn.mine_raw(k, ms)
@@ -352,6 +356,7 @@ def main() -> int:
         },
         "patterns": len(pats),
         "extended_per_step": ext_per_step,
+        "create_ms": round(create_ms, 1),
         "mining_wall_s": total_s / steps,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

@@ -308,11 +308,10 @@ struct gs_ctx {
     int tag = 0;
     // two-edge last level: run masks M1 / M2 (te_W words per vertex) and the (vertex, run)
     // pairs ordered by class, built on first use (they depend on the graph only)
-    DBuf d_te_m1, d_te_m2, d_te_pairs;
+    // and the first pair of every class (vertex label * VL * EL + run bit; VL * VL * EL + 1
+    // entries); built by gs_create
+    DBuf d_te_m1, d_te_m2, d_te_pairs, d_te_cstart;
     bool te_ready = false;
-    // per class (vertex label * VL * EL + run bit): first pair (te_cstart, VL * VL * EL + 1
-    // entries) and CSR entries of its runs (te_clen), on the host
-    std::vector<long long> te_cstart, te_clen;
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
     // runs make the run modes and per-group new-vertex inserts pay at the last level
@@ -1031,8 +1030,9 @@ bool two_edge_graph(const gs_ctx* c) {
            (long long)c->VL * nb * nb <= (1LL << 26);
 }
 
-// Run masks, class-ordered (vertex, run) pairs and the per-class pair starts and entry
-// counts of the graph, built once per context.
+// The graph's index for the two-edge last level -- run masks, class-ordered (vertex, run)
+// pairs and class starts -- built by gs_create, beside the label-ordered adjacency and
+// runs it derives from.
 void te_build(gs_ctx* c) {
     if (c->te_ready) return;
     DevPool* P = &c->pool;
@@ -1040,17 +1040,14 @@ void te_build(gs_ctx* c) {
     const long long NC = (long long)c->VL * NB;
     c->d_te_m1 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
     c->d_te_m2 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
-    c->d_te_pairs = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
+    c->d_te_pairs = DBuf(P, 2 * sizeof(int4) * std::max(c->dg.nruns, 1));
+    c->d_te_cstart = DBuf(P, 8 * (NC + 1));
     const long long te = gsk::scan_tmp_elems(NC);
-    DBuf hist(P, 4 * NC), cst(P, 8 * (NC + 1)), clen(P, 8 * NC), tmp(P, 8 * te);
+    DBuf hist(P, 4 * NC), tmp(P, 8 * te);
     HIPCHECK(gsk::te_prepare(c->dg, c->EL, NB, W, c->d_te_m1.as<uint32_t>(),
-                             c->d_te_m2.as<uint32_t>(), hist.as<int>(), cst.as<long long>(),
-                             clen.as<unsigned long long>(), tmp.as<long long>(), te,
-                             c->d_te_pairs.as<int2>(), c->stream));
-    c->te_cstart.resize(NC + 1);
-    c->te_clen.resize(NC);
-    d2h(c, c->te_cstart.data(), cst.as<long long>(), NC + 1);
-    d2h(c, c->te_clen.data(), clen.as<long long>(), NC);
+                             c->d_te_m2.as<uint32_t>(), hist.as<int>(),
+                             c->d_te_cstart.as<long long>(), tmp.as<long long>(), te,
+                             c->d_te_pairs.as<int4>(), c->stream));
     sync(c);
     c->te_ready = true;
 }
@@ -1490,7 +1487,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         time_begin();
         HIPCHECK(gsk::te_level(c->dg, ks, d_pats.as<PatDev>(), B, K, d_pof.as<int>(),
                                c->d_te_m1.as<uint32_t>(), c->d_te_m2.as<uint32_t>(),
-                               c->d_te_pairs.as<int2>(), d_q.as<int>(),
+                               c->d_te_pairs.as<int4>(), d_q.as<int>(),
                                d_cnt.as<unsigned long long>(), d_cen.as<int>(), d_end.as<int>(),
                                d_kc.as<int>(), d_te_img.as<int>(),
                                d_te_work.as<unsigned long long>(), c->stream));
@@ -2000,14 +1997,24 @@ void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) 
 void seeds_from_runs(gs_ctx* c) {
     te_build(c);
     const int VL = c->VL, EL = c->EL, NB = VL * EL;
-    auto pairs_of = [&](long long cl) { return c->te_cstart[cl + 1] - c->te_cstart[cl]; };
+    const long long NC = (long long)VL * NB;
+    DevPool* P = &c->pool;
+    DBuf d_np(P, sizeof(int) * NC), d_nl(P, sizeof(long long) * NC);
+    HIPCHECK(gsk::te_classes(c->d_te_pairs.as<int4>(), c->d_te_cstart.as<long long>(), NC,
+                             d_np.as<int>(), d_nl.as<unsigned long long>(), c->stream));
+    std::vector<int> np(NC);
+    std::vector<long long> nl(NC);
+    d2h(c, np.data(), d_np.as<int>(), NC);
+    d2h(c, nl.data(), d_nl.as<long long>(), NC);
+    sync(c);
+    auto pairs_of = [&](long long cl) { return (long long)np[cl]; };
     std::vector<HPat> freq;
     long long codes = 0, rows = 0;
     for (int a = 0; a < VL; ++a) {
         for (int e0 = 0; e0 < EL; ++e0) {
             for (int b = a; b < VL; ++b) {
                 const long long cl = (long long)a * NB + (long long)b * EL + e0;
-                const long long cnt = c->te_clen[cl];
+                const long long cnt = nl[cl];
                 if (cnt == 0) continue;
                 ++codes;
                 HPat h;
@@ -2457,6 +2464,7 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->dg.rkey = x->d_rkey.as<int>();
         x->dg.runid = x->d_runid.as<int>();
         x->dg.nruns = nruns;
+        if (two_edge_graph(x)) te_build(x);
     });
     if (rc != GS_OK) {
         std::fprintf(stderr, "gs_create: %s\n", c->err.c_str());
@@ -2877,6 +2885,7 @@ int gs_destroy(gs_ctx* c) {
     c->d_te_m1.reset();
     c->d_te_m2.reset();
     c->d_te_pairs.reset();
+    c->d_te_cstart.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();

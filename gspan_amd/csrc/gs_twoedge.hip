@@ -68,45 +68,51 @@ __global__ void k_te_masks(DevGraph g, int EL, int W, uint32_t* __restrict__ m1,
     }
 }
 
-// pairs and CSR entries per class (hist, clen), then the pairs at their class positions
-// (cursor = class starts, advanced); one thread per vertex over its runs
+// pairs per class (hist), then the pairs at their class positions (cursor = class starts,
+// advanced); one thread per vertex over its runs.  A pair's record is two int4: {vertex,
+// run, first entry of the run, run length} and {first run of the vertex, last run of the
+// vertex, vertex label, run bit}.
 template <bool SCATTER>
 __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or_cursor,
-                           unsigned long long* __restrict__ clen,
-                           const long long* __restrict__ cstart, int2* __restrict__ pairs) {
+                           const long long* __restrict__ cstart, int4* __restrict__ rec) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += stride) {
         const int o0 = g.offsets[v], o1 = g.offsets[v + 1];
         if (o0 == o1) continue;
         const int r0 = g.runid[o0], r1 = g.runid[o1 - 1];
-        const long long cbase = (long long)g.vlab[v] * NB;
+        const int lv = g.vlab[v];
+        const long long cbase = (long long)lv * NB;
         for (int k = r0; k <= r1; ++k) {
-            const long long c = cbase + run_bit(g.rkey[k], EL);
+            const int b = run_bit(g.rkey[k], EL);
+            const long long c = cbase + b;
+            const int st = g.rstart[k], len = g.rstart[k + 1] - st;
             if (!SCATTER) {
                 atomicAdd(&hist_or_cursor[c], 1);
-                atomicAdd(&clen[c], (unsigned long long)(g.rstart[k + 1] - g.rstart[k]));
             } else {
                 const long long at = cstart[c] + atomicAdd(&hist_or_cursor[c], 1);
-                pairs[at] = make_int2((int)v, k);
+                rec[2 * at] = make_int4((int)v, k, st, len);
+                rec[2 * at + 1] = make_int4(r0, r1, lv, b);
             }
         }
     }
 }
 
 // The three tables from the class-ordered pairs.  One wave per workgroup takes chunks of
-// TE_CHUNK consecutive pairs from a queue and sums them in LDS per class (NB counters of
-// each table), adding the non-zero sums to the global tables when the class changes and
-// at the end.  p_of[(a * EL + e0) * VL + b]: the batch index of the single-edge parent
-// (0, 1, a, e0, b), a <= b, or -1; CNT / CEN only for pairs that are parent runs of one.
-// Per pair, the OR over its run's vertices is taken by G = 64 / WP lane groups of WP >= W
-// lanes (lane w of a group holds word w).  work[0] += runs visited for CNT / CEN, work[1]
-// += mask words read for END.
-constexpr int TE_CHUNK = 32;
+// TE_CHUNK consecutive pairs from a queue (lane i loads the record of the chunk's pair i)
+// and sums them per class -- CNT / CEN in LDS, END in registers (lane j holds the
+// counts of bits j, j + 64, ...; NT blocks of 64 bits) -- adding the non-zero sums to the
+// global tables when the class changes and at the end.  p_of[(a * EL + e0) * VL + b]: the
+// batch index of the single-edge parent (0, 1, a, e0, b), a <= b, or -1; CNT / CEN only
+// for classes that are parent runs of one.  Per pair, the OR over its run's vertices is
+// taken by G = 64 / WP lane groups of WP >= W lanes (lane w of a group holds word w).
+// work[0] += runs visited for CNT / CEN, work[1] += mask words read for END.
+constexpr int TE_CHUNK = 64;
 
+template <int NT>
 __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int NB, int W,
                                                  int WP, const uint32_t* __restrict__ m1,
                                                  const uint32_t* __restrict__ m2,
-                                                 const int2* __restrict__ pairs, long long npairs,
+                                                 const int4* __restrict__ rec, long long npairs,
                                                  const int* __restrict__ p_of, int* queue,
                                                  unsigned long long* __restrict__ CNT,
                                                  int* __restrict__ CEN, int* __restrict__ END,
@@ -114,17 +120,19 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
     extern __shared__ unsigned long long te_smem[];
     unsigned long long* s_cnt = te_smem;
     int* s_cen = reinterpret_cast<int*>(s_cnt + NB);
-    int* s_end = s_cen + NB;
     const int lane = threadIdx.x;
     for (int b = lane; b < NB; b += 64) {
         s_cnt[b] = 0;
         s_cen[b] = 0;
-        s_end[b] = 0;
     }
+    int endc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) endc[t] = 0;
     __syncthreads();
     const int G = 64 / WP, grp = lane / WP, wi = lane % WP;
     const long long nchunks = (npairs + TE_CHUNK - 1) / TE_CHUNK;
     long long cls = -1;
+    bool parent = false;  // the current class is a parent run of a batch pattern
     unsigned long long w_runs = 0, w_words = 0;
     auto flush = [&]() {
         __syncthreads();
@@ -132,10 +140,14 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
         for (int b = lane; b < NB; b += 64) {
             if (s_cnt[b]) atomicAdd(&CNT[row + b], s_cnt[b]);
             if (s_cen[b]) atomicAdd(&CEN[row + b], s_cen[b]);
-            if (s_end[b]) atomicAdd(&END[row + b], s_end[b]);
             s_cnt[b] = 0;
             s_cen[b] = 0;
-            s_end[b] = 0;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int b = lane + 64 * t;
+            if (endc[t]) atomicAdd(&END[row + b], endc[t]);
+            endc[t] = 0;
         }
         __syncthreads();
     };
@@ -144,26 +156,44 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
         if (lane == 0) ch = atomicAdd(queue, 1);
         ch = __shfl(ch, 0);
         if (ch >= nchunks) break;
-        const long long pe = min(npairs, (ch + 1) * TE_CHUNK);
-        for (long long pi = ch * TE_CHUNK; pi < pe; ++pi) {
-            const int2 pr = pairs[pi];
-            const int x = pr.x, k = pr.y;
-            const int lx = g.vlab[x];
-            const int rk = g.rkey[k];
-            const int tl = rk >> 16, el = rk & 0xffff;
-            const int b = tl * EL + el;
+        const long long p0 = ch * TE_CHUNK;
+        const int cnt = (int)min((long long)TE_CHUNK, npairs - p0);
+        int4 ra = make_int4(0, 0, 0, 0), rb = make_int4(0, 0, 0, 0);
+        if (lane < cnt) {
+            ra = rec[2 * (p0 + lane)];
+            rb = rec[2 * (p0 + lane) + 1];
+        }
+        for (int i = 0; i < cnt; ++i) {
+            const int x = __builtin_amdgcn_readlane(ra.x, i);
+            const int k = __builtin_amdgcn_readlane(ra.y, i);
+            const int st = __builtin_amdgcn_readlane(ra.z, i);
+            const int len = __builtin_amdgcn_readlane(ra.w, i);
+            const int q0 = __builtin_amdgcn_readlane(rb.x, i);
+            const int q1 = __builtin_amdgcn_readlane(rb.y, i);
+            const int lx = __builtin_amdgcn_readlane(rb.z, i);
+            const int b = __builtin_amdgcn_readlane(rb.w, i);
+            const int el = b % EL;
             const long long c = (long long)lx * NB + b;
             if (c != cls) {
                 if (cls >= 0) flush();
                 cls = c;
+                const int tl = b / EL;
+                const int lo = lx < tl ? lx : tl, hi = lx < tl ? tl : lx;
+                parent = p_of[(lo * EL + el) * VL + hi] >= 0;
             }
-            const int st = g.rstart[k], len = g.rstart[k + 1] - st;
+            // x as an end vertex z: the runs of its neighbours u in run b, beyond z (the
+            // loads are issued first, the centre sums below overlap them)
+            const int bs = lx * EL + el;  // the run of u that holds x
+            uint32_t acc = 0;
+            bool two = false;
+            for (int j = grp; j < len; j += G) {
+                const int u = (int)g.adjl[st + j].x;
+                if (wi < W) acc |= m1[(long long)u * W + wi];
+                if (wi == 0) two |= ((m2[(long long)u * W + (bs >> 5)] >> (bs & 31)) & 1u) != 0;
+            }
             // x as the segment vertex of a parent row (x, w), w in run b
-            const int lo = lx < tl ? lx : tl, hi = lx < tl ? tl : lx;
-            if (p_of[(lo * EL + el) * VL + hi] >= 0) {
-                const int o0 = g.offsets[x];
-                const int r0 = g.runid[o0], r1 = g.runid[g.offsets[x + 1] - 1];
-                for (int q = r0 + lane; q <= r1; q += 64) {
+            if (parent) {
+                for (int q = q0 + lane; q <= q1; q += 64) {
                     const int wq = g.rstart[q + 1] - g.rstart[q] - (q == k ? 1 : 0);
                     if (wq >= 1) {
                         const int bq = run_bit(g.rkey[q], EL);
@@ -171,25 +201,18 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                         atomicAdd(&s_cen[bq], 1);
                     }
                 }
-                if (lane == 0) w_runs += (unsigned long long)(r1 - r0 + 1);
-            }
-            // x as an end vertex z: the runs of its neighbours u in run b, beyond z
-            const int bs = lx * EL + el;  // the run of u that holds x
-            uint32_t acc = 0;
-            bool two = false;
-            for (int i = grp; i < len; i += G) {
-                const int u = (int)g.adjl[st + i].x;
-                if (wi < W) acc |= m1[(long long)u * W + wi];
-                if (wi == 0) two |= ((m2[(long long)u * W + (bs >> 5)] >> (bs & 31)) & 1u) != 0;
+                w_runs += (unsigned long long)(q1 - q0 + 1);
             }
             for (int off = WP; off < 64; off <<= 1) acc |= __shfl_xor(acc, off);
             two = __ballot(two) != 0;
             if (wi == (bs >> 5))
                 acc = (acc & ~(1u << (bs & 31))) | (two ? 1u << (bs & 31) : 0u);
-            if (lane == 0) w_words += (unsigned long long)len * (unsigned long long)(W + 1);
-            for (int bb = lane; bb < NB; bb += 64) {
-                const uint32_t word = (uint32_t)__shfl((int)acc, bb >> 5);
-                if ((word >> (bb & 31)) & 1u) atomicAdd(&s_end[bb], 1);
+            w_words += (unsigned long long)len * (unsigned long long)(W + 1);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int bb = lane + 64 * t;
+                const uint32_t word = (uint32_t)__shfl((int)acc, (bb >> 5) & 63);
+                endc[t] += (bb < NB && ((word >> (bb & 31)) & 1u)) ? 1 : 0;
             }
         }
     }
@@ -197,6 +220,27 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
     if (lane == 0) {
         atomicAdd(&work[0], w_runs);
         atomicAdd(&work[1], w_words);
+    }
+}
+
+// Per class: its pairs (npairs) and the CSR entries of its runs (nlen), one wave per class
+// over the class's records.
+__global__ __launch_bounds__(256) void k_te_classes(const int4* __restrict__ rec,
+                                                    const long long* __restrict__ cstart,
+                                                    long long NC, int* __restrict__ npairs,
+                                                    unsigned long long* __restrict__ nlen) {
+    const int lane = threadIdx.x & 63;
+    const long long waves = (long long)gridDim.x * (blockDim.x / 64);
+    for (long long c = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; c < NC;
+         c += waves) {
+        const long long s0 = cstart[c], s1 = cstart[c + 1];
+        unsigned long long sum = 0;
+        for (long long i = s0 + lane; i < s1; i += 64) sum += (unsigned long long)rec[2 * i].w;
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if (lane == 0) {
+            npairs[c] = (int)(s1 - s0);
+            nlen[c] = sum;
+        }
     }
 }
 
@@ -259,27 +303,26 @@ namespace gsk {
 static_assert(GS_MAXV == 8, "image counts are 8 per key");
 
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
-                      int* hist, long long* cstart, unsigned long long* clen,
-                      long long* scan_tmp, long long scan_elems, int2* pairs, hipStream_t s) {
+                      int* hist, long long* cstart, long long* scan_tmp, long long scan_elems,
+                      int4* rec, hipStream_t s) {
     const long long NC = (long long)(NB / EL) * NB;  // VL * NB classes
     hipError_t e;
     if ((e = fill_u32(m1, 0u, (long long)g.n * W, s)) != hipSuccess) return e;
     if ((e = fill_u32(m2, 0u, (long long)g.n * W, s)) != hipSuccess) return e;
     if ((e = fill_i32(hist, 0, NC, s)) != hipSuccess) return e;
-    if ((e = fill_u32(reinterpret_cast<uint32_t*>(clen), 0u, 2 * NC, s)) != hipSuccess) return e;
     const int grid = te_grid(g.n, 256, 8192);
     hipLaunchKernelGGL(k_te_masks, dim3(grid), dim3(256), 0, s, g, EL, W, m1, m2);
-    hipLaunchKernelGGL(k_te_pairs<false>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist, clen,
-                       (const long long*)nullptr, (int2*)nullptr);
+    hipLaunchKernelGGL(k_te_pairs<false>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist,
+                       (const long long*)nullptr, (int4*)nullptr);
     if ((e = scan_exclusive(hist, cstart, NC, scan_tmp, scan_elems, s)) != hipSuccess) return e;
     if ((e = fill_i32(hist, 0, NC, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_te_pairs<true>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist,
-                       (unsigned long long*)nullptr, cstart, pairs);
+    hipLaunchKernelGGL(k_te_pairs<true>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist, cstart,
+                       rec);
     return hipGetLastError();
 }
 
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
-                    const int* p_of, const uint32_t* m1, const uint32_t* m2, const int2* pairs,
+                    const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     int* queue, unsigned long long* CNT, int* CEN, int* END, int* key_count,
                     int* img, unsigned long long* work, hipStream_t s) {
     const int NB = ks.VL * ks.EL, W = (NB + 31) / 32;
@@ -292,14 +335,31 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
     if ((e = fill_i32(END, 0, rows, s)) != hipSuccess) return e;
     if ((e = fill_i32(queue, 0, 1, s)) != hipSuccess) return e;
     const long long np = g.nruns;
-    const size_t lds = (size_t)NB * 16;
+    const size_t lds = (size_t)NB * 12;
     const long long nchunks = (np + TE_CHUNK - 1) / TE_CHUNK;
     const int grid = (int)(nchunks < 8192 ? (nchunks < 1 ? 1 : nchunks) : 8192);
-    hipLaunchKernelGGL(k_te_accum, dim3(grid), dim3(64), lds, s, g, ks.EL, ks.VL, NB, W, WP, m1,
-                       m2, pairs, np, p_of, queue, CNT, CEN, END, work);
+    const int nt = (NB + 63) / 64;
+#define GS_TE_ACCUM(T)                                                                        \
+    hipLaunchKernelGGL(k_te_accum<T>, dim3(grid), dim3(64), lds, s, g, ks.EL, ks.VL, NB, W, WP, \
+                       m1, m2, rec, np, p_of, queue, CNT, CEN, END, work)
+    if (nt <= 1) GS_TE_ACCUM(1);
+    else if (nt <= 2) GS_TE_ACCUM(2);
+    else if (nt <= 4) GS_TE_ACCUM(4);
+    else if (nt <= 8) GS_TE_ACCUM(8);
+    else if (nt <= 16) GS_TE_ACCUM(16);
+    else return hipErrorInvalidValue;
+#undef GS_TE_ACCUM
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_te_keys, dim3(te_grid(K, 256, 8192)), dim3(256), 0, s, pats, npats, ks,
                        K, NB, CNT, CEN, END, key_count, img);
+    return hipGetLastError();
+}
+
+hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, int* npairs,
+                      unsigned long long* nlen, hipStream_t s) {
+    if (NC <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_te_classes, dim3(te_grid(NC, 4, 8192)), dim3(256), 0, s, rec, cstart, NC,
+                       npairs, nlen);
     return hipGetLastError();
 }
 
