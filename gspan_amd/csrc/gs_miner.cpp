@@ -631,10 +631,13 @@ void order_reports(gs_ctx* c) {
         std::vector<int> at(start.begin(), start.end() - 1);
         for (int i = 0; i < n; ++i) kids[at[r[i].parent + 1]++] = i;
     }
-    // group g (parent g - 1) occupies kids[start[g] .. start[g+1])
-    for (int g = 0; g <= n; ++g)
-        std::sort(kids.begin() + start[g], kids.begin() + start[g + 1],
-                  [&](int a, int b) { return r[a].edge_key < r[b].edge_key; });
+    // group g (parent g - 1) occupies kids[start[g] .. start[g+1]); children reported in
+    // key order mostly arrive in edge order already
+    auto by_edge = [&](int a, int b) { return r[a].edge_key < r[b].edge_key; };
+    for (int g = 0; g <= n; ++g) {
+        const auto b0 = kids.begin() + start[g], b1 = kids.begin() + start[g + 1];
+        if (!std::is_sorted(b0, b1, by_edge)) std::sort(b0, b1, by_edge);
+    }
     std::vector<int> order;
     order.reserve(n);
     std::vector<int> stack;
@@ -1604,7 +1607,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             return (size_t)rows_total * sizeof(int2) + (size_t)pool_words * 4 +
                    (size_t)A * (sizeof(SetDesc) + 8 + 4 * GS_MAXV + 24) + (size_t)K * 4;
         };
-        plan(bytes);
+        // a single-pass last level has neither rows nor sets to place
+        if (plan_sets || write_rows) plan(bytes);
         if (plan_sets && bytes && c->opts.set_kind == GS_SETS_AUTO &&
             c->pool.in_use() + need_of() > c->budget) {
             bytes = false;  // bytemaps do not fit: bitmaps take an eighth of the memory
@@ -1631,14 +1635,18 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             }
             out.rows_total = rows_total;
         }
-        HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
-        if (plan_sets) HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
-        HIPCHECK(gsk::build_sets(d_pats.as<PatDev>(), B, ks, d_active.as<int>(),
-                                 d_acnt.as<int>(), d_flags.as<int>(), A, c->d_lcnt.as<int>(),
-                                 bytes,
-                                 d_soff.as<long long>(), write_rows ? d_roff.as<long long>() : nullptr,
-                                 single ? d_all_sets.as<SetDesc>() : nullptr, d_sets.as<SetDesc>(),
-                                 d_slot.as<int>(), d_cursor.as<unsigned long long>(), c->stream));
+        if (!two_edge) {  // the two-edge level's image counts come without sets
+            HIPCHECK(gsk::fill_i32(d_slot.as<int>(), -1, K, c->stream));
+            if (plan_sets)
+                HIPCHECK(gsk::fill_u32(d_pool.as<uint32_t>(), 0u, pool_words, c->stream));
+            HIPCHECK(gsk::build_sets(d_pats.as<PatDev>(), B, ks, d_active.as<int>(),
+                                     d_acnt.as<int>(), d_flags.as<int>(), A, c->d_lcnt.as<int>(),
+                                     bytes, d_soff.as<long long>(),
+                                     write_rows ? d_roff.as<long long>() : nullptr,
+                                     single ? d_all_sets.as<SetDesc>() : nullptr,
+                                     d_sets.as<SetDesc>(), d_slot.as<int>(),
+                                     d_cursor.as<unsigned long long>(), c->stream));
+        }
         c->st.t_plan_ms += std::chrono::duration<double, std::milli>(
                                std::chrono::steady_clock::now() - t_plan0)
                                .count();

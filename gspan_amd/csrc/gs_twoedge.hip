@@ -103,10 +103,13 @@ __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or
 // counts of bits j, j + 64, ...; NT blocks of 64 bits) -- adding the non-zero sums to the
 // global tables when the class changes and at the end.  p_of[(a * EL + e0) * VL + b]: the
 // batch index of the single-edge parent (0, 1, a, e0, b), a <= b, or -1; CNT / CEN only
-// for classes that are parent runs of one.  Per pair, the OR over its run's vertices is
-// taken by G = 64 / WP lane groups of WP >= W lanes (lane w of a group holds word w).
+// for classes that are parent runs of one.  The wave is G = 64 / WP lane groups of WP >= W
+// lanes (lane w of a group holds word w of an OR): up to G consecutive pairs of one class
+// with short runs go one per group, so their load chains overlap; a pair with a longer run
+// has its run's vertices split over the groups.
 // work[0] += runs visited for CNT / CEN, work[1] += mask words read for END.
 constexpr int TE_CHUNK = 64;
+constexpr int TE_SHORT = 16;  // runs up to this length share the wave with other pairs
 
 template <int NT>
 __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int NB, int W,
@@ -163,13 +166,7 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             ra = rec[2 * (p0 + lane)];
             rb = rec[2 * (p0 + lane) + 1];
         }
-        for (int i = 0; i < cnt; ++i) {
-            const int x = __builtin_amdgcn_readlane(ra.x, i);
-            const int k = __builtin_amdgcn_readlane(ra.y, i);
-            const int st = __builtin_amdgcn_readlane(ra.z, i);
-            const int len = __builtin_amdgcn_readlane(ra.w, i);
-            const int q0 = __builtin_amdgcn_readlane(rb.x, i);
-            const int q1 = __builtin_amdgcn_readlane(rb.y, i);
+        for (int i = 0; i < cnt;) {
             const int lx = __builtin_amdgcn_readlane(rb.z, i);
             const int b = __builtin_amdgcn_readlane(rb.w, i);
             const int el = b % EL;
@@ -181,19 +178,40 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                 const int lo = lx < tl ? lx : tl, hi = lx < tl ? tl : lx;
                 parent = p_of[(lo * EL + el) * VL + hi] >= 0;
             }
+            // Pairs i .. i + n - 1: of this class, at most G, runs of at most TE_SHORT
+            // entries -- lane group q takes pair i + q.  A longer run is taken alone, its
+            // entries split over the groups.
+            const unsigned long long same =
+                __ballot(lane < cnt && rb.z == lx && rb.w == b) >> i;  // bit 0 set
+            const unsigned long long longr = __ballot(lane < cnt && ra.w > TE_SHORT) >> i;
+            const bool alone = (longr & 1ull) != 0;
+            int n = 1;
+            if (!alone) {
+                const unsigned long long z = ~same;
+                n = z ? __builtin_ctzll(z) : 64 - i;
+                n = n < G ? n : G;
+                const unsigned long long lf = longr & (n >= 64 ? ~0ull : (1ull << n) - 1);
+                if (lf) n = __builtin_ctzll(lf);
+            }
+            const bool act = alone || grp < n;
+            const int pi = alone ? i : i + (act ? grp : 0);
+            const int k = __shfl(ra.y, pi), st = __shfl(ra.z, pi), len = __shfl(ra.w, pi);
+            const int q0 = __shfl(rb.x, pi), q1 = __shfl(rb.y, pi);
             // x as an end vertex z: the runs of its neighbours u in run b, beyond z (the
             // loads are issued first, the centre sums below overlap them)
             const int bs = lx * EL + el;  // the run of u that holds x
             uint32_t acc = 0;
             bool two = false;
-            for (int j = grp; j < len; j += G) {
+            const int j0 = alone ? grp : 0, js = alone ? G : 1;
+            for (int j = j0; act && j < len; j += js) {
                 const int u = (int)g.adjl[st + j].x;
                 if (wi < W) acc |= m1[(long long)u * W + wi];
                 if (wi == 0) two |= ((m2[(long long)u * W + (bs >> 5)] >> (bs & 31)) & 1u) != 0;
             }
             // x as the segment vertex of a parent row (x, w), w in run b
             if (parent) {
-                for (int q = q0 + lane; q <= q1; q += 64) {
+                const int qs = alone ? 64 : WP;
+                for (int q = q0 + (alone ? lane : wi); act && q <= q1; q += qs) {
                     const int wq = g.rstart[q + 1] - g.rstart[q] - (q == k ? 1 : 0);
                     if (wq >= 1) {
                         const int bq = run_bit(g.rkey[q], EL);
@@ -201,22 +219,36 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                         atomicAdd(&s_cen[bq], 1);
                     }
                 }
-                w_runs += (unsigned long long)(q1 - q0 + 1);
             }
-            for (int off = WP; off < 64; off <<= 1) acc |= __shfl_xor(acc, off);
-            two = __ballot(two) != 0;
+            const bool lead = alone ? lane == 0 : (act && wi == 0);  // counts the pair's work
+            if (lead) {
+                if (parent) w_runs += (unsigned long long)(q1 - q0 + 1);
+                w_words += (unsigned long long)len * (unsigned long long)(W + 1);
+            }
+            if (alone) {
+                for (int off = WP; off < 64; off <<= 1) acc |= __shfl_xor(acc, off);
+                two = __ballot(two) != 0;
+            } else {
+                two = __shfl((int)two, grp * WP) != 0;
+            }
             if (wi == (bs >> 5))
                 acc = (acc & ~(1u << (bs & 31))) | (two ? 1u << (bs & 31) : 0u);
-            w_words += (unsigned long long)len * (unsigned long long)(W + 1);
+            for (int q = 0; q < n; ++q) {
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const int bb = lane + 64 * t;
-                const uint32_t word = (uint32_t)__shfl((int)acc, (bb >> 5) & 63);
-                endc[t] += (bb < NB && ((word >> (bb & 31)) & 1u)) ? 1 : 0;
+                for (int t = 0; t < NT; ++t) {
+                    const int bb = lane + 64 * t;
+                    const uint32_t word = (uint32_t)__shfl((int)acc, (q * WP + (bb >> 5)) & 63);
+                    endc[t] += (bb < NB && ((word >> (bb & 31)) & 1u)) ? 1 : 0;
+                }
             }
+            i += n;
         }
     }
     if (cls >= 0) flush();
+    for (int off = 32; off > 0; off >>= 1) {
+        w_runs += __shfl_xor(w_runs, off);
+        w_words += __shfl_xor(w_words, off);
+    }
     if (lane == 0) {
         atomicAdd(&work[0], w_runs);
         atomicAdd(&work[1], w_words);
