@@ -425,7 +425,7 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
 // Two-edge last level (gs_twoedge.hip).  te_prepare: run masks m1 / m2 (W words per
 // vertex, zeroed here) and the (vertex, run) pairs ordered by class (vertex label * NB +
 // run bit), two int4 records per pair, with hist (VL * NB ints), cstart (VL * NB + 1: the
-// class starts) and scan scratch.  te_level:
+// class starts), rbl ({bit, length} of every run) and scan scratch.  te_level:
 // the count tables (CNT, CEN, END: VL * NB * NB entries each, cleared here) from the pairs,
 // then key_count and img[8 * key + v] for the batch's K keys; p_of: batch index of each
 // single-edge parent (a, e0, b) at (a * EL + e0) * VL + b, or -1; queue: one int;
@@ -433,14 +433,14 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
 // img[8 * akeys[a] + v] for a < *n_active (n_active null: a < max_active).
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
                       int* hist, long long* cstart, long long* scan_tmp, long long scan_elems,
-                      int4* rec, hipStream_t s);
+                      int4* rec, int2* rbl, hipStream_t s);
 // per class c < NC: npairs[c] = its pairs, nlen[c] = CSR entries of its runs
 hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, int* npairs,
                       unsigned long long* nlen, hipStream_t s);
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
-                    int* queue, unsigned long long* CNT, int* CEN, int* END, int* key_count,
-                    int* img, unsigned long long* work, hipStream_t s);
+                    const int2* rbl, int* queue, unsigned long long* CNT, int* CEN, int* END,
+                    int* key_count, int* img, unsigned long long* work, hipStream_t s);
 hipError_t te_gather(const int* img, const int* akeys, const int* n_active, int max_active,
                      int* counts, hipStream_t s);
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,

@@ -310,7 +310,7 @@ struct gs_ctx {
     // pairs ordered by class, built on first use (they depend on the graph only)
     // and the first pair of every class (vertex label * VL * EL + run bit; VL * VL * EL + 1
     // entries); built by gs_create
-    DBuf d_te_m1, d_te_m2, d_te_pairs, d_te_cstart;
+    DBuf d_te_m1, d_te_m2, d_te_pairs, d_te_cstart, d_te_rbl;
     bool te_ready = false;
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
@@ -1045,12 +1045,13 @@ void te_build(gs_ctx* c) {
     c->d_te_m2 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
     c->d_te_pairs = DBuf(P, 2 * sizeof(int4) * std::max(c->dg.nruns, 1));
     c->d_te_cstart = DBuf(P, 8 * (NC + 1));
+    c->d_te_rbl = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
     const long long te = gsk::scan_tmp_elems(NC);
     DBuf hist(P, 4 * NC), tmp(P, 8 * te);
     HIPCHECK(gsk::te_prepare(c->dg, c->EL, NB, W, c->d_te_m1.as<uint32_t>(),
                              c->d_te_m2.as<uint32_t>(), hist.as<int>(),
                              c->d_te_cstart.as<long long>(), tmp.as<long long>(), te,
-                             c->d_te_pairs.as<int4>(), c->stream));
+                             c->d_te_pairs.as<int4>(), c->d_te_rbl.as<int2>(), c->stream));
     sync(c);
     c->te_ready = true;
 }
@@ -1490,7 +1491,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         time_begin();
         HIPCHECK(gsk::te_level(c->dg, ks, d_pats.as<PatDev>(), B, K, d_pof.as<int>(),
                                c->d_te_m1.as<uint32_t>(), c->d_te_m2.as<uint32_t>(),
-                               c->d_te_pairs.as<int4>(), d_q.as<int>(),
+                               c->d_te_pairs.as<int4>(), c->d_te_rbl.as<int2>(),
+                               d_q.as<int>(),
                                d_cnt.as<unsigned long long>(), d_cen.as<int>(), d_end.as<int>(),
                                d_kc.as<int>(), d_te_img.as<int>(),
                                d_te_work.as<unsigned long long>(), c->stream));
@@ -2894,6 +2896,7 @@ int gs_destroy(gs_ctx* c) {
     c->d_te_m2.reset();
     c->d_te_pairs.reset();
     c->d_te_cstart.reset();
+    c->d_te_rbl.reset();
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();

@@ -38,6 +38,13 @@ __device__ __forceinline__ int run_bit(int rkey, int EL) {  // rkey = (label << 
     return (rkey >> 16) * EL + (rkey & 0xffff);
 }
 
+// {bit, length} of every run
+__global__ void k_te_runs(DevGraph g, int EL, int2* __restrict__ rbl) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < g.nruns; q += stride)
+        rbl[q] = make_int2(run_bit(g.rkey[q], EL), g.rstart[q + 1] - g.rstart[q]);
+}
+
 // M1 / M2 words of every vertex: one thread per vertex over its runs (ascending bits, so
 // each word is complete before the next starts); the arrays are zero beforehand.
 __global__ void k_te_masks(DevGraph g, int EL, int W, uint32_t* __restrict__ m1,
@@ -115,7 +122,8 @@ template <int NT>
 __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int NB, int W,
                                                  int WP, const uint32_t* __restrict__ m1,
                                                  const uint32_t* __restrict__ m2,
-                                                 const int4* __restrict__ rec, long long npairs,
+                                                 const int4* __restrict__ rec,
+                                                 const int2* __restrict__ rbl, long long npairs,
                                                  const int* __restrict__ p_of, int* queue,
                                                  unsigned long long* __restrict__ CNT,
                                                  int* __restrict__ CEN, int* __restrict__ END,
@@ -212,9 +220,10 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             if (parent) {
                 const int qs = alone ? 64 : WP;
                 for (int q = q0 + (alone ? lane : wi); act && q <= q1; q += qs) {
-                    const int wq = g.rstart[q + 1] - g.rstart[q] - (q == k ? 1 : 0);
+                    const int2 r = rbl[q];
+                    const int wq = r.y - (q == k ? 1 : 0);
                     if (wq >= 1) {
-                        const int bq = run_bit(g.rkey[q], EL);
+                        const int bq = r.x;
                         atomicAdd(&s_cnt[bq], (unsigned long long)len * (unsigned long long)wq);
                         atomicAdd(&s_cen[bq], 1);
                     }
@@ -336,7 +345,7 @@ static_assert(GS_MAXV == 8, "image counts are 8 per key");
 
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
                       int* hist, long long* cstart, long long* scan_tmp, long long scan_elems,
-                      int4* rec, hipStream_t s) {
+                      int4* rec, int2* rbl, hipStream_t s) {
     const long long NC = (long long)(NB / EL) * NB;  // VL * NB classes
     hipError_t e;
     if ((e = fill_u32(m1, 0u, (long long)g.n * W, s)) != hipSuccess) return e;
@@ -344,6 +353,7 @@ hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, ui
     if ((e = fill_i32(hist, 0, NC, s)) != hipSuccess) return e;
     const int grid = te_grid(g.n, 256, 8192);
     hipLaunchKernelGGL(k_te_masks, dim3(grid), dim3(256), 0, s, g, EL, W, m1, m2);
+    hipLaunchKernelGGL(k_te_runs, dim3(te_grid(g.nruns, 256, 8192)), dim3(256), 0, s, g, EL, rbl);
     hipLaunchKernelGGL(k_te_pairs<false>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist,
                        (const long long*)nullptr, (int4*)nullptr);
     if ((e = scan_exclusive(hist, cstart, NC, scan_tmp, scan_elems, s)) != hipSuccess) return e;
@@ -355,8 +365,8 @@ hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, ui
 
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
-                    int* queue, unsigned long long* CNT, int* CEN, int* END, int* key_count,
-                    int* img, unsigned long long* work, hipStream_t s) {
+                    const int2* rbl, int* queue, unsigned long long* CNT, int* CEN, int* END,
+                    int* key_count, int* img, unsigned long long* work, hipStream_t s) {
     const int NB = ks.VL * ks.EL, W = (NB + 31) / 32;
     int WP = 1;
     while (WP < W) WP <<= 1;
@@ -373,7 +383,7 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
     const int nt = (NB + 63) / 64;
 #define GS_TE_ACCUM(T)                                                                        \
     hipLaunchKernelGGL(k_te_accum<T>, dim3(grid), dim3(64), lds, s, g, ks.EL, ks.VL, NB, W, WP, \
-                       m1, m2, rec, np, p_of, queue, CNT, CEN, END, work)
+                       m1, m2, rec, rbl, np, p_of, queue, CNT, CEN, END, work)
     if (nt <= 1) GS_TE_ACCUM(1);
     else if (nt <= 2) GS_TE_ACCUM(2);
     else if (nt <= 4) GS_TE_ACCUM(4);
