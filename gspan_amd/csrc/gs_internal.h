@@ -428,7 +428,8 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
 // class starts), rbl ({bit, length} of every run) and scan scratch.  te_level:
 // the count tables (CNT, CEN, END: VL * NB * NB entries each, cleared here) from the pairs,
 // then key_count and img[8 * key + v] for the batch's K keys; p_of: batch index of each
-// single-edge parent (a, e0, b) at (a * EL + e0) * VL + b, or -1; queue: one int;
+// single-edge parent (a, e0, b) at (a * EL + e0) * VL + b, or -1; pmask[l * W + w]: word w
+// of the run bits (l', e0) with (l', e0, l) a batch parent (either label order); queue: one int;
 // work[0..1] += runs visited, mask words read.  te_gather: counts[8 a + v] =
 // img[8 * akeys[a] + v] for a < *n_active (n_active null: a < max_active).
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
@@ -439,8 +440,9 @@ hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, in
                       unsigned long long* nlen, hipStream_t s);
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
-                    const int2* rbl, int* queue, unsigned long long* CNT, int* CEN, int* END,
-                    int* key_count, int* img, unsigned long long* work, hipStream_t s);
+                    const int2* rbl, const uint32_t* pmask, int* queue,
+                    unsigned long long* CNT, int* CEN, int* END, int* key_count, int* img,
+                    unsigned long long* work, hipStream_t s);
 hipError_t te_gather(const int* img, const int* akeys, const int* n_active, int max_active,
                      int* counts, hipStream_t s);
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,

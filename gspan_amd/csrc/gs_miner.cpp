@@ -1477,11 +1477,19 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     if (two_edge) {
         const int NB = (int)te_nb;
         te_build(c);
+        const int W = (NB + 31) / 32;
         std::vector<int> p_of((size_t)c->VL * c->EL * c->VL, -1);
-        for (int i = 0; i < B; ++i)
-            p_of[((size_t)pd[i].vlab[0] * c->EL + pd[i].cel[0]) * c->VL + pd[i].vlab[1]] = i;
-        DBuf d_pof(P, sizeof(int) * p_of.size());
+        std::vector<uint32_t> pmask((size_t)c->VL * W, 0u);
+        for (int i = 0; i < B; ++i) {
+            const int a = pd[i].vlab[0], e0 = pd[i].cel[0], b = pd[i].vlab[1];
+            p_of[((size_t)a * c->EL + e0) * c->VL + b] = i;
+            const int ba = a * c->EL + e0, bb = b * c->EL + e0;  // other end seen from b, a
+            pmask[(size_t)b * W + (ba >> 5)] |= 1u << (ba & 31);
+            pmask[(size_t)a * W + (bb >> 5)] |= 1u << (bb & 31);
+        }
+        DBuf d_pof(P, sizeof(int) * p_of.size()), d_pmask(P, sizeof(uint32_t) * pmask.size());
         h2d(c, d_pof.as<int>(), p_of.data(), p_of.size());
+        h2d(c, d_pmask.as<uint32_t>(), pmask.data(), pmask.size());
         const long long rows = (long long)c->VL * NB * NB;
         DBuf d_cnt(P, 8 * rows), d_cen(P, 4 * rows), d_end(P, 4 * rows), d_q(P, 4);
         d_te_img = DBuf(P, sizeof(int) * 8 * K);
@@ -1492,7 +1500,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         HIPCHECK(gsk::te_level(c->dg, ks, d_pats.as<PatDev>(), B, K, d_pof.as<int>(),
                                c->d_te_m1.as<uint32_t>(), c->d_te_m2.as<uint32_t>(),
                                c->d_te_pairs.as<int4>(), c->d_te_rbl.as<int2>(),
-                               d_q.as<int>(),
+                               d_pmask.as<uint32_t>(), d_q.as<int>(),
                                d_cnt.as<unsigned long long>(), d_cen.as<int>(), d_end.as<int>(),
                                d_kc.as<int>(), d_te_img.as<int>(),
                                d_te_work.as<unsigned long long>(), c->stream));

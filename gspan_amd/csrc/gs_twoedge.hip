@@ -110,7 +110,10 @@ __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or
 // counts of bits j, j + 64, ...; NT blocks of 64 bits) -- adding the non-zero sums to the
 // global tables when the class changes and at the end.  p_of[(a * EL + e0) * VL + b]: the
 // batch index of the single-edge parent (0, 1, a, e0, b), a <= b, or -1; CNT / CEN only
-// for classes that are parent runs of one.  The wave is G = 64 / WP lane groups of WP >= W
+// for classes that are parent runs of one.  END rows of other classes keep only the bits
+// b' = (label lo, edge label eo) with (lo, eo, run label) a batch parent: pmask[l * W + w],
+// word w of those bits for run label l -- the only entries a key reads there.  The wave
+// is G = 64 / WP lane groups of WP >= W
 // lanes (lane w of a group holds word w of an OR): up to G consecutive pairs of one class
 // with short runs go one per group, so their load chains overlap; a pair with a longer run
 // has its run's vertices split over the groups.
@@ -124,7 +127,8 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                                                  const uint32_t* __restrict__ m2,
                                                  const int4* __restrict__ rec,
                                                  const int2* __restrict__ rbl, long long npairs,
-                                                 const int* __restrict__ p_of, int* queue,
+                                                 const int* __restrict__ p_of,
+                                                 const uint32_t* __restrict__ pmask, int* queue,
                                                  unsigned long long* __restrict__ CNT,
                                                  int* __restrict__ CEN, int* __restrict__ END,
                                                  unsigned long long* work) {
@@ -144,6 +148,8 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
     const long long nchunks = (npairs + TE_CHUNK - 1) / TE_CHUNK;
     long long cls = -1;
     bool parent = false;  // the current class is a parent run of a batch pattern
+    uint32_t keep = 0;    // word wi of the END bits the class's row needs
+    bool need_end = false;  // some of them
     unsigned long long w_runs = 0, w_words = 0;
     auto flush = [&]() {
         __syncthreads();
@@ -185,6 +191,8 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                 const int tl = b / EL;
                 const int lo = lx < tl ? lx : tl, hi = lx < tl ? tl : lx;
                 parent = p_of[(lo * EL + el) * VL + hi] >= 0;
+                keep = parent ? ~0u : (wi < W ? pmask[(long long)tl * W + wi] : 0u);
+                need_end = __ballot(keep != 0) != 0;
             }
             // Pairs i .. i + n - 1: of this class, at most G, runs of at most TE_SHORT
             // entries -- lane group q takes pair i + q.  A longer run is taken alone, its
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             uint32_t acc = 0;
             bool two = false;
             const int j0 = alone ? grp : 0, js = alone ? G : 1;
-            for (int j = j0; act && j < len; j += js) {
+            for (int j = j0; need_end && act && j < len; j += js) {
                 const int u = (int)g.adjl[st + j].x;
                 if (wi < W) acc |= m1[(long long)u * W + wi];
                 if (wi == 0) two |= ((m2[(long long)u * W + (bs >> 5)] >> (bs & 31)) & 1u) != 0;
@@ -232,7 +240,7 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             const bool lead = alone ? lane == 0 : (act && wi == 0);  // counts the pair's work
             if (lead) {
                 if (parent) w_runs += (unsigned long long)(q1 - q0 + 1);
-                w_words += (unsigned long long)len * (unsigned long long)(W + 1);
+                if (need_end) w_words += (unsigned long long)len * (unsigned long long)(W + 1);
             }
             if (alone) {
                 for (int off = WP; off < 64; off <<= 1) acc |= __shfl_xor(acc, off);
@@ -242,6 +250,7 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             }
             if (wi == (bs >> 5))
                 acc = (acc & ~(1u << (bs & 31))) | (two ? 1u << (bs & 31) : 0u);
+            acc &= keep;
             for (int q = 0; q < n; ++q) {
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
@@ -365,8 +374,9 @@ hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, ui
 
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
-                    const int2* rbl, int* queue, unsigned long long* CNT, int* CEN, int* END,
-                    int* key_count, int* img, unsigned long long* work, hipStream_t s) {
+                    const int2* rbl, const uint32_t* pmask, int* queue,
+                    unsigned long long* CNT, int* CEN, int* END, int* key_count, int* img,
+                    unsigned long long* work, hipStream_t s) {
     const int NB = ks.VL * ks.EL, W = (NB + 31) / 32;
     int WP = 1;
     while (WP < W) WP <<= 1;
@@ -383,7 +393,7 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
     const int nt = (NB + 63) / 64;
 #define GS_TE_ACCUM(T)                                                                        \
     hipLaunchKernelGGL(k_te_accum<T>, dim3(grid), dim3(64), lds, s, g, ks.EL, ks.VL, NB, W, WP, \
-                       m1, m2, rec, rbl, np, p_of, queue, CNT, CEN, END, work)
+                       m1, m2, rec, rbl, np, p_of, pmask, queue, CNT, CEN, END, work)
     if (nt <= 1) GS_TE_ACCUM(1);
     else if (nt <= 2) GS_TE_ACCUM(2);
     else if (nt <= 4) GS_TE_ACCUM(4);
