@@ -253,7 +253,10 @@ def main() -> int:
     # dominant extension kernel: per-launch algorithmic bytes / per-launch time, the time
     # from HIP events the library records around each launch on its own stream
     fill_ms, count_ms = st["t_fill_ms"], st["t_count_ms"]
-    if fill_ms >= count_ms:
+    if st["two_edge_levels"] > 0 and st["two_edge_levels"] == st["fill_launches"]:
+        kname, kms, kbytes, klaunch = "k_te_accum + k_te_keys (two-edge run pass)", fill_ms, \
+            st["alg_bytes_fill"], st["fill_launches"]
+    elif fill_ms >= count_ms:
         kname, kms, kbytes, klaunch = "k_items fill (MNI sets / child rows)", fill_ms, \
             st["alg_bytes_fill"], st["fill_launches"]
     else:
@@ -270,6 +273,27 @@ def main() -> int:
                 "kernel": kname, "launches_per_step": klaunch / steps,
                 "avg_launch_ms": kms / max(klaunch, 1),
                 "alg_bytes_per_launch": kbytes / max(klaunch, 1)}
+
+    # the same batched method with the last level extended from the parent rows, as every
+    # other level (the two-edge pass off)
+    generic = None
+    if st["two_edge_levels"] > 0:
+        with Miner(g, mode=args.timed_mode, timing=True, tile=args.tile, groups=args.groups,
+                   sets=args.sets, two_edge="off") as mo:
+            for _ in range(3):
+                mo.mine_raw(k, ms)
+            mo.reset_stats()
+            gt = []
+            for _ in range(5):
+                t = time.perf_counter()
+                gres = mo.mine_raw(k, ms)
+                gt.append(time.perf_counter() - t)
+            gst = mo.stats()
+        if Miner.to_patterns(*gres) != pats:
+            log("ERROR: two-edge and generic last levels differ")
+            return 1
+        generic = {"value": gst["extended"] / sum(gt), "ms_per_step": sum(gt) / len(gt) * 1e3}
+        log(f"two-edge pass off: {generic['ms_per_step']:.2f} ms/step")
 
     plain = None
     if not args.no_plain:
@@ -360,6 +384,7 @@ def main() -> int:
         "mining_wall_s": total_s / steps,
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "two_edge_off": generic,
         "predicted": predicted,
         "predicted_vs_plain": (predicted["value"] / plain["value"]) if (plain and predicted) else None,
         "roofline": roofline,

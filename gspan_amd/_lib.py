@@ -81,6 +81,7 @@ class GsStats(ctypes.Structure):
         ("t_sync_ms", ctypes.c_double),
         ("t_order_ms", ctypes.c_double),
         ("t_plan_ms", ctypes.c_double),
+        ("two_edge_levels", ctypes.c_longlong),
     ]
 
     def as_dict(self) -> dict:

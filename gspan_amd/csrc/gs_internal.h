@@ -425,8 +425,9 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
 // Two-edge last level (gs_twoedge.hip).  te_prepare: run masks m1 / m2 (W words per
 // vertex, zeroed here) and the (vertex, run) pairs ordered by class (vertex label * NB +
 // run bit), two int4 records per pair, with hist (VL * NB ints), cstart (VL * NB + 1: the
-// class starts), rbl ({bit, length} of every run) and scan scratch.  te_level:
-// the count tables (CNT, CEN, END: VL * NB * NB entries each, cleared here) from the pairs,
+// class starts), rbl ({bit, length} of every run) and scan scratch.  te_clear: zeroes the
+// count tables (CNT, CEN, END: VL * NB * NB entries each) and the queue.  te_level:
+// the count tables from the pairs,
 // then key_count and img[8 * key + v] for the batch's K keys; p_of: batch index of each
 // single-edge parent (a, e0, b) at (a * EL + e0) * VL + b, or -1; pmask[l * W + w]: word w
 // of the run bits (l', e0) with (l', e0, l) a batch parent (either label order); queue: one int;
@@ -438,6 +439,8 @@ hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, ui
 // per class c < NC: npairs[c] = its pairs, nlen[c] = CSR entries of its runs
 hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, int* npairs,
                       unsigned long long* nlen, hipStream_t s);
+hipError_t te_clear(KeySpace ks, int* queue, unsigned long long* CNT, int* CEN, int* END,
+                    hipStream_t s);
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     const int2* rbl, const uint32_t* pmask, int* queue,

@@ -1496,6 +1496,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         d_te_work = DBuf(P, 2 * sizeof(unsigned long long));
         HIPCHECK(gsk::fill_i32(d_te_work.as<int>(), 0, 4, c->stream));
         Wc_host[0] = c->dg.nruns;  // (vertex, run) pairs
+        HIPCHECK(gsk::te_clear(ks, d_q.as<int>(), d_cnt.as<unsigned long long>(), d_cen.as<int>(),
+                               d_end.as<int>(), c->stream));
         time_begin();
         HIPCHECK(gsk::te_level(c->dg, ks, d_pats.as<PatDev>(), B, K, d_pof.as<int>(),
                                c->d_te_m1.as<uint32_t>(), c->d_te_m2.as<uint32_t>(),
@@ -1506,6 +1508,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                d_te_work.as<unsigned long long>(), c->stream));
         time_end(1, &Wc_host[0]);
         c->st.fill_launches++;
+        c->st.two_edge_levels++;
         d2h(c, te_work, d_te_work.as<unsigned long long>(), 2);
     }
     for (int ci = 0; ci < (two_edge ? 0 : nchunks); ++ci) {
@@ -1575,10 +1578,12 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
     auto account_bytes = [&](long long emitted) {
-        if (two_edge) {  // per pair: its record, run and label (24 B); per run visited 12 B;
-                         // per mask word 4 B
-            c->st.alg_bytes_fill += 24.0 * (double)c->dg.nruns + 12.0 * (double)te_work[0] +
-                                    4.0 * (double)te_work[1];
+        if (two_edge) {  // per pair its 32-byte record; per run visited its 8-byte {bit,
+                         // length}; per adjacency or mask word 4 B; per key 4 table reads
+                         // (20 B) and its
+                         // count and 8 image counts (36 B)
+            c->st.alg_bytes_fill += 32.0 * (double)c->dg.nruns + 8.0 * (double)te_work[0] +
+                                    4.0 * (double)te_work[1] + 56.0 * (double)K;
             return;
         }
         double bytes = 16.0 * (double)W_full;

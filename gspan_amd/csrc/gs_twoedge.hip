@@ -117,7 +117,8 @@ __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or
 // lanes (lane w of a group holds word w of an OR): up to G consecutive pairs of one class
 // with short runs go one per group, so their load chains overlap; a pair with a longer run
 // has its run's vertices split over the groups.
-// work[0] += runs visited for CNT / CEN, work[1] += mask words read for END.
+// work[0] += runs visited for CNT / CEN, work[1] += words read for END (per run vertex: its
+// adjacency entry, two words, and W + 1 mask words).
 constexpr int TE_CHUNK = 64;
 constexpr int TE_SHORT = 16;  // runs up to this length share the wave with other pairs
 
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             const bool lead = alone ? lane == 0 : (act && wi == 0);  // counts the pair's work
             if (lead) {
                 if (parent) w_runs += (unsigned long long)(q1 - q0 + 1);
-                if (need_end) w_words += (unsigned long long)len * (unsigned long long)(W + 1);
+                if (need_end) w_words += (unsigned long long)len * (unsigned long long)(W + 3);
             }
             if (alone) {
                 for (int off = WP; off < 64; off <<= 1) acc |= __shfl_xor(acc, off);
@@ -372,6 +373,16 @@ hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, ui
     return hipGetLastError();
 }
 
+hipError_t te_clear(KeySpace ks, int* queue, unsigned long long* CNT, int* CEN, int* END,
+                    hipStream_t s) {
+    const long long NB = (long long)ks.VL * ks.EL, rows = ks.VL * NB * NB;
+    hipError_t e;
+    if ((e = fill_u32(reinterpret_cast<uint32_t*>(CNT), 0u, 2 * rows, s)) != hipSuccess) return e;
+    if ((e = fill_i32(CEN, 0, rows, s)) != hipSuccess) return e;
+    if ((e = fill_i32(END, 0, rows, s)) != hipSuccess) return e;
+    return fill_i32(queue, 0, 1, s);
+}
+
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     const int2* rbl, const uint32_t* pmask, int* queue,
@@ -380,12 +391,7 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
     const int NB = ks.VL * ks.EL, W = (NB + 31) / 32;
     int WP = 1;
     while (WP < W) WP <<= 1;
-    const long long rows = (long long)ks.VL * NB * NB;
     hipError_t e;
-    if ((e = fill_u32(reinterpret_cast<uint32_t*>(CNT), 0u, 2 * rows, s)) != hipSuccess) return e;
-    if ((e = fill_i32(CEN, 0, rows, s)) != hipSuccess) return e;
-    if ((e = fill_i32(END, 0, rows, s)) != hipSuccess) return e;
-    if ((e = fill_i32(queue, 0, 1, s)) != hipSuccess) return e;
     const long long np = g.nruns;
     const size_t lds = (size_t)NB * 12;
     const long long nchunks = (np + TE_CHUNK - 1) / TE_CHUNK;

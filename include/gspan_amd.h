@@ -112,6 +112,8 @@ typedef struct gs_stats {
     double t_sync_ms;          /* host time waiting for the device at synchronisations    */
     double t_order_ms;         /* host time putting the reports into the reference order  */
     double t_plan_ms;          /* host time turning candidate keys into children and sets */
+    long long two_edge_levels; /* last levels of two-edge children taken from neighbour
+                                  runs (their launches count as fill launches)           */
 } gs_stats;
 
 /* Receives one frequent pattern.  Return 0 to continue, non-zero to stop mining. */
