@@ -265,9 +265,9 @@ def main() -> int:
     achieved = (kbytes / klaunch) / (kms / klaunch * 1e-3) / 1e9 if kms > 0 else 0.0
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and not args.k and not args.min_sup:
-        rec = json.load(open(TRAFFIC_FILE)).get(args.workload)
-        if rec and rec.get("kernel") == kname:
-            traffic = rec["bytes_per_launch"]
+        for rec in json.load(open(TRAFFIC_FILE)).get(args.workload, []):
+            if rec.get("kernel") == kname:
+                traffic = rec["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kname, "launches_per_step": klaunch / steps,
