@@ -146,6 +146,8 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
     for (int t = 0; t < NT; ++t) endc[t] = 0;
     __syncthreads();
     const int G = 64 / WP, grp = lane / WP, wi = lane % WP;
+    // runs up to this length share the wave with other pairs: one entry per lane of a group
+    const int short_run = WP < TE_SHORT ? WP : TE_SHORT;
     const long long nchunks = (npairs + TE_CHUNK - 1) / TE_CHUNK;
     long long cls = -1;
     bool parent = false;  // the current class is a parent run of a batch pattern
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             // entries split over the groups.
             const unsigned long long same =
                 __ballot(lane < cnt && rb.z == lx && rb.w == b) >> i;  // bit 0 set
-            const unsigned long long longr = __ballot(lane < cnt && ra.w > TE_SHORT) >> i;
+            const unsigned long long longr = __ballot(lane < cnt && ra.w > short_run) >> i;
             const bool alone = (longr & 1ull) != 0;
             int n = 1;
             if (!alone) {
@@ -219,11 +221,38 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             const int bs = lx * EL + el;  // the run of u that holds x
             uint32_t acc = 0;
             bool two = false;
-            const int j0 = alone ? grp : 0, js = alone ? G : 1;
-            for (int j = j0; need_end && act && j < len; j += js) {
-                const int u = (int)g.adjl[st + j].x;
-                if (wi < W) acc |= m1[(long long)u * W + wi];
-                if (wi == 0) two |= ((m2[(long long)u * W + (bs >> 5)] >> (bs & 31)) & 1u) != 0;
+            const int bw = bs >> 5;
+            const uint32_t bbit = 1u << (bs & 31);
+            if (need_end && act) {
+                if (!alone) {
+                    // lane wi of the group loads run entry wi (one round trip), then the
+                    // group reads the entries' masks four at a time
+                    const int u_own = wi < len ? (int)g.adjl[st + wi].x : 0;
+                    if (wi < len) two = (m2[(long long)u_own * W + bw] & bbit) != 0;
+                    for (int j = 0; j < len; j += 4) {
+                        uint32_t a[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int u = __shfl(u_own, grp * WP + ((j + t) & (WP - 1)));
+                            a[t] = (j + t < len && wi < W) ? m1[(long long)u * W + wi] : 0u;
+                        }
+                        acc |= a[0] | a[1] | a[2] | a[3];
+                    }
+                } else {
+                    // the run's entries split over the groups, four per group at a time
+                    for (int j = grp; j < len; j += 4 * G) {
+                        int u[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            u[t] = j + t * G < len ? (int)g.adjl[st + j + t * G].x : -1;
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            if (u[t] < 0) continue;
+                            if (wi < W) acc |= m1[(long long)u[t] * W + wi];
+                            if (wi == 0) two |= (m2[(long long)u[t] * W + bw] & bbit) != 0;
+                        }
+                    }
+                }
             }
             // x as the segment vertex of a parent row (x, w), w in run b
             if (parent) {
@@ -246,11 +275,11 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
             if (alone) {
                 for (int off = WP; off < 64; off <<= 1) acc |= __shfl_xor(acc, off);
                 two = __ballot(two) != 0;
-            } else {
-                two = __shfl((int)two, grp * WP) != 0;
+            } else {  // any lane of the group
+                const unsigned long long gm = WP == 64 ? ~0ull : ((1ull << WP) - 1) << (grp * WP);
+                two = (__ballot(two) & gm) != 0;
             }
-            if (wi == (bs >> 5))
-                acc = (acc & ~(1u << (bs & 31))) | (two ? 1u << (bs & 31) : 0u);
+            if (wi == bw) acc = (acc & ~bbit) | (two ? bbit : 0u);
             acc &= keep;
             for (int q = 0; q < n; ++q) {
 #pragma unroll
