@@ -448,6 +448,13 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
                     unsigned long long* work, hipStream_t s);
 hipError_t te_gather(const int* img, const int* akeys, const int* n_active, int max_active,
                      int* counts, hipStream_t s);
+// The reports of a two-edge last level over every frequent single-edge code (pats, in key
+// order), in the reference's order: each code (support ssup[p]) and then its frequent
+// children (fkey ascending, supports fsup); codes5 / lens / sups of npats + F reports and
+// npats + 2 F edges, original labels from vl_vals / el_vals.
+hipError_t te_emit(const PatDev* pats, int npats, KeySpace ks, const int* fkey, const int* fsup,
+                   int F, const int* ssup, const int* vl_vals, const int* el_vals, int* codes,
+                   int* lens, int* sups, hipStream_t s);
 hipError_t support_rows(const DevGraph& g, const PatDev* pat, long long N, LayerPtrs L,
                         int mni_mode, const SetDesc* set, uint32_t* pool,
                         long long pool_words, int* guard_err, hipStream_t s);

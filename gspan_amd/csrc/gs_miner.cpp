@@ -312,6 +312,14 @@ struct gs_ctx {
     // entries); built by gs_create
     DBuf d_te_m1, d_te_m2, d_te_pairs, d_te_cstart, d_te_rbl;
     bool te_ready = false;
+    DBuf d_vl_vals, d_el_vals;  // original label of each rank, on the device
+    // A run whose two-edge last level is reached from every frequent single-edge code
+    // (flat_ok; flat_seeds codes) builds its reports on the device, in the reference's
+    // order (flat): codes, edge counts and supports land in pinned host buffers.
+    bool flat = false, flat_ok = false;
+    int flat_seeds = 0;
+    int *flat_codes = nullptr, *flat_lens = nullptr, *flat_sups = nullptr;
+    long long flat_cap_e = 0, flat_cap_l = 0, flat_cap_s = 0, flat_reports = 0, flat_edges = 0;
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
     // runs make the run modes and per-group new-vertex inserts pay at the last level
@@ -657,6 +665,14 @@ void order_reports(gs_ctx* c) {
 // so far.
 template <class Fn>
 void for_each_report(const gs_ctx* c, Fn&& fn) {
+    if (c->flat) {
+        long long e = 0;
+        for (long long r = 0; r < c->flat_reports; ++r) {
+            fn(c->flat_codes + 5 * e, c->flat_lens[r], c->flat_sups[r]);
+            e += c->flat_lens[r];
+        }
+        return;
+    }
     int path[5 * GS_MAXE];
     for (int i : c->report_order) {
         const Report& r = c->reports[i];
@@ -1054,6 +1070,46 @@ void te_build(gs_ctx* c) {
                              c->d_te_pairs.as<int4>(), c->d_te_rbl.as<int2>(), c->stream));
     sync(c);
     c->te_ready = true;
+}
+
+// The reports of a run whose two-edge last level was extended in one step from every
+// frequent single-edge code: built by gsk::te_emit in the reference's order and copied to
+// the context's pinned buffers (report ordering and decoding then cost the host nothing).
+void emit_flat(gs_ctx* c, const PatDev* d_pats, const std::vector<HPat>& pats, KeySpace ks,
+               const int* d_fkey, const int* d_fsup, int F) {
+    DevPool* P = &c->pool;
+    const int B = (int)pats.size();
+    const long long R = B + (long long)F, E = B + 2LL * F;
+    auto grow = [](int*& buf, long long& cap, long long need) {
+        if (cap >= need) return;
+        if (buf) HIPCHECK(hipHostFree(buf));
+        buf = nullptr;
+        const long long n = std::max<long long>(need + need / 2, 1024);
+        HIPCHECK(hipHostMalloc((void**)&buf, sizeof(int) * n, hipHostMallocDefault));
+        cap = n;
+    };
+    grow(c->flat_codes, c->flat_cap_e, 5 * E);
+    grow(c->flat_lens, c->flat_cap_l, R);
+    grow(c->flat_sups, c->flat_cap_s, R);
+    std::vector<int> ssup(B);
+    for (int i = 0; i < B; ++i) ssup[i] = pats[i].support;
+    DBuf d_ssup(P, sizeof(int) * std::max(B, 1)), d_codes(P, sizeof(int) * 5 * std::max(E, 1LL));
+    DBuf d_lens(P, sizeof(int) * std::max(R, 1LL)), d_sups(P, sizeof(int) * std::max(R, 1LL));
+    h2d(c, d_ssup.as<int>(), ssup.data(), B);
+    HIPCHECK(gsk::te_emit(d_pats, B, ks, d_fkey, d_fsup, F, d_ssup.as<int>(),
+                          c->d_vl_vals.as<int>(), c->d_el_vals.as<int>(), d_codes.as<int>(),
+                          d_lens.as<int>(), d_sups.as<int>(), c->stream));
+    HIPCHECK(hipMemcpyAsync(c->flat_codes, d_codes.as<int>(), sizeof(int) * 5 * E,
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->flat_lens, d_lens.as<int>(), sizeof(int) * R,
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->flat_sups, d_sups.as<int>(), sizeof(int) * R,
+                            hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    c->flat = true;
+    c->flat_reports = R;
+    c->flat_edges = E;
+    c->st.reported += F;
 }
 
 // One extension step of a batch of size-m patterns.  write_rows: keep child rows (needed
@@ -1715,6 +1771,17 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         d2h(c, totals, c->step_totals(), 2);
         sync(c);
         resolve_timing(c);
+        if (two_edge && c->flat_ok && B == c->flat_seeds) {
+            emit_flat(c, d_pats.as<PatDev>(), pats, ks, d_fkey.as<int>(), d_fsup.as<int>(), F);
+            out.candidates = A;
+            out.canonical = (long long)totals[1];
+            out.emitted = (long long)totals[0];
+            out.reported_inline = true;
+            account_bytes(out.emitted);
+            c->st.batches++;
+            return STEP_OK;
+        }
+        c->flat_ok = false;  // reports of this run go through the host
         std::vector<int> fkey(F), fcnt(F), fsup(F);
         std::vector<long long> frow(F);
         d2h(c, fkey.data(), d_fkey.as<int>(), F);
@@ -2063,6 +2130,8 @@ void seeds_from_runs(gs_ctx* c) {
     c->st.canonical += codes;
     if (!freq.empty()) {
         std::vector<Layer> layers{Layer{nullptr, rows}};
+        c->flat_ok = true;  // one step over every frequent code may build the reports
+        c->flat_seeds = (int)freq.size();
         mine_level(c, freq, layers);
     }
 }
@@ -2487,7 +2556,12 @@ int gs_create(const int* offsets, const int* vlabels, const int* edges3, int n, 
         x->dg.rkey = x->d_rkey.as<int>();
         x->dg.runid = x->d_runid.as<int>();
         x->dg.nruns = nruns;
+        x->d_vl_vals = DBuf(&x->pool, sizeof(int) * x->vl_vals.size());
+        x->d_el_vals = DBuf(&x->pool, sizeof(int) * x->el_vals.size());
+        h2d(x, x->d_vl_vals.as<int>(), x->vl_vals.data(), x->vl_vals.size());
+        h2d(x, x->d_el_vals.as<int>(), x->el_vals.data(), x->el_vals.size());
         if (two_edge_graph(x)) te_build(x);
+        sync(x);
     });
     if (rc != GS_OK) {
         std::fprintf(stderr, "gs_create: %s\n", c->err.c_str());
@@ -2511,12 +2585,13 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->min_sup = min_support;
         c->mni_mode = c->opts.mni_mode;
         c->reports.clear();
+        c->flat = c->flat_ok = false;
         c->report_order.clear();
         clear_guard(c);
         c->reset_tile_queues();
         mine_seeds(c);
         const auto t_ord = std::chrono::steady_clock::now();
-        order_reports(c);
+        if (!c->flat) order_reports(c);
         c->st.t_order_ms += std::chrono::duration<double, std::milli>(
                                 std::chrono::steady_clock::now() - t_ord)
                                 .count();
@@ -2855,6 +2930,11 @@ int gs_tuner_report(const gs_ctx* c, char* buf, int cap, int* len) {
 
 int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
     if (!c || !count || !total_edges) return GS_ERR_INVALID;
+    if (c->flat) {
+        *count = c->flat_reports;
+        *total_edges = c->flat_edges;
+        return GS_OK;
+    }
     long long e = 0;
     for (int i : c->report_order) e += c->reports[i].m;
     *count = (long long)c->report_order.size();
@@ -2864,6 +2944,12 @@ int gs_result_count(const gs_ctx* c, long long* count, long long* total_edges) {
 
 int gs_result_copy(const gs_ctx* c, int* codes5, int* lens, int* supports) {
     if (!c || (!c->reports.empty() && (!codes5 || !lens || !supports))) return GS_ERR_INVALID;
+    if (c->flat) {
+        std::memcpy(codes5, c->flat_codes, sizeof(int) * 5 * c->flat_edges);
+        std::memcpy(lens, c->flat_lens, sizeof(int) * c->flat_reports);
+        std::memcpy(supports, c->flat_sups, sizeof(int) * c->flat_reports);
+        return GS_OK;
+    }
     long long at = 0, i = 0;
     for_each_report(c, [&](const int* code5, int m, int support) {
         std::memcpy(codes5 + at, code5, sizeof(int) * 5 * m);
@@ -2910,6 +2996,11 @@ int gs_destroy(gs_ctx* c) {
     c->d_te_pairs.reset();
     c->d_te_cstart.reset();
     c->d_te_rbl.reset();
+    c->d_vl_vals.reset();
+    c->d_el_vals.reset();
+    if (c->flat_codes) (void)hipHostFree(c->flat_codes);
+    if (c->flat_lens) (void)hipHostFree(c->flat_lens);
+    if (c->flat_sups) (void)hipHostFree(c->flat_sups);
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();

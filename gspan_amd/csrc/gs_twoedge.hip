@@ -371,6 +371,74 @@ __global__ void k_te_gather(const int* __restrict__ img, const int* __restrict__
         counts[i] = img[8LL * akeys[i / 8] + i % 8];
 }
 
+// Reports of a two-edge last level reached from every frequent single-edge code (the
+// batch, in key order = their report order), in the reference's order: each single-edge
+// code, then its frequent children in key order -- within a parent, key order (segment of
+// the rightmost vertex first, then edge label, then label) is the order of their last edges.
+// fkey / fsup: the F frequent children, keys ascending, and their supports; ssup: the
+// codes' supports; vl_vals / el_vals: original label of each rank.  Report r of the output
+// has lens[r] edges at codes[5 * (first edge of r)]; single-edge code p is report
+// p + C_p with its edge at p + 2 C_p, C_p = children of the codes before it.
+__device__ __forceinline__ int te_lower_bound(const int* __restrict__ a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_te_emit(const PatDev* __restrict__ pats, int npats, KeySpace ks,
+                          const int* __restrict__ fkey, const int* __restrict__ fsup, int F,
+                          const int* __restrict__ ssup, const int* __restrict__ vl_vals,
+                          const int* __restrict__ el_vals, int* __restrict__ codes,
+                          int* __restrict__ lens, int* __restrict__ sups) {
+    const long long total = (long long)npats + F;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        int p, i = -1;
+        if (t < npats) {
+            p = (int)t;
+        } else {
+            i = (int)(t - npats);
+            const int key = fkey[i];
+            int lo = 0, hi = npats - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pats[mid].key_base <= key) lo = mid;
+                else hi = mid - 1;
+            }
+            p = lo;
+        }
+        const PatDev& P = pats[p];
+        const long long cp = te_lower_bound(fkey, F, P.key_base);
+        long long r = p + cp, e = p + 2 * cp;
+        if (i >= 0) {
+            const long long j = i - cp;
+            r += 1 + j;
+            e += 1 + 2 * j;
+        }
+        int* out = codes + 5 * e;
+        out[0] = 0;
+        out[1] = 1;
+        out[2] = vl_vals[P.vlab[0]];
+        out[3] = el_vals[P.cel[0]];
+        out[4] = vl_vals[P.vlab[1]];
+        lens[r] = i < 0 ? 1 : 2;
+        sups[r] = i < 0 ? ssup[p] : fsup[i];
+        if (i >= 0) {  // the child's edge, as its key names it (no backward slots)
+            const int s2 = fkey[i] - P.key_base - (P.r - 1) * ks.EL;
+            const int seg = s2 / (ks.EL * ks.VL);
+            out[5] = seg == 0 ? P.v_n : P.cfrom[P.rm[seg - 1]];
+            out[6] = P.v_n + 1;
+            out[7] = -1;
+            out[8] = el_vals[(s2 / ks.VL) % ks.EL];
+            out[9] = vl_vals[s2 % ks.VL];
+        }
+    }
+}
+
 inline int te_grid(long long n, int block, int cap) {
     long long g = (n + block - 1) / block;
     return (int)(g < 1 ? 1 : (g > cap ? cap : g));
@@ -447,6 +515,16 @@ hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, in
     if (NC <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_te_classes, dim3(te_grid(NC, 4, 8192)), dim3(256), 0, s, rec, cstart, NC,
                        npairs, nlen);
+    return hipGetLastError();
+}
+
+hipError_t te_emit(const PatDev* pats, int npats, KeySpace ks, const int* fkey, const int* fsup,
+                   int F, const int* ssup, const int* vl_vals, const int* el_vals, int* codes,
+                   int* lens, int* sups, hipStream_t s) {
+    const long long total = (long long)npats + F;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_te_emit, dim3(te_grid(total, 256, 8192)), dim3(256), 0, s, pats, npats,
+                       ks, fkey, fsup, F, ssup, vl_vals, el_vals, codes, lens, sups);
     return hipGetLastError();
 }
 

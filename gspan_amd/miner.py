@@ -102,9 +102,9 @@ class Miner:
     def results(self):
         n, e = ctypes.c_longlong(), ctypes.c_longlong()
         self._check(self._lib.gs_result_count(self._h, ctypes.byref(n), ctypes.byref(e)))
-        codes = np.zeros((e.value, 5), np.int32)
-        lens = np.zeros(n.value, np.int32)
-        sups = np.zeros(n.value, np.int32)
+        codes = np.empty((e.value, 5), np.int32)
+        lens = np.empty(n.value, np.int32)
+        sups = np.empty(n.value, np.int32)
         self._check(self._lib.gs_result_copy(self._h, _ip(codes), _ip(lens), _ip(sups)))
         return codes, lens, sups
 
