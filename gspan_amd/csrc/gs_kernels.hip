@@ -1840,6 +1840,162 @@ __device__ int canon_wave(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, in
     }
 }
 
+// canon_wave's walk run by one thread, for codes of at most three edges: at most 3!
+// injective maps of the code's edges onto the pattern graph's, in 2^3 directions, so at
+// most 48 projections at any step (CAPL = 64 leaves room; err bit 0 past it).  The
+// reductions over projections become loops; every rule is the one canon_wave applies.
+template <int CAPL>
+__device__ int canon_seq(const CanonCode& C, uint32_t* bufA, uint32_t* bufB, int* err) {
+    const int m = C.m;
+    if (m == 1) return 1;  // gspan.cu:592-593
+    long long kmin = KEY_NONE;
+    for (int e = 0; e < 2 * m; ++e) {
+        const int j = e >> 1;
+        const int x = (e & 1) ? C.ct[j] : C.cf[j], y = (e & 1) ? C.cf[j] : C.ct[j];
+        if (C.lab[x] <= C.lab[y]) {
+            const long long key =
+                ((long long)C.lab[x] << 32) | ((long long)C.cel[j] << 16) | C.lab[y];
+            if (key < kmin) kmin = key;
+        }
+    }
+    int P = 0;
+    for (int e = 0; e < 2 * m; ++e) {
+        const int j = e >> 1;
+        const int x = (e & 1) ? C.ct[j] : C.cf[j], y = (e & 1) ? C.cf[j] : C.ct[j];
+        if (C.lab[x] > C.lab[y]) continue;
+        const long long key = ((long long)C.lab[x] << 32) | ((long long)C.cel[j] << 16) | C.lab[y];
+        if (key == kmin) bufA[P++] = (uint32_t)x | ((uint32_t)y << 3) | (1u << (24 + j));
+    }
+    uint32_t* cur = bufA;
+    uint32_t* nxt = bufB;
+    int mf[GS_MAXE], mt[GS_MAXE], mel[GS_MAXE];
+    mf[0] = 0;
+    mt[0] = 1;
+    mel[0] = (int)((kmin >> 16) & 0xffff);
+    const int min_label = (int)(kmin >> 32);
+    int len = 1;
+    for (;;) {
+        int rm[GS_MAXE];
+        int r = 0, last = -1;
+        for (int i = len - 1; i >= 0; --i)
+            if (mf[i] < mt[i] && (r == 0 || mt[i] == last)) {
+                rm[r++] = i;
+                last = mf[i];
+            }
+        const int v_n = mt[rm[0]];
+        const int nvm = v_n + 1;
+        int bi = -1, bl = 0x7fffffff;
+        for (int i = r - 1; i >= 1 && bi < 0; --i) {
+            const int k = rm[i];
+            int best = 0x7fffffff;
+            for (int pi = 0; pi < P; ++pi) {
+                int je;
+                const int l = canon_backward(C, cur[pi], mf[k], mt[k], mel[k], v_n, &je);
+                if (l >= 0 && l < best) best = l;
+            }
+            if (best != 0x7fffffff) {
+                bi = i;
+                bl = best;
+            }
+        }
+        if (bi >= 0) {
+            const int k = rm[bi];
+            if (len >= m) {
+                *err |= 2;
+                return 1;
+            }
+            if (!(C.cf[len] == v_n && C.ct[len] == mf[k] && C.cel[len] == bl && C.ctl[len] < 0))
+                return 0;
+            int np = 0;
+            for (int pi = 0; pi < P; ++pi) {
+                int je;
+                const uint32_t pr = cur[pi];
+                if (canon_backward(C, pr, mf[k], mt[k], mel[k], v_n, &je) != bl) continue;
+                if (np == CAPL) {
+                    *err |= 1;
+                    return 1;
+                }
+                nxt[np++] = pr | (1u << (24 + je));
+            }
+            mf[len] = v_n;
+            mt[len] = mf[k];
+            mel[len] = bl;
+            ++len;
+            uint32_t* t = cur;
+            cur = nxt;
+            nxt = t;
+            P = np;
+            continue;
+        }
+        int seg = -1;
+        long long fkey = KEY_NONE;
+        for (int sg = 0; sg <= r && seg < 0; ++sg) {
+            long long best = KEY_NONE;
+            for (int pi = 0; pi < P; ++pi) {
+                const uint32_t pr = cur[pi];
+                int u, e_el = 0, bound = 0;
+                if (sg == 0) {
+                    u = pj_phi(pr, v_n);
+                } else {
+                    const int k = rm[sg - 1];
+                    u = pj_phi(pr, mf[k]);
+                    e_el = mel[k];
+                    bound = C.lab[pj_phi(pr, mt[k])];
+                }
+                for (int j = m - 1; j >= 0; --j) {
+                    int w;
+                    if (C.cf[j] == u) w = C.ct[j];
+                    else if (C.ct[j] == u) w = C.cf[j];
+                    else continue;
+                    const int l = C.cel[j], tl = C.lab[w];
+                    if (tl < min_label || pj_has_vertex(pr, nvm, w)) continue;
+                    if (sg > 0 && !(e_el < l || (e_el == l && bound <= tl))) continue;
+                    const long long kk = ((long long)l << 32) | tl;
+                    if (kk < best) best = kk;
+                }
+            }
+            if (best != KEY_NONE) {
+                seg = sg;
+                fkey = best;
+            }
+        }
+        if (seg < 0) return 1;  // gspan.cu:583
+        const int fel = (int)(fkey >> 32), ftl = (int)(fkey & 0xffffffff);
+        const int v_i = seg == 0 ? v_n : mf[rm[seg - 1]];
+        if (len >= m) {
+            *err |= 2;
+            return 1;
+        }
+        if (!(C.cf[len] == v_i && C.ct[len] == v_n + 1 && C.cel[len] == fel && C.ctl[len] == ftl))
+            return 0;
+        int np = 0;
+        for (int pi = 0; pi < P; ++pi) {
+            const uint32_t pr = cur[pi];
+            const int u = pj_phi(pr, v_i);
+            for (int j = m - 1; j >= 0; --j) {
+                int w;
+                if (C.cf[j] == u) w = C.ct[j];
+                else if (C.ct[j] == u) w = C.cf[j];
+                else continue;
+                if (C.cel[j] != fel || C.lab[w] != ftl || pj_has_vertex(pr, nvm, w)) continue;
+                if (np == CAPL) {
+                    *err |= 1;
+                    return 1;
+                }
+                nxt[np++] = pr | ((uint32_t)w << (3 * (v_n + 1))) | (1u << (24 + j));
+            }
+        }
+        mf[len] = v_i;
+        mt[len] = v_n + 1;
+        mel[len] = fel;
+        ++len;
+        uint32_t* t = cur;
+        cur = nxt;
+        nxt = t;
+        P = np;
+    }
+}
+
 // is_canonical of a two-edge code (gspan.cu:586-606 with projection_is_canonical,
 // gspan.cu:485-584), walked on the code's pattern graph held in registers: at most three
 // vertices and two edges, i.e. four directed entries.  The walk starts from every entry
@@ -1991,6 +2147,31 @@ __device__ void decode_child(const PatDev& Pp, int slot, KeySpace ks, CanonCode&
     }
 }
 
+// Canonicality of three-edge children: one thread per active key runs canon_seq.
+__global__ __launch_bounds__(GS_BLOCK) void k_canon_three(const PatDev* __restrict__ pats,
+                                                          int npats, KeySpace ks,
+                                                          const int* active_keys,
+                                                          const int* n_active, int* flags,
+                                                          int* err) {
+    const long long na = *n_active;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
+        const int key = active_keys[a];
+        int lo = 0, hi = npats - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pats[mid].key_base <= key) lo = mid;
+            else hi = mid - 1;
+        }
+        CanonCode C;
+        decode_child(pats[lo], key - pats[lo].key_base, ks, C);
+        uint32_t bufA[64], bufB[64];
+        int e = 0;
+        flags[a] = canon_seq<64>(C, bufA, bufB, &e);
+        if (e) atomicOr(err, e);
+    }
+}
+
 template <int CAP>
 __global__ __launch_bounds__(64) void k_canon(const PatDev* __restrict__ pats, int npats,
                                               KeySpace ks, const int* active_keys,
@@ -2046,9 +2227,15 @@ __global__ __launch_bounds__(64) void k_canon_codes(const int* codes, int ncodes
         }
         __syncthreads();
         int e = 0;
-        // C.m is uniform: two-edge codes take the register walk (thread 0)
-        const int f = C.m == 2 ? (threadIdx.x == 0 ? canon_two(C) : 0)
-                               : canon_wave<CAP>(C, bufA, bufB, mf, mt, mel, &e);
+        // C.m is uniform: two- and three-edge codes take the one-thread walks (thread 0)
+        int f = 0;
+        if (C.m == 2) {
+            if (threadIdx.x == 0) f = canon_two(C);
+        } else if (C.m == 3) {
+            if (threadIdx.x == 0) f = canon_seq<64>(C, bufA, bufB, &e);
+        } else {
+            f = canon_wave<CAP>(C, bufA, bufB, mf, mt, mel, &e);
+        }
         if (threadIdx.x == 0) {
             flags[a] = f;
             if (e) atomicOr(err, e);
@@ -2463,6 +2650,12 @@ hipError_t canon(const PatDev* pats, int npats, KeySpace ks, const int* active_k
     if (child_m == 2) {
         hipLaunchKernelGGL(k_canon_two, dim3(grid_for(max_active, GS_BLOCK, 4096)), dim3(GS_BLOCK),
                            0, s, pats, npats, ks, active_keys, n_active, flags);
+        return hipGetLastError();
+    }
+    if (child_m == 3) {
+        hipLaunchKernelGGL(k_canon_three, dim3(grid_for(max_active, GS_BLOCK, 8192)),
+                           dim3(GS_BLOCK), 0, s, pats, npats, ks, active_keys, n_active, flags,
+                           err);
         return hipGetLastError();
     }
     int grid = max_active < 16384 ? max_active : 16384;

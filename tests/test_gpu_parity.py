@@ -66,6 +66,24 @@ def test_canonical_two_edge_codes_exhaustive(oracle_mod):
     assert 20 < sum(want) < len(want)
 
 
+def test_canonical_three_edge_codes(oracle_mod):
+    """Random three-edge codes (paths, stars, triangles, in every DFS order, canonical or
+    not) against the oracle: the kernels walk codes of three edges with one thread."""
+    _announce("canonical three-edge codes")
+    g = synthetic("er", n=60, m=150, n_vlabels=3, n_elabels=2, seed=3)
+    rng = np.random.default_rng(11)
+    codes = []
+    while len(codes) < 3000:
+        c = brute.random_code(rng, 3, int(rng.integers(1, 4)), int(rng.integers(1, 3)))
+        if len(c) == 3:
+            codes.append(c)
+    with Miner(g) as mn:
+        got = mn.is_canonical(codes)
+    want = [oracle_mod.is_canonical(c) == 1 for c in codes]
+    assert got == want
+    assert 100 < sum(want) < len(want)
+
+
 def test_canonical_star_projection_count(oracle_mod):
     # K_{1,6} with one label: its minimum code starts at a leaf and walks the other five
     # leaves from the centre; the walk carries up to 6! = 720 projections (cap 2048).
