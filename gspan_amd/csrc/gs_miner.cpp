@@ -642,21 +642,36 @@ void order_reports(gs_ctx* c) {
     // group g (parent g - 1) occupies kids[start[g] .. start[g+1]); children reported in
     // key order mostly arrive in edge order already
     auto by_edge = [&](int a, int b) { return r[a].edge_key < r[b].edge_key; };
-    for (int g = 0; g <= n; ++g) {
-        const auto b0 = kids.begin() + start[g], b1 = kids.begin() + start[g + 1];
-        if (!std::is_sorted(b0, b1, by_edge)) std::sort(b0, b1, by_edge);
+    parallel_ranges((long long)n + 1, [&](long long g0, long long g1) {
+        for (long long g = g0; g < g1; ++g) {
+            const auto b0 = kids.begin() + start[g], b1 = kids.begin() + start[g + 1];
+            if (!std::is_sorted(b0, b1, by_edge)) std::sort(b0, b1, by_edge);
+        }
+    });
+    // Pre-order positions without a walk: a report's parent is reported before it, so
+    // subtree sizes add up in one backward pass; then each parent (ascending index, so its
+    // own position is set) places its children's subtrees one after another in edge order.
+    std::vector<int> size(n, 1), pos(n);
+    for (int i = n - 1; i >= 0; --i) {
+        const int q = r[i].parent;
+        if (q >= i) throw GsError(GS_ERR_INTERNAL, "report before its parent");
+        if (q >= 0) size[q] += size[i];
     }
-    std::vector<int> order;
-    order.reserve(n);
-    std::vector<int> stack;
-    for (int i = start[1] - 1; i >= start[0]; --i) stack.push_back(kids[i]);
-    while (!stack.empty()) {
-        const int x = stack.back();
-        stack.pop_back();
-        order.push_back(x);
-        for (int i = start[x + 2] - 1; i >= start[x + 1]; --i) stack.push_back(kids[i]);
+    int at = 0;
+    for (int i = start[0]; i < start[1]; ++i) {
+        pos[kids[i]] = at;
+        at += size[kids[i]];
     }
-    if ((int)order.size() != n) throw GsError(GS_ERR_INTERNAL, "report tree is not connected");
+    if (at != n) throw GsError(GS_ERR_INTERNAL, "report tree is not connected");
+    for (int g = 1; g <= n; ++g) {
+        int q = pos[g - 1] + 1;
+        for (int i = start[g]; i < start[g + 1]; ++i) {
+            pos[kids[i]] = q;
+            q += size[kids[i]];
+        }
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[pos[i]] = i;
     c->report_order.swap(order);
 }
 
@@ -702,7 +717,7 @@ int add_report(gs_ctx* c, const HPat& h) {
 // Appends the report of the child of parent pattern p (device pattern d) at key slot `slot`
 // with its support, straight from the key: the last edge alone is decoded (as in
 // decode_child_into), for children that are reported and not extended.
-void report_child(gs_ctx* c, const HPat& p, const PatDev& d, int slot, int support) {
+Report child_report(const gs_ctx* c, const HPat& p, const PatDev& d, int slot, int support) {
     const int EL = c->EL, VL = c->VL;
     const int nb = (d.r - 1) * EL;
     int f, t, el, tl;
@@ -731,7 +746,11 @@ void report_child(gs_ctx* c, const HPat& p, const PatDev& d, int slot, int suppo
     r.edge_key = f > t ? put(0, t + 8, el, 0) : put(1, 8 - f, el, tl);  // as edge_word
     r.parent = p.report_id;
     r.support = support;
-    c->reports.push_back(r);
+    return r;
+}
+
+void report_child(gs_ctx* c, const HPat& p, const PatDev& d, int slot, int support) {
+    c->reports.push_back(child_report(c, p, d, slot, support));
     c->st.reported++;
 }
 
@@ -1797,13 +1816,21 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         }
         int p = 0;  // pattern owning `key`: keys ascend, so it only moves forward
         if (!write_rows) {  // the last level: frequent children are reported, not extended
-            c->reports.reserve(c->reports.size() + F);
-            for (int i = 0; i < F; ++i) {
-                const int key = fkey[i];
-                while (p + 1 < B && pd[p + 1].key_base <= key) ++p;
-                if (fsup[i] >= c->min_sup)
-                    report_child(c, pats[p], pd[p], key - pd[p].key_base, fsup[i]);
-            }
+            // (every child here is frequent: its support passed min_sup on the device);
+            // decoded on the host threads, each range from its own first pattern
+            const size_t base = c->reports.size();
+            c->reports.resize(base + F);
+            parallel_ranges(F, [&](long long i0, long long i1) {
+                if (i0 >= i1) return;
+                int q = owner_of(pd, fkey[i0]);
+                for (long long i = i0; i < i1; ++i) {
+                    const int key = fkey[i];
+                    while (q + 1 < B && pd[q + 1].key_base <= key) ++q;
+                    c->reports[base + i] =
+                        child_report(c, pats[q], pd[q], key - pd[q].key_base, fsup[i]);
+                }
+            });
+            c->st.reported += F;
             F = 0;
             out.reported_inline = true;
         }
