@@ -16,6 +16,9 @@
 #define GS_BLOCK 256
 #define GS_TILE 1024
 
+// Work counters of the two-edge run pass: slot t holds {runs, words} at work[2 t].
+#define GS_TE_WORK_SLOTS 64
+
 // XCD-aware tile queues of the item kernels: one counter per XCD, 16 ints apart.
 #define GS_XCDS 8
 #define GS_XCD_STRIDE 16
@@ -401,8 +404,9 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
                       const SetDesc* key_sets, SetDesc* sets, int* key_slot,
                       unsigned long long* cursor, hipStream_t s);
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
-                    const int* acnt, const int* flags, int na, const int* counts, int mni_mode,
-                    int min_sup, int* sup, unsigned long long* totals, hipStream_t s);
+                    const int* acnt, const int* flags, int na, const int* counts,
+                    const int* ksup, int mni_mode, int min_sup, int* sup,
+                    unsigned long long* totals, hipStream_t s);
 // Sets of every potential child key of a batch (single-pass last level): each key's vertex
 // sets are bitmaps or bytemaps over their label classes, lw[l] words for label rank l and
 // lcum[l] = lw[0] + ... + lw[l-1] (VL + 1 entries); a pattern's keys start at its set_base:
@@ -424,30 +428,32 @@ hipError_t set_counts(const SetDesc* sets, int nsets, const uint32_t* pool,
                       const int* rows = nullptr, int min_rows = 0);
 // Two-edge last level (gs_twoedge.hip).  te_prepare: run masks m1 / m2 (W words per
 // vertex, zeroed here) and the (vertex, run) pairs ordered by class (vertex label * NB +
-// run bit), two int4 records per pair, with hist (VL * NB ints), cstart (VL * NB + 1: the
-// class starts), rbl ({bit, length} of every run) and scan scratch.  te_clear: zeroes the
-// count tables (CNT, CEN, END: VL * NB * NB entries each) and the queue.  te_level:
-// the count tables from the pairs,
-// then key_count and img[8 * key + v] for the batch's K keys; p_of: batch index of each
-// single-edge parent (a, e0, b) at (a * EL + e0) * VL + b, or -1; pmask[l * W + w]: word w
-// of the run bits (l', e0) with (l', e0, l) a batch parent (either label order); queue: one int;
-// work[0..1] += runs visited, mask words read.  te_gather: counts[8 a + v] =
-// img[8 * akeys[a] + v] for a < *n_active (n_active null: a < max_active).
+// run bit), two int4 records per pair and their {class, run length} (pcl), with hist (VL * NB
+// ints), cstart (VL * NB + 1: the class starts), rbl ({bit, length} of every run) and scan
+// scratch.  te_clear: zeroes the count tables (CNT, CEN: 2 * npats * NB entries, two rows
+// per single-edge parent; END: VL * NB * NB entries) and the queue.  te_level: the count
+// tables from the pairs, then key_count and the child's support ksup (mni_mode rules;
+// INT_MAX when no image count is non-zero) for the batch's K keys; p_of: batch index of
+// each single-edge parent (a, e0, b), a <= b, at (a * EL + e0) * VL + b, or -1;
+// pmask[l * W + w]: word w of the run bits (l', e0) with (l', e0, l) a batch parent (either
+// label order); queue: one int; work[2 t], work[2 t + 1] += runs visited, mask words read
+// (the sums over the slots t < GS_TE_WORK_SLOTS are the pass's).  te_gather:
+// out[a] = ksup[akeys[a]] for a < *n_active (n_active null: a < max_active).
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
                       int* hist, long long* cstart, long long* scan_tmp, long long scan_elems,
-                      int4* rec, int2* rbl, hipStream_t s);
+                      int4* rec, int2* pcl, int2* rbl, hipStream_t s);
 // per class c < NC: npairs[c] = its pairs, nlen[c] = CSR entries of its runs
-hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, int* npairs,
-                      unsigned long long* nlen, hipStream_t s);
-hipError_t te_clear(KeySpace ks, int* queue, unsigned long long* CNT, int* CEN, int* END,
-                    hipStream_t s);
+hipError_t te_classes(const int2* pcl, long long np, const long long* cstart, long long NC,
+                      int* npairs, unsigned long long* nlen, hipStream_t s);
+hipError_t te_clear(KeySpace ks, int npats, int* queue, unsigned long long* CNT, int* CEN,
+                    int* END, hipStream_t s);
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     const int2* rbl, const uint32_t* pmask, int* queue,
-                    unsigned long long* CNT, int* CEN, int* END, int* key_count, int* img,
-                    unsigned long long* work, hipStream_t s);
-hipError_t te_gather(const int* img, const int* akeys, const int* n_active, int max_active,
-                     int* counts, hipStream_t s);
+                    unsigned long long* CNT, int* CEN, int* END, int mni_mode, int* key_count,
+                    int* ksup, unsigned long long* work, hipStream_t s);
+hipError_t te_gather(const int* ksup, const int* akeys, const int* n_active, int max_active,
+                     int* out, hipStream_t s);
 // The reports of a two-edge last level over every frequent single-edge code (pats, in key
 // order), in the reference's order: each code (support ssup[p]) and then its frequent
 // children (fkey ascending, supports fsup); codes5 / lens / sups of npats + F reports and

@@ -2345,11 +2345,13 @@ __global__ void k_build_sets(const PatDev* __restrict__ pats, int npats, KeySpac
 // Support of the child of active key a (gspan.cu:282-288 / 367-373: the least non-zero
 // count of distinct images over its vertices, vertex 0 only where it takes part);
 // sup[a] = the support when it reaches min_sup, else 0.  Totals: [0] += child rows of all
-// active keys (a negative count adds 2^31), [1] += canonical keys.
+// active keys (a negative count adds 2^31), [1] += canonical keys.  ksup (not null): the
+// support of every key, computed with its counts, in place of counts.
 __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace ks,
                            const int* __restrict__ akeys, const int* __restrict__ acnt,
                            const int* __restrict__ flags, int na, const int* __restrict__ counts,
-                           int mni_mode, int min_sup, int* sup, unsigned long long* totals) {
+                           const int* __restrict__ ksup, int mni_mode, int min_sup, int* sup,
+                           unsigned long long* totals) {
     __shared__ unsigned long long s_t[2][GS_BLOCK / 64];
     unsigned long long rows = 0, canon = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -2360,7 +2362,10 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
         if (flags[a]) ++canon;
         // a support never exceeds the child count (each child row adds at most one image
         // to each vertex set)
-        if (flags[a] && cr >= (long long)min_sup) {
+        if (flags[a] && cr >= (long long)min_sup && ksup) {
+            const int s = ksup[akeys[a]];
+            out = s >= min_sup ? s : 0;
+        } else if (flags[a] && cr >= (long long)min_sup) {
             const int key = akeys[a];
             const PatDev& P = pats[owner_of(pats, npats, key)];
             int nv, lab[GS_MAXV];
@@ -2735,12 +2740,14 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
 }
 
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
-                    const int* acnt, const int* flags, int na, const int* counts, int mni_mode,
-                    int min_sup, int* sup, unsigned long long* totals, hipStream_t s) {
+                    const int* acnt, const int* flags, int na, const int* counts,
+                    const int* ksup, int mni_mode, int min_sup, int* sup,
+                    unsigned long long* totals, hipStream_t s) {
     if (na <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_supports, dim3(grid_for(na, GS_BLOCK, 16384)), dim3(GS_BLOCK), 0, s,
-                       pats, npats, ks, akeys, acnt, flags, na, counts, mni_mode, min_sup, sup,
-                       totals);
+    // every block adds its totals with two atomics on the same two words: few blocks
+    hipLaunchKernelGGL(k_supports, dim3(grid_for(na, GS_BLOCK, 512)), dim3(GS_BLOCK), 0, s,
+                       pats, npats, ks, akeys, acnt, flags, na, counts, ksup, mni_mode, min_sup,
+                       sup, totals);
     return hipGetLastError();
 }
 

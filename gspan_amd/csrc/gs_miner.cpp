@@ -310,7 +310,7 @@ struct gs_ctx {
     // pairs ordered by class, built on first use (they depend on the graph only)
     // and the first pair of every class (vertex label * VL * EL + run bit; VL * VL * EL + 1
     // entries); built by gs_create
-    DBuf d_te_m1, d_te_m2, d_te_pairs, d_te_cstart, d_te_rbl;
+    DBuf d_te_m1, d_te_m2, d_te_pairs, d_te_pcl, d_te_cstart, d_te_rbl;
     bool te_ready = false;
     DBuf d_vl_vals, d_el_vals;  // original label of each rank, on the device
     // A run whose two-edge last level is reached from every frequent single-edge code
@@ -1079,6 +1079,7 @@ void te_build(gs_ctx* c) {
     c->d_te_m1 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
     c->d_te_m2 = DBuf(P, 4 * std::max<long long>((long long)c->n * W, 1));
     c->d_te_pairs = DBuf(P, 2 * sizeof(int4) * std::max(c->dg.nruns, 1));
+    c->d_te_pcl = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
     c->d_te_cstart = DBuf(P, 8 * (NC + 1));
     c->d_te_rbl = DBuf(P, sizeof(int2) * std::max(c->dg.nruns, 1));
     const long long te = gsk::scan_tmp_elems(NC);
@@ -1086,7 +1087,8 @@ void te_build(gs_ctx* c) {
     HIPCHECK(gsk::te_prepare(c->dg, c->EL, NB, W, c->d_te_m1.as<uint32_t>(),
                              c->d_te_m2.as<uint32_t>(), hist.as<int>(),
                              c->d_te_cstart.as<long long>(), tmp.as<long long>(), te,
-                             c->d_te_pairs.as<int4>(), c->d_te_rbl.as<int2>(), c->stream));
+                             c->d_te_pairs.as<int4>(), c->d_te_pcl.as<int2>(),
+                             c->d_te_rbl.as<int2>(), c->stream));
     sync(c);
     c->te_ready = true;
 }
@@ -1268,10 +1270,10 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                    d_lcum.as<int>(), single_bytes ? GS_SET_BYTEMAP : GS_SET_BITMAP,
                                    all_tag, d_all_sets.as<SetDesc>(), c->stream));
     }
-    if (two_edge) single = true;  // one pass, no fill pass below; supports from d_te_img
-    DBuf d_te_img;                  // image counts of every key, 8 per key
+    if (two_edge) single = true;  // one pass, no fill pass below; supports from d_te_sup
+    DBuf d_te_sup;                  // support of every key's child
     DBuf d_te_work;                 // work of the run pass (runs visited, mask words read)
-    unsigned long long te_work[2] = {0, 0};
+    unsigned long long te_work[2 * GS_TE_WORK_SLOTS] = {};
 
     long long W_total = 0;
     unsigned long long W_full = 0;
@@ -1565,26 +1567,26 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         DBuf d_pof(P, sizeof(int) * p_of.size()), d_pmask(P, sizeof(uint32_t) * pmask.size());
         h2d(c, d_pof.as<int>(), p_of.data(), p_of.size());
         h2d(c, d_pmask.as<uint32_t>(), pmask.data(), pmask.size());
-        const long long rows = (long long)c->VL * NB * NB;
-        DBuf d_cnt(P, 8 * rows), d_cen(P, 4 * rows), d_end(P, 4 * rows), d_q(P, 4);
-        d_te_img = DBuf(P, sizeof(int) * 8 * K);
-        d_te_work = DBuf(P, 2 * sizeof(unsigned long long));
-        HIPCHECK(gsk::fill_i32(d_te_work.as<int>(), 0, 4, c->stream));
+        const long long rows = (long long)c->VL * NB * NB, prows = 2LL * B * NB;
+        DBuf d_cnt(P, 8 * prows), d_cen(P, 4 * prows), d_end(P, 4 * rows), d_q(P, 4);
+        d_te_sup = DBuf(P, sizeof(int) * K);
+        d_te_work = DBuf(P, sizeof(te_work));
+        HIPCHECK(gsk::fill_i32(d_te_work.as<int>(), 0, 4 * GS_TE_WORK_SLOTS, c->stream));
         Wc_host[0] = c->dg.nruns;  // (vertex, run) pairs
-        HIPCHECK(gsk::te_clear(ks, d_q.as<int>(), d_cnt.as<unsigned long long>(), d_cen.as<int>(),
-                               d_end.as<int>(), c->stream));
+        HIPCHECK(gsk::te_clear(ks, B, d_q.as<int>(), d_cnt.as<unsigned long long>(),
+                               d_cen.as<int>(), d_end.as<int>(), c->stream));
         time_begin();
         HIPCHECK(gsk::te_level(c->dg, ks, d_pats.as<PatDev>(), B, K, d_pof.as<int>(),
                                c->d_te_m1.as<uint32_t>(), c->d_te_m2.as<uint32_t>(),
                                c->d_te_pairs.as<int4>(), c->d_te_rbl.as<int2>(),
                                d_pmask.as<uint32_t>(), d_q.as<int>(),
                                d_cnt.as<unsigned long long>(), d_cen.as<int>(), d_end.as<int>(),
-                               d_kc.as<int>(), d_te_img.as<int>(),
+                               c->mni_mode, d_kc.as<int>(), d_te_sup.as<int>(),
                                d_te_work.as<unsigned long long>(), c->stream));
         time_end(1, &Wc_host[0]);
         c->st.fill_launches++;
         c->st.two_edge_levels++;
-        d2h(c, te_work, d_te_work.as<unsigned long long>(), 2);
+        d2h(c, te_work, d_te_work.as<unsigned long long>(), 2 * GS_TE_WORK_SLOTS);
     }
     for (int ci = 0; ci < (two_edge ? 0 : nchunks); ++ci) {
         const long long b0 = (long long)ci * chunk;
@@ -1655,10 +1657,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     auto account_bytes = [&](long long emitted) {
         if (two_edge) {  // per pair its 32-byte record; per run visited its 8-byte {bit,
                          // length}; per adjacency or mask word 4 B; per key 4 table reads
-                         // (20 B) and its
-                         // count and 8 image counts (36 B)
-            c->st.alg_bytes_fill += 32.0 * (double)c->dg.nruns + 8.0 * (double)te_work[0] +
-                                    4.0 * (double)te_work[1] + 56.0 * (double)K;
+                         // (20 B), its count and its support (8 B)
+            double runs = 0, words = 0;
+            for (int t = 0; t < GS_TE_WORK_SLOTS; ++t) {
+                runs += (double)te_work[2 * t];
+                words += (double)te_work[2 * t + 1];
+            }
+            c->st.alg_bytes_fill += 32.0 * (double)c->dg.nruns + 8.0 * runs + 4.0 * words +
+                                    28.0 * (double)K;
             return;
         }
         double bytes = 16.0 * (double)W_full;
@@ -1762,11 +1768,9 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
             }
         }
         // supports, then the frequent children in key order
-        DBuf d_counts(P, sizeof(int) * (size_t)A * GS_MAXV), d_sup(P, sizeof(int) * A);
-        if (two_edge)
-            HIPCHECK(gsk::te_gather(d_te_img.as<int>(), d_active.as<int>(), c->n_active(), A,
-                                    d_counts.as<int>(), c->stream));
-        else
+        DBuf d_counts(P, two_edge ? 4 : sizeof(int) * (size_t)A * GS_MAXV);
+        DBuf d_sup(P, sizeof(int) * A);
+        if (!two_edge)
             HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), A,
                                      single ? all_pool : d_pool.as<uint32_t>(),
                                      std::max<long long>(single ? all_words : pool_words, 4),
@@ -1774,7 +1778,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                      d_acnt.as<int>(), c->min_sup));
         HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
         HIPCHECK(gsk::supports(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), d_acnt.as<int>(),
-                               d_flags.as<int>(), A, d_counts.as<int>(), c->mni_mode, c->min_sup,
+                               d_flags.as<int>(), A, d_counts.as<int>(),
+                               two_edge ? d_te_sup.as<int>() : nullptr, c->mni_mode, c->min_sup,
                                d_sup.as<int>(), c->step_totals(), c->stream));
         DBuf d_fidx(P, sizeof(int) * A), d_fsup(P, sizeof(int) * A);
         compact_ordered(c, d_sup.as<int>(), A, d_fidx.as<int>(), d_fsup.as<int>());
@@ -2018,8 +2023,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     if (two_edge) {
         d_lkeys = DBuf(P, sizeof(int) * std::max(NL, 1));
         h2d(c, d_lkeys.as<int>(), live_keys.data(), NL);
-        HIPCHECK(gsk::te_gather(d_te_img.as<int>(), d_lkeys.as<int>(), nullptr, NL,
-                                d_counts.as<int>(), c->stream));
+        HIPCHECK(gsk::te_gather(d_te_sup.as<int>(), d_lkeys.as<int>(), nullptr, NL,
+                                d_counts.as<int>(), c->stream));  // one support per key
     } else {
         HIPCHECK(gsk::set_counts(d_sets.as<SetDesc>(), NL,
                                  single ? all_pool : d_pool.as<uint32_t>(),
@@ -2028,15 +2033,15 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                                  all_live ? nullptr : d_lrows.as<int>(), c->min_sup));
     }
     std::vector<int> counts((size_t)NL * GS_MAXV);
-    d2h(c, counts.data(), d_counts.as<int>(), counts.size());
+    d2h(c, counts.data(), d_counts.as<int>(), two_edge ? (size_t)NL : counts.size());
     sync(c);
     resolve_timing(c);
     int li = 0;
     for (auto& ch : out.children) {
         if (!(ch.canonical || all_live)) continue;
-        ch.support = !all_live && ch.count < c->min_sup
-                         ? 0
-                         : support_of(ch.pat, &counts[(size_t)li * GS_MAXV], c->mni_mode);
+        ch.support = !all_live && ch.count < c->min_sup ? 0
+                     : two_edge ? counts[li]
+                                : support_of(ch.pat, &counts[(size_t)li * GS_MAXV], c->mni_mode);
         ch.pat.support = ch.support;
         ch.pat.row_begin = ch.row_off;
         ch.pat.row_count = ch.count;
@@ -2117,8 +2122,9 @@ void seeds_from_runs(gs_ctx* c) {
     const long long NC = (long long)VL * NB;
     DevPool* P = &c->pool;
     DBuf d_np(P, sizeof(int) * NC), d_nl(P, sizeof(long long) * NC);
-    HIPCHECK(gsk::te_classes(c->d_te_pairs.as<int4>(), c->d_te_cstart.as<long long>(), NC,
-                             d_np.as<int>(), d_nl.as<unsigned long long>(), c->stream));
+    HIPCHECK(gsk::te_classes(c->d_te_pcl.as<int2>(), c->dg.nruns,
+                             c->d_te_cstart.as<long long>(), NC, d_np.as<int>(),
+                             d_nl.as<unsigned long long>(), c->stream));
     std::vector<int> np(NC);
     std::vector<long long> nl(NC);
     d2h(c, np.data(), d_np.as<int>(), NC);
@@ -3021,6 +3027,7 @@ int gs_destroy(gs_ctx* c) {
     c->d_te_m1.reset();
     c->d_te_m2.reset();
     c->d_te_pairs.reset();
+    c->d_te_pcl.reset();
     c->d_te_cstart.reset();
     c->d_te_rbl.reset();
     c->d_vl_vals.reset();

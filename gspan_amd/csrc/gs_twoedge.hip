@@ -18,7 +18,9 @@
 // Runs are named by bit b = label * EL + edge label (NB = VL * EL bits).  Per vertex u,
 // M1(u) marks its runs and M2(u) its runs of two or more entries.  The (vertex, run)
 // pairs of the graph, ordered by class c = (vertex label, run bit), fill three tables of NB
-// counts per class:
+// counts per class (CNT and CEN only for the classes that are a parent's segment class --
+// class (a, (b, e0)) or (b, (a, e0)) of the batch's single-edge parent p = (a, e0, b),
+// a <= b -- at row 2 p, for the class of a, and 2 p + 1, for that of b when b != a):
 //   CNT[c][b']  sum over the class's pairs (u, b): |run b| * (|run b'| - [b' = b])
 //   CEN[c][b']  the class's pairs (u, b) with |run b'| - [b' = b] >= 1
 //   END[c][b']  the class's pairs (z, b) for which some vertex u of run b has run b' beyond
@@ -78,10 +80,12 @@ __global__ void k_te_masks(DevGraph g, int EL, int W, uint32_t* __restrict__ m1,
 // pairs per class (hist), then the pairs at their class positions (cursor = class starts,
 // advanced); one thread per vertex over its runs.  A pair's record is two int4: {vertex,
 // run, first entry of the run, run length} and {first run of the vertex, last run of the
-// vertex, vertex label, run bit}.
+// vertex, vertex label, run bit}; pcl holds {class, run length} again, 8 bytes per pair,
+// for the per-class sums.
 template <bool SCATTER>
 __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or_cursor,
-                           const long long* __restrict__ cstart, int4* __restrict__ rec) {
+                           const long long* __restrict__ cstart, int4* __restrict__ rec,
+                           int2* __restrict__ pcl) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < g.n; v += stride) {
         const int o0 = g.offsets[v], o1 = g.offsets[v + 1];
@@ -99,6 +103,7 @@ __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or
                 const long long at = cstart[c] + atomicAdd(&hist_or_cursor[c], 1);
                 rec[2 * at] = make_int4((int)v, k, st, len);
                 rec[2 * at + 1] = make_int4(r0, r1, lv, b);
+                pcl[at] = make_int2((int)c, len);
             }
         }
     }
@@ -117,8 +122,9 @@ __global__ void k_te_pairs(DevGraph g, int EL, int NB, int* __restrict__ hist_or
 // lanes (lane w of a group holds word w of an OR): up to G consecutive pairs of one class
 // with short runs go one per group, so their load chains overlap; a pair with a longer run
 // has its run's vertices split over the groups.
-// work[0] += runs visited for CNT / CEN, work[1] += words read for END (per run vertex: its
-// adjacency entry, two words, and W + 1 mask words).
+// work[2 t] += runs visited for CNT / CEN, work[2 t + 1] += words read for END (per run
+// vertex: its adjacency entry, two words, and W + 1 mask words), t = workgroup mod
+// GS_TE_WORK_SLOTS.
 constexpr int TE_CHUNK = 64;
 constexpr int TE_SHORT = 16;  // runs up to this length share the wave with other pairs
 
@@ -151,17 +157,20 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
     const long long nchunks = (npairs + TE_CHUNK - 1) / TE_CHUNK;
     long long cls = -1;
     bool parent = false;  // the current class is a parent run of a batch pattern
+    long long prow = 0;   // its CNT / CEN row
     uint32_t keep = 0;    // word wi of the END bits the class's row needs
     bool need_end = false;  // some of them
     unsigned long long w_runs = 0, w_words = 0;
     auto flush = [&]() {
         __syncthreads();
         const long long row = cls * NB;
-        for (int b = lane; b < NB; b += 64) {
-            if (s_cnt[b]) atomicAdd(&CNT[row + b], s_cnt[b]);
-            if (s_cen[b]) atomicAdd(&CEN[row + b], s_cen[b]);
-            s_cnt[b] = 0;
-            s_cen[b] = 0;
+        if (parent) {
+            for (int b = lane; b < NB; b += 64) {
+                if (s_cnt[b]) atomicAdd(&CNT[prow + b], s_cnt[b]);
+                if (s_cen[b]) atomicAdd(&CEN[prow + b], s_cen[b]);
+                s_cnt[b] = 0;
+                s_cen[b] = 0;
+            }
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -193,7 +202,9 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                 cls = c;
                 const int tl = b / EL;
                 const int lo = lx < tl ? lx : tl, hi = lx < tl ? tl : lx;
-                parent = p_of[(lo * EL + el) * VL + hi] >= 0;
+                const int pi = p_of[(lo * EL + el) * VL + hi];
+                parent = pi >= 0;
+                prow = (2LL * pi + (lx == lo ? 0 : 1)) * NB;
                 keep = parent ? ~0u : (wi < W ? pmask[(long long)tl * W + wi] : 0u);
                 need_end = __ballot(keep != 0) != 0;
             }
@@ -297,41 +308,49 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
         w_runs += __shfl_xor(w_runs, off);
         w_words += __shfl_xor(w_words, off);
     }
-    if (lane == 0) {
-        atomicAdd(&work[0], w_runs);
-        atomicAdd(&work[1], w_words);
+    if (lane == 0) {  // spread over slots: every wave ends at about the same time
+        const int t = (int)(blockIdx.x % GS_TE_WORK_SLOTS);
+        atomicAdd(&work[2 * t], w_runs);
+        atomicAdd(&work[2 * t + 1], w_words);
     }
 }
 
-// Per class: its pairs (npairs) and the CSR entries of its runs (nlen), one wave per class
-// over the class's records.
-__global__ __launch_bounds__(256) void k_te_classes(const int4* __restrict__ rec,
+// Per class: its pairs (npairs, from the class starts) and the CSR entries of its runs
+// (nlen, zero beforehand): each wave sums 64 consecutive pairs by class -- a segmented
+// scan over its lanes, the pairs being class-ordered -- and the last lane of each class
+// adds the class's sum.
+__global__ __launch_bounds__(256) void k_te_classes(const int2* __restrict__ pcl, long long np,
                                                     const long long* __restrict__ cstart,
                                                     long long NC, int* __restrict__ npairs,
                                                     unsigned long long* __restrict__ nlen) {
     const int lane = threadIdx.x & 63;
-    const long long waves = (long long)gridDim.x * (blockDim.x / 64);
-    for (long long c = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64; c < NC;
-         c += waves) {
-        const long long s0 = cstart[c], s1 = cstart[c + 1];
-        unsigned long long sum = 0;
-        for (long long i = s0 + lane; i < s1; i += 64) sum += (unsigned long long)rec[2 * i].w;
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-        if (lane == 0) {
-            npairs[c] = (int)(s1 - s0);
-            nlen[c] = sum;
+    const long long n = np > NC ? np : NC;
+    const long long stride = (long long)gridDim.x * blockDim.x;  // a multiple of 64
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i - lane < n;
+         i += stride) {
+        if (i < NC) npairs[i] = (int)(cstart[i + 1] - cstart[i]);
+        const int2 v = i < np ? pcl[i] : make_int2(-1, 0);
+        unsigned long long sum = (unsigned long long)v.y;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned long long up = __shfl_up(sum, off);
+            const int uc = __shfl_up(v.x, off);
+            if (lane >= off && uc == v.x) sum += up;
         }
+        const int nc = __shfl_down(v.x, 1);
+        if (v.x >= 0 && (lane == 63 || nc != v.x)) atomicAdd(&nlen[v.x], sum);
     }
 }
 
 // Every key of the batch (parents of one edge, so no backward slots): its child rows
-// (key_count; past INT_MAX: INT_MIN, as the extension kernels' saturated counts) and the
-// distinct images of its three vertices (img[8 * key + v]).  Slots the rules reject
+// (key_count; past INT_MAX: INT_MIN, as the extension kernels' saturated counts) and its
+// support (ksup): the least non-zero count of distinct images over the child's three
+// vertices, vertex 0 only where it takes part (GS_MNI_OMP_LITERAL: when the new edge leaves
+// it), INT_MAX when none is non-zero (gspan.cu:282-288 / 367-373).  Slots the rules reject
 // (gspan.cu:455, 475-477) have no children.
 __global__ void k_te_keys(const PatDev* __restrict__ pats, int npats, KeySpace ks, long long K,
-                          int NB, const unsigned long long* __restrict__ CNT,
+                          int NB, int mni_mode, const unsigned long long* __restrict__ CNT,
                           const int* __restrict__ CEN, const int* __restrict__ END,
-                          int* __restrict__ key_count, int* __restrict__ img) {
+                          int* __restrict__ key_count, int* __restrict__ ksup) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long key = (long long)blockIdx.x * blockDim.x + threadIdx.x; key < K;
          key += stride) {
@@ -344,31 +363,39 @@ __global__ void k_te_keys(const PatDev* __restrict__ pats, int npats, KeySpace k
         const PatDev& P = pats[lo];
         const int slot = (int)(key - P.key_base);
         const int j = slot / (ks.EL * ks.VL), e = (slot / ks.VL) % ks.EL, l = slot % ks.VL;
-        int* out = img + 8 * key;
-        for (int v = 0; v < 8; ++v) out[v] = 0;
-        key_count[key] = 0;
-        if (l < P.min_label || !(P.seg_el[j] < e || (P.seg_el[j] == e && P.seg_tl[j] <= l)))
+        if (l < P.min_label || !(P.seg_el[j] < e || (P.seg_el[j] == e && P.seg_tl[j] <= l))) {
+            key_count[key] = 0;
+            ksup[key] = INT_MAX;
             continue;
+        }
         const int sv = P.seg_v[j], ov = 1 - sv;
         const int ls = P.vlab[sv], lo_ = P.vlab[ov], e0 = P.cel[0];
         const long long A = (long long)lo_ * ks.EL + e0, C = (long long)l * ks.EL + e;
-        const long long cs = ((long long)ls * NB + A) * NB + C;
+        // the segment class's CNT / CEN row: 2 lo for the parent's smaller label
+        const int lmin = P.vlab[0] < P.vlab[1] ? P.vlab[0] : P.vlab[1];
+        const long long cs = (2LL * lo + (ls == lmin ? 0 : 1)) * NB + C;
         const unsigned long long cnt = CNT[cs];
         key_count[key] = cnt > (unsigned long long)INT_MAX ? INT_MIN : (int)cnt;
-        out[sv] = CEN[cs];
-        out[ov] = END[((long long)lo_ * NB + ((long long)ls * ks.EL + e0)) * NB + C];
-        out[2] = END[((long long)l * NB + ((long long)ls * ks.EL + e)) * NB + A];
+        int im[3];
+        im[sv] = CEN[cs];
+        im[ov] = END[((long long)lo_ * NB + ((long long)ls * ks.EL + e0)) * NB + C];
+        im[2] = END[((long long)l * NB + ((long long)ls * ks.EL + e)) * NB + A];
+        const bool v0 = mni_mode != GS_MNI_OMP_LITERAL || sv == 0;
+        int sup = INT_MAX;
+        for (int v = v0 ? 0 : 1; v < 3; ++v)
+            if (im[v] > 0 && im[v] < sup) sup = im[v];
+        ksup[key] = sup;
     }
 }
 
-// counts[8 a + v] = img[8 * akeys[a] + v] for the active keys (n_active null: max_active)
-__global__ void k_te_gather(const int* __restrict__ img, const int* __restrict__ akeys,
+// out[a] = ksup[akeys[a]] for the active keys (n_active null: max_active)
+__global__ void k_te_gather(const int* __restrict__ ksup, const int* __restrict__ akeys,
                             const int* __restrict__ n_active, int max_active,
-                            int* __restrict__ counts) {
+                            int* __restrict__ out) {
     const long long na = n_active ? *n_active : max_active;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < 8 * na; i += stride)
-        counts[i] = img[8LL * akeys[i / 8] + i % 8];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < na; i += stride)
+        out[i] = ksup[akeys[i]];
 }
 
 // Reports of a two-edge last level reached from every frequent single-edge code (the
@@ -448,11 +475,9 @@ inline int te_grid(long long n, int block, int cap) {
 
 namespace gsk {
 
-static_assert(GS_MAXV == 8, "image counts are 8 per key");
-
 hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, uint32_t* m2,
                       int* hist, long long* cstart, long long* scan_tmp, long long scan_elems,
-                      int4* rec, int2* rbl, hipStream_t s) {
+                      int4* rec, int2* pcl, int2* rbl, hipStream_t s) {
     const long long NC = (long long)(NB / EL) * NB;  // VL * NB classes
     hipError_t e;
     if ((e = fill_u32(m1, 0u, (long long)g.n * W, s)) != hipSuccess) return e;
@@ -462,20 +487,20 @@ hipError_t te_prepare(const DevGraph& g, int EL, int NB, int W, uint32_t* m1, ui
     hipLaunchKernelGGL(k_te_masks, dim3(grid), dim3(256), 0, s, g, EL, W, m1, m2);
     hipLaunchKernelGGL(k_te_runs, dim3(te_grid(g.nruns, 256, 8192)), dim3(256), 0, s, g, EL, rbl);
     hipLaunchKernelGGL(k_te_pairs<false>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist,
-                       (const long long*)nullptr, (int4*)nullptr);
+                       (const long long*)nullptr, (int4*)nullptr, (int2*)nullptr);
     if ((e = scan_exclusive(hist, cstart, NC, scan_tmp, scan_elems, s)) != hipSuccess) return e;
     if ((e = fill_i32(hist, 0, NC, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_te_pairs<true>, dim3(grid), dim3(256), 0, s, g, EL, NB, hist, cstart,
-                       rec);
+                       rec, pcl);
     return hipGetLastError();
 }
 
-hipError_t te_clear(KeySpace ks, int* queue, unsigned long long* CNT, int* CEN, int* END,
-                    hipStream_t s) {
-    const long long NB = (long long)ks.VL * ks.EL, rows = ks.VL * NB * NB;
+hipError_t te_clear(KeySpace ks, int npats, int* queue, unsigned long long* CNT, int* CEN,
+                    int* END, hipStream_t s) {
+    const long long NB = (long long)ks.VL * ks.EL, rows = ks.VL * NB * NB, prows = 2LL * npats * NB;
     hipError_t e;
-    if ((e = fill_u32(reinterpret_cast<uint32_t*>(CNT), 0u, 2 * rows, s)) != hipSuccess) return e;
-    if ((e = fill_i32(CEN, 0, rows, s)) != hipSuccess) return e;
+    if ((e = fill_u32(reinterpret_cast<uint32_t*>(CNT), 0u, 2 * prows, s)) != hipSuccess) return e;
+    if ((e = fill_i32(CEN, 0, prows, s)) != hipSuccess) return e;
     if ((e = fill_i32(END, 0, rows, s)) != hipSuccess) return e;
     return fill_i32(queue, 0, 1, s);
 }
@@ -483,8 +508,8 @@ hipError_t te_clear(KeySpace ks, int* queue, unsigned long long* CNT, int* CEN, 
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     const int2* rbl, const uint32_t* pmask, int* queue,
-                    unsigned long long* CNT, int* CEN, int* END, int* key_count, int* img,
-                    unsigned long long* work, hipStream_t s) {
+                    unsigned long long* CNT, int* CEN, int* END, int mni_mode, int* key_count,
+                    int* ksup, unsigned long long* work, hipStream_t s) {
     const int NB = ks.VL * ks.EL, W = (NB + 31) / 32;
     int WP = 1;
     while (WP < W) WP <<= 1;
@@ -506,15 +531,17 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
 #undef GS_TE_ACCUM
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_te_keys, dim3(te_grid(K, 256, 8192)), dim3(256), 0, s, pats, npats, ks,
-                       K, NB, CNT, CEN, END, key_count, img);
+                       K, NB, mni_mode, CNT, CEN, END, key_count, ksup);
     return hipGetLastError();
 }
 
-hipError_t te_classes(const int4* rec, const long long* cstart, long long NC, int* npairs,
-                      unsigned long long* nlen, hipStream_t s) {
+hipError_t te_classes(const int2* pcl, long long np, const long long* cstart, long long NC,
+                      int* npairs, unsigned long long* nlen, hipStream_t s) {
     if (NC <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_te_classes, dim3(te_grid(NC, 4, 8192)), dim3(256), 0, s, rec, cstart, NC,
-                       npairs, nlen);
+    hipError_t e;
+    if ((e = fill_u32(reinterpret_cast<uint32_t*>(nlen), 0u, 2 * NC, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_te_classes, dim3(te_grid(np > NC ? np : NC, 256, 4096)), dim3(256), 0, s,
+                       pcl, np, cstart, NC, npairs, nlen);
     return hipGetLastError();
 }
 
@@ -528,11 +555,11 @@ hipError_t te_emit(const PatDev* pats, int npats, KeySpace ks, const int* fkey, 
     return hipGetLastError();
 }
 
-hipError_t te_gather(const int* img, const int* akeys, const int* n_active, int max_active,
-                     int* counts, hipStream_t s) {
+hipError_t te_gather(const int* ksup, const int* akeys, const int* n_active, int max_active,
+                     int* out, hipStream_t s) {
     if (max_active <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_te_gather, dim3(te_grid(8LL * max_active, 256, 8192)), dim3(256), 0, s,
-                       img, akeys, n_active, max_active, counts);
+    hipLaunchKernelGGL(k_te_gather, dim3(te_grid(max_active, 256, 8192)), dim3(256), 0, s, ksup,
+                       akeys, n_active, max_active, out);
     return hipGetLastError();
 }
 
