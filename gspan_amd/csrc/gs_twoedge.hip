@@ -265,16 +265,23 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
                     }
                 }
             }
-            // x as the segment vertex of a parent row (x, w), w in run b
+            // x as the segment vertex of a parent row (x, w), w in run b: x's runs, two per
+            // lane at a time (both loads in flight before the sums)
             if (parent) {
                 const int qs = alone ? 64 : WP;
-                for (int q = q0 + (alone ? lane : wi); act && q <= q1; q += qs) {
-                    const int2 r = rbl[q];
-                    const int wq = r.y - (q == k ? 1 : 0);
-                    if (wq >= 1) {
-                        const int bq = r.x;
-                        atomicAdd(&s_cnt[bq], (unsigned long long)len * (unsigned long long)wq);
-                        atomicAdd(&s_cen[bq], 1);
+                for (int qb = q0 + (alone ? lane : wi); act && qb <= q1; qb += 2 * qs) {
+                    int2 r[2];
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        r[t] = qb + t * qs <= q1 ? rbl[qb + t * qs] : make_int2(0, 0);
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int wq = r[t].y - (qb + t * qs == k ? 1 : 0);
+                        if (wq >= 1) {
+                            atomicAdd(&s_cnt[r[t].x],
+                                      (unsigned long long)len * (unsigned long long)wq);
+                            atomicAdd(&s_cen[r[t].x], 1);
+                        }
                     }
                 }
             }
