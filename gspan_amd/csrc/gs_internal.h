@@ -447,6 +447,25 @@ hipError_t te_classes(const int2* pcl, long long np, const long long* cstart, lo
                       int* npairs, unsigned long long* nlen, hipStream_t s);
 hipError_t te_clear(KeySpace ks, int npats, int* queue, unsigned long long* CNT, int* CEN,
                     int* END, hipStream_t s);
+// The single-edge codes of the class sums on the device (flags[i] for code i = (a * EL +
+// e0) * VL + b: a <= b, rows, support >= min_sup under mni_mode), then p_of and pmask as
+// te_level reads them (p_of[i]: index among the flagged codes in ascending i, or -1) and
+// *ncodes = the flagged codes; scan: VL * EL * VL long longs.
+hipError_t te_seeds(const int* npairs, const unsigned long long* nlen, int VL, int EL,
+                    int mni_mode, int min_sup, int* flags, long long* scan, long long* scan_tmp,
+                    long long scan_elems, int* p_of, uint32_t* pmask, int* ncodes,
+                    hipStream_t s);
+// te_clear for *npats single-edge parents (tables sized for max_pats), count on the device
+hipError_t te_clear_dev(KeySpace ks, const int* npats, long long max_pats, int* queue,
+                        unsigned long long* CNT, int* CEN, int* END, hipStream_t s);
+// te_level's two launches: the tables from the pairs (te_accum), then the keys (te_keys)
+hipError_t te_accum(const DevGraph& g, KeySpace ks, const int* p_of, const uint32_t* m1,
+                    const uint32_t* m2, const int4* rec, const int2* rbl, const uint32_t* pmask,
+                    int* queue, unsigned long long* CNT, int* CEN, int* END,
+                    unsigned long long* work, hipStream_t s);
+hipError_t te_keys(const PatDev* pats, int npats, KeySpace ks, long long K,
+                   const unsigned long long* CNT, const int* CEN, const int* END, int mni_mode,
+                   int* key_count, int* ksup, hipStream_t s);
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     const int2* rbl, const uint32_t* pmask, int* queue,

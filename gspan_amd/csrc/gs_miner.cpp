@@ -320,6 +320,19 @@ struct gs_ctx {
     int flat_seeds = 0;
     int *flat_codes = nullptr, *flat_lens = nullptr, *flat_sups = nullptr;
     long long flat_cap_e = 0, flat_cap_l = 0, flat_cap_s = 0, flat_reports = 0, flat_edges = 0;
+    // The two-edge pass over every frequent single-edge code, launched from the class sums
+    // before the host has listed the codes (te_prelaunch): its tables, counters, the codes'
+    // count on the device and the slot of its timing event (-1: untimed).  The step over
+    // those codes launches the key pass on the tables (on: not taken yet).
+    struct TePre {
+        bool on = false;
+        int ev = -1;
+        DBuf cnt, cen, end, q, work, flags, scan, tmp, pof, pmask, ncodes;
+    } te_pre;
+    int* h_te_np = nullptr;                 // pinned copies of the class sums
+    unsigned long long* h_te_nl = nullptr;
+    long long h_te_cap = 0;
+    hipEvent_t ev_seeds = nullptr;  // the class sums have reached the host
     // neighbours per run as a row scan sees them: a vertex is scanned about once per
     // incident row, so items ~ sum deg^2 and run items ~ sum deg * runs; their ratio.  Long
     // runs make the run modes and per-group new-vertex inserts pay at the last level
@@ -882,6 +895,17 @@ void mlp_init(gs_ctx* c) {
     c->mlp_trained = false;
 }
 
+// Records the first event of the next timing pair (ev_pool[ev_used]) on the stream.
+void ev_begin(gs_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        c->ev_pool.emplace_back(a, b);
+    }
+    HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].first, c->stream));
+}
+
 // Folds the event pairs recorded since the last host synchronisation into the stats.
 void resolve_timing(gs_ctx* c) {
     for (auto& pk : c->ev_pending) {
@@ -1321,13 +1345,7 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     };
     auto time_begin = [&]() {
         if (!c->opts.timing && t_bucket < 0) return;
-        if (c->ev_used == c->ev_pool.size()) {
-            hipEvent_t a, b;
-            HIPCHECK(hipEventCreate(&a));
-            HIPCHECK(hipEventCreate(&b));
-            c->ev_pool.emplace_back(a, b);
-        }
-        HIPCHECK(hipEventRecord(c->ev_pool[c->ev_used].first, c->stream));
+        ev_begin(c);
     };
     auto time_end = [&](int kind, const long long* items) {
         if (!c->opts.timing && t_bucket < 0) return;
@@ -1551,7 +1569,23 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     }
 
     // ---- pass 1: count children per key (single pass: also fill every key's sets)
-    if (two_edge) {
+    int pre_codes = -1;  // single-edge codes the pre-launched pass found on the device
+    gs_ctx::TePre& tp = c->te_pre;
+    if (two_edge && tp.on && c->flat_ok && B == c->flat_seeds) {
+        // every frequent single-edge code: the tables are being summed already
+        // (te_prelaunch); the keys read them
+        tp.on = false;
+        d_te_sup = DBuf(P, sizeof(int) * K);
+        Wc_host[0] = c->dg.nruns;  // (vertex, run) pairs
+        HIPCHECK(gsk::te_keys(d_pats.as<PatDev>(), B, ks, K, tp.cnt.as<unsigned long long>(),
+                              tp.cen.as<int>(), tp.end.as<int>(), c->mni_mode, d_kc.as<int>(),
+                              d_te_sup.as<int>(), c->stream));
+        if (tp.ev >= 0 && tp.ev == (int)c->ev_used) time_end(1, &Wc_host[0]);
+        c->st.fill_launches++;
+        c->st.two_edge_levels++;
+        d2h(c, te_work, tp.work.as<unsigned long long>(), 2 * GS_TE_WORK_SLOTS);
+        d2h(c, &pre_codes, tp.ncodes.as<int>(), 1);
+    } else if (two_edge) {
         const int NB = (int)te_nb;
         te_build(c);
         const int W = (NB + 31) / 32;
@@ -1651,6 +1685,8 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     if (na_err[1] & 1)
         throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
     if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
+    if (pre_codes >= 0 && pre_codes != B)
+        throw GsError(GS_ERR_INTERNAL, "single-edge codes of the device and the host differ");
     const int A = na_err[0];
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
@@ -2116,6 +2152,53 @@ void mine_level(gs_ctx* c, std::vector<HPat>& pats, std::vector<Layer>& layers) 
 // a <= b, has as many rows as CSR entries from label a to label b over e0 (gspan.cu:410-426),
 // the vertices of label a with a run (b, e0) as vertex-0 images and those of label b with a
 // run (a, e0) as vertex-1 images.  Codes in key order, as mine_seeds reports them.
+// The two-edge pass over every frequent single-edge code, launched from the class sums
+// (d_np, d_nl) while the host lists the codes: the codes' flags, batch indexes and end-bit
+// masks on the device (gsk::te_seeds), the tables cleared for as many codes as it flags,
+// then the run pass (gsk::te_accum).  Skipped when tables for every code a <= b would not
+// fit the budget (the step then launches its own pass).
+void te_prelaunch(gs_ctx* c, const int* d_np, const unsigned long long* d_nl) {
+    const int VL = c->VL, EL = c->EL, NB = VL * EL, W = (NB + 31) / 32;
+    const long long NS = (long long)VL * EL * VL;              // single-edge code space
+    const long long maxp = (long long)EL * VL * (VL + 1) / 2;  // codes with a <= b
+    const long long rows = (long long)VL * NB * NB, prows = 2 * maxp * NB;
+    const long long te = gsk::scan_tmp_elems(NS);
+    const size_t need = (size_t)(12 * prows + 4 * rows + 16 * NS + 8 * te + 4 * VL * W) + 4096;
+    if (c->pool.in_use() + need > c->budget) return;
+    DevPool* P = &c->pool;
+    gs_ctx::TePre& t = c->te_pre;
+    t.flags = DBuf(P, 4 * NS);
+    t.scan = DBuf(P, 8 * NS);
+    t.tmp = DBuf(P, 8 * te);
+    t.pof = DBuf(P, 4 * NS);
+    t.pmask = DBuf(P, 4 * (size_t)VL * W);
+    t.ncodes = DBuf(P, 4);
+    HIPCHECK(gsk::te_seeds(d_np, d_nl, VL, EL, c->mni_mode, c->min_sup, t.flags.as<int>(),
+                           t.scan.as<long long>(), t.tmp.as<long long>(), te, t.pof.as<int>(),
+                           t.pmask.as<uint32_t>(), t.ncodes.as<int>(), c->stream));
+    t.cnt = DBuf(P, 8 * prows);
+    t.cen = DBuf(P, 4 * prows);
+    t.end = DBuf(P, 4 * rows);
+    t.q = DBuf(P, 4);
+    t.work = DBuf(P, sizeof(unsigned long long) * 2 * GS_TE_WORK_SLOTS);
+    HIPCHECK(gsk::fill_i32(t.work.as<int>(), 0, 4 * GS_TE_WORK_SLOTS, c->stream));
+    const KeySpace ks{EL, VL};
+    HIPCHECK(gsk::te_clear_dev(ks, t.ncodes.as<int>(), maxp, t.q.as<int>(),
+                               t.cnt.as<unsigned long long>(), t.cen.as<int>(),
+                               t.end.as<int>(), c->stream));
+    t.ev = -1;
+    if (c->opts.timing) {
+        ev_begin(c);
+        t.ev = (int)c->ev_used;
+    }
+    HIPCHECK(gsk::te_accum(c->dg, ks, t.pof.as<int>(), c->d_te_m1.as<uint32_t>(),
+                           c->d_te_m2.as<uint32_t>(), c->d_te_pairs.as<int4>(),
+                           c->d_te_rbl.as<int2>(), t.pmask.as<uint32_t>(), t.q.as<int>(),
+                           t.cnt.as<unsigned long long>(), t.cen.as<int>(), t.end.as<int>(),
+                           t.work.as<unsigned long long>(), c->stream));
+    t.on = true;
+}
+
 void seeds_from_runs(gs_ctx* c) {
     te_build(c);
     const int VL = c->VL, EL = c->EL, NB = VL * EL;
@@ -2125,11 +2208,29 @@ void seeds_from_runs(gs_ctx* c) {
     HIPCHECK(gsk::te_classes(c->d_te_pcl.as<int2>(), c->dg.nruns,
                              c->d_te_cstart.as<long long>(), NC, d_np.as<int>(),
                              d_nl.as<unsigned long long>(), c->stream));
-    std::vector<int> np(NC);
-    std::vector<long long> nl(NC);
-    d2h(c, np.data(), d_np.as<int>(), NC);
-    d2h(c, nl.data(), d_nl.as<long long>(), NC);
-    sync(c);
+    if (c->h_te_cap < NC) {
+        if (c->h_te_np) HIPCHECK(hipHostFree(c->h_te_np));
+        if (c->h_te_nl) HIPCHECK(hipHostFree(c->h_te_nl));
+        c->h_te_np = nullptr;
+        c->h_te_nl = nullptr;
+        c->h_te_cap = 0;
+        HIPCHECK(hipHostMalloc((void**)&c->h_te_np, sizeof(int) * NC, hipHostMallocDefault));
+        HIPCHECK(hipHostMalloc((void**)&c->h_te_nl, sizeof(long long) * NC,
+                               hipHostMallocDefault));
+        c->h_te_cap = NC;
+    }
+    HIPCHECK(hipMemcpyAsync(c->h_te_np, d_np.as<int>(), sizeof(int) * NC, hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCHECK(hipMemcpyAsync(c->h_te_nl, d_nl.as<unsigned long long>(), sizeof(long long) * NC,
+                            hipMemcpyDeviceToHost, c->stream));
+    if (!c->ev_seeds) HIPCHECK(hipEventCreateWithFlags(&c->ev_seeds, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(c->ev_seeds, c->stream));
+    // with mining to k = 2 every frequent code is extended in one step: its run pass starts
+    // now and overlaps the host's work below
+    if (c->k == 2) te_prelaunch(c, d_np.as<int>(), d_nl.as<unsigned long long>());
+    HIPCHECK(hipEventSynchronize(c->ev_seeds));
+    const int* np = c->h_te_np;
+    const unsigned long long* nl = c->h_te_nl;
     auto pairs_of = [&](long long cl) { return (long long)np[cl]; };
     std::vector<HPat> freq;
     long long codes = 0, rows = 0;
@@ -2137,7 +2238,7 @@ void seeds_from_runs(gs_ctx* c) {
         for (int e0 = 0; e0 < EL; ++e0) {
             for (int b = a; b < VL; ++b) {
                 const long long cl = (long long)a * NB + (long long)b * EL + e0;
-                const long long cnt = nl[cl];
+                const long long cnt = (long long)nl[cl];
                 if (cnt == 0) continue;
                 ++codes;
                 HPat h;
@@ -2167,6 +2268,7 @@ void seeds_from_runs(gs_ctx* c) {
         c->flat_seeds = (int)freq.size();
         mine_level(c, freq, layers);
     }
+    c->te_pre = gs_ctx::TePre{};  // a pass no step took (the batch was split) is dropped
 }
 
 void mine_seeds(gs_ctx* c) {
@@ -2619,6 +2721,7 @@ int gs_mine(gs_ctx* c, int k, int min_support, gs_report_cb cb, void* user) {
         c->mni_mode = c->opts.mni_mode;
         c->reports.clear();
         c->flat = c->flat_ok = false;
+        c->te_pre = gs_ctx::TePre{};
         c->report_order.clear();
         clear_guard(c);
         c->reset_tile_queues();
@@ -3035,6 +3138,10 @@ int gs_destroy(gs_ctx* c) {
     if (c->flat_codes) (void)hipHostFree(c->flat_codes);
     if (c->flat_lens) (void)hipHostFree(c->flat_lens);
     if (c->flat_sups) (void)hipHostFree(c->flat_sups);
+    c->te_pre = gs_ctx::TePre{};
+    if (c->h_te_np) (void)hipHostFree(c->h_te_np);
+    if (c->h_te_nl) (void)hipHostFree(c->h_te_nl);
+    if (c->ev_seeds) (void)hipEventDestroy(c->ev_seeds);
     c->d_vlab.reset();
     c->d_misc.reset();
     c->d_tiles.reset();

@@ -348,6 +348,67 @@ __global__ __launch_bounds__(256) void k_te_classes(const int2* __restrict__ pcl
     }
 }
 
+// Single-edge code i = (a * EL + e0) * VL + b: flags[i] = 1 when a <= b, the code has rows
+// (CSR entries from label a to label b over e0, nlen of class (a, (b, e0))) and its support
+// reaches min_sup: the least non-zero count of its vertices' images -- the pairs of classes
+// (a, (b, e0)) and (b, (a, e0)) -- vertex 0 left out under GS_MNI_OMP_LITERAL, as no later
+// edge leaves it (gspan.cu:282-288 / 367-373; seeding gspan.cu:410-426).
+__global__ void k_te_seed_flags(const int* __restrict__ npairs,
+                                const unsigned long long* __restrict__ nlen, int VL, int EL,
+                                int mni_mode, int min_sup, int* __restrict__ flags) {
+    const long long n = (long long)VL * EL * VL, NB = (long long)VL * EL;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int a = (int)(i / ((long long)EL * VL)), e0 = (int)((i / VL) % EL),
+                  b = (int)(i % VL);
+        int f = 0;
+        if (a <= b) {
+            const long long cl = a * NB + (long long)b * EL + e0;
+            if (nlen[cl] > 0) {
+                const int n0 = npairs[cl];
+                const int n1 = npairs[(long long)b * NB + (long long)a * EL + e0];
+                int s = INT_MAX;
+                if (mni_mode != GS_MNI_OMP_LITERAL && n0 > 0) s = n0;
+                if (n1 > 0 && n1 < s) s = n1;
+                f = s >= min_sup ? 1 : 0;
+            }
+        }
+        flags[i] = f;
+    }
+}
+
+// p_of[i] = the index of flagged code i among the flagged codes (scan: exclusive sums of the
+// flags), or -1; the pmask bits of every flagged code (a, e0, b): run bit (a, e0) of label b
+// and (b, e0) of label a; *ncodes = the flagged codes.
+__global__ void k_te_pof(const int* __restrict__ flags, const long long* __restrict__ scan,
+                         int VL, int EL, int W, int* __restrict__ p_of, uint32_t* pmask,
+                         int* ncodes) {
+    const long long n = (long long)VL * EL * VL;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int f = flags[i];
+        p_of[i] = f ? (int)scan[i] : -1;
+        if (i == n - 1) *ncodes = (int)(scan[i] + f);
+        if (f) {
+            const int a = (int)(i / ((long long)EL * VL)), e0 = (int)((i / VL) % EL),
+                      b = (int)(i % VL);
+            const int ba = a * EL + e0, bb = b * EL + e0;
+            atomicOr(&pmask[(long long)b * W + (ba >> 5)], 1u << (ba & 31));
+            atomicOr(&pmask[(long long)a * W + (bb >> 5)], 1u << (bb & 31));
+        }
+    }
+}
+
+// p[0 .. min(*nrows * row_words, max_words)) = 0
+__global__ void k_te_zero_rows(uint32_t* __restrict__ p, const int* __restrict__ nrows,
+                               long long row_words, long long max_words) {
+    long long n = (long long)*nrows * row_words;
+    n = n < max_words ? n : max_words;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        p[i] = 0u;
+}
+
 // Every key of the batch (parents of one edge, so no backward slots): its child rows
 // (key_count; past INT_MAX: INT_MIN, as the extension kernels' saturated counts) and its
 // support (ksup): the least non-zero count of distinct images over the child's three
@@ -512,15 +573,55 @@ hipError_t te_clear(KeySpace ks, int npats, int* queue, unsigned long long* CNT,
     return fill_i32(queue, 0, 1, s);
 }
 
+hipError_t te_seeds(const int* npairs, const unsigned long long* nlen, int VL, int EL,
+                    int mni_mode, int min_sup, int* flags, long long* scan, long long* scan_tmp,
+                    long long scan_elems, int* p_of, uint32_t* pmask, int* ncodes,
+                    hipStream_t s) {
+    const long long n = (long long)VL * EL * VL;
+    const int W = (VL * EL + 31) / 32;
+    hipError_t e;
+    if ((e = fill_u32(pmask, 0u, (long long)VL * W, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_te_seed_flags, dim3(te_grid(n, 256, 8192)), dim3(256), 0, s, npairs,
+                       nlen, VL, EL, mni_mode, min_sup, flags);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = scan_exclusive(flags, scan, n, scan_tmp, scan_elems, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_te_pof, dim3(te_grid(n, 256, 8192)), dim3(256), 0, s, flags, scan, VL,
+                       EL, W, p_of, pmask, ncodes);
+    return hipGetLastError();
+}
+
+hipError_t te_clear_dev(KeySpace ks, const int* npats, long long max_pats, int* queue,
+                        unsigned long long* CNT, int* CEN, int* END, hipStream_t s) {
+    const long long NB = (long long)ks.VL * ks.EL, rows = ks.VL * NB * NB;
+    const long long cnt_words = 4 * NB * max_pats, cen_words = 2 * NB * max_pats;
+    hipLaunchKernelGGL(k_te_zero_rows, dim3(te_grid(cnt_words, 256, 8192)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(CNT), npats, 4 * NB, cnt_words);
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_te_zero_rows, dim3(te_grid(cen_words, 256, 8192)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(CEN), npats, 2 * NB, cen_words);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = fill_i32(END, 0, rows, s)) != hipSuccess) return e;
+    return fill_i32(queue, 0, 1, s);
+}
+
 hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npats, long long K,
                     const int* p_of, const uint32_t* m1, const uint32_t* m2, const int4* rec,
                     const int2* rbl, const uint32_t* pmask, int* queue,
                     unsigned long long* CNT, int* CEN, int* END, int mni_mode, int* key_count,
                     int* ksup, unsigned long long* work, hipStream_t s) {
+    hipError_t e = te_accum(g, ks, p_of, m1, m2, rec, rbl, pmask, queue, CNT, CEN, END, work, s);
+    if (e != hipSuccess) return e;
+    return te_keys(pats, npats, ks, K, CNT, CEN, END, mni_mode, key_count, ksup, s);
+}
+
+hipError_t te_accum(const DevGraph& g, KeySpace ks, const int* p_of, const uint32_t* m1,
+                    const uint32_t* m2, const int4* rec, const int2* rbl, const uint32_t* pmask,
+                    int* queue, unsigned long long* CNT, int* CEN, int* END,
+                    unsigned long long* work, hipStream_t s) {
     const int NB = ks.VL * ks.EL, W = (NB + 31) / 32;
     int WP = 1;
     while (WP < W) WP <<= 1;
-    hipError_t e;
     const long long np = g.nruns;
     const size_t lds = (size_t)NB * 12;
     const long long nchunks = (np + TE_CHUNK - 1) / TE_CHUNK;
@@ -536,9 +637,15 @@ hipError_t te_level(const DevGraph& g, KeySpace ks, const PatDev* pats, int npat
     else if (nt <= 16) GS_TE_ACCUM(16);
     else return hipErrorInvalidValue;
 #undef GS_TE_ACCUM
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t te_keys(const PatDev* pats, int npats, KeySpace ks, long long K,
+                   const unsigned long long* CNT, const int* CEN, const int* END, int mni_mode,
+                   int* key_count, int* ksup, hipStream_t s) {
+    if (K <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_te_keys, dim3(te_grid(K, 256, 8192)), dim3(256), 0, s, pats, npats, ks,
-                       K, NB, mni_mode, CNT, CEN, END, key_count, ksup);
+                       K, ks.VL * ks.EL, mni_mode, CNT, CEN, END, key_count, ksup);
     return hipGetLastError();
 }
 
