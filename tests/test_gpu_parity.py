@@ -280,6 +280,26 @@ def test_spill_rows_to_host_memory(oracle_mod):
             assert s["spilled_rows"] > 0
 
 
+def test_two_edge_level_with_and_without_early_pass(oracle_mod):
+    """Mining to k = 2 on a simple graph: by default the two-edge pass starts from the class
+    sums before the host lists the single-edge codes; with a device budget too small for its
+    tables the step over the codes runs its own pass.  Reports equal the oracle's either way."""
+    g = synthetic("rmat", scale=11, edge_factor=8, n_vlabels=6, vlabel_zipf=1.1, n_elabels=2,
+                  seed=5)
+    for mni, name in ((0, "gpu"), (1, "omp-literal")):
+        _announce(f"mine rmat scale 11 k=2 ms=30 budget 1 B mni={name}")
+        want, st = oracle_mod.fsm(g, 2, 30, mni_mode=mni)
+        assert want
+        with Miner(g, mni=name, budget_bytes=1) as mn:
+            assert mn.mine(2, 30) == want, name
+            s = mn.stats()
+            assert s["extended"] == st["extended"]
+            assert s["two_edge_levels"] >= 1
+        with Miner(g, mni=name) as mn:
+            assert mn.mine(2, 30) == want, name
+            assert mn.stats()["two_edge_levels"] == 1
+
+
 def test_prune_graph_matches_oracle(oracle_mod):
     """gs_prune_graph (graph.cc:216-266 on the device) against the oracle's restatement,
     at every threshold that changes the kept set; then mining the pruned graph must give
