@@ -643,18 +643,28 @@ OrderKey order_key(const HPat& h) {
 // Siblings have distinct last edges, so the order is total.  Linear apart from the sorts
 // of sibling groups.
 void order_reports(gs_ctx* c) {
-    std::vector<Report>& r = c->reports;
+    const std::vector<Report>& r = c->reports;
     const int n = (int)r.size();
+    // the two fields the order needs, gathered once by all host threads: the passes below
+    // then stream 12 bytes per report instead of the whole report
+    std::vector<int> par(n);
+    std::vector<uint64_t> key(n);
+    parallel_ranges(n, [&](long long a, long long b) {
+        for (long long i = a; i < b; ++i) {
+            par[i] = r[i].parent;
+            key[i] = r[i].edge_key;
+        }
+    });
     std::vector<int> start(n + 2, 0), kids(n);
-    for (const Report& x : r) ++start[x.parent + 2];  // parent -1 counts at index 1
+    for (int i = 0; i < n; ++i) ++start[par[i] + 2];  // parent -1 counts at index 1
     for (int i = 0; i <= n; ++i) start[i + 1] += start[i];
     {
         std::vector<int> at(start.begin(), start.end() - 1);
-        for (int i = 0; i < n; ++i) kids[at[r[i].parent + 1]++] = i;
+        for (int i = 0; i < n; ++i) kids[at[par[i] + 1]++] = i;
     }
     // group g (parent g - 1) occupies kids[start[g] .. start[g+1]); children reported in
     // key order mostly arrive in edge order already
-    auto by_edge = [&](int a, int b) { return r[a].edge_key < r[b].edge_key; };
+    auto by_edge = [&](int a, int b) { return key[a] < key[b]; };
     parallel_ranges((long long)n + 1, [&](long long g0, long long g1) {
         for (long long g = g0; g < g1; ++g) {
             const auto b0 = kids.begin() + start[g], b1 = kids.begin() + start[g + 1];
@@ -666,7 +676,7 @@ void order_reports(gs_ctx* c) {
     // own position is set) places its children's subtrees one after another in edge order.
     std::vector<int> size(n, 1), pos(n);
     for (int i = n - 1; i >= 0; --i) {
-        const int q = r[i].parent;
+        const int q = par[i];
         if (q >= i) throw GsError(GS_ERR_INTERNAL, "report before its parent");
         if (q >= 0) size[q] += size[i];
     }
