@@ -1036,39 +1036,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_items(DevGraph g, const PatDev* __
             // ins: this item inserts images at all
             bool lead = true, ins = false;
             if (ROWS) {
-                // Wave-level aggregation: consecutive items mostly share a key (a row's
-                // neighbours are ordered by (label, edge label)), so the lanes of one key take
-                // their tile counts with one LDS atomic by their first lane; keys left after a
-                // few rounds take one atomic per lane.  lead: the pair's first item in the tile.
-                unsigned long long pend = __ballot(act);
-                for (int round = 0; round < 4 && pend; ++round) {
-                    // the leader is wave-uniform: values move by readlane, not LDS shuffles
-                    const int leader = __ffsll((long long)pend) - 1;
-                    const int lkey = __builtin_amdgcn_readlane(agg, leader);
-                    const int lrow = __builtin_amdgcn_readlane(row, leader);
-                    const bool mine = act && agg == lkey;
-                    const unsigned long long m = __ballot(mine);
-                    int h = 0, base = 0, first = 0;
-                    if (lane == leader) {
-                        h = lds_slot<LDS_SLOTS>(s_key, lkey);
-                        base = atomicAdd(&s_cnt[h], (int)__popcll(m));
-                        // a pair is named by the key's slot in this tile's table and the row:
-                        // below 2 * TILE * TILE, so distinct pairs never share a name
-                        first = !dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + lrow);
-                    }
-                    h = __builtin_amdgcn_readlane(h, leader);
-                    base = __builtin_amdgcn_readlane(base, leader);
-                    first = __builtin_amdgcn_readlane(first, leader);
-                    if (mine) {
-                        hs[q] = h;
-                        rank[q] = base + (int)mbcnt64(m);
-                        lead = row == lrow
-                                   ? (lane == leader && first != 0)
-                                   : (!dedup || lds_first<LDS_SLOTS>(s_pair, h * TILE + row));
-                        act = false;
-                    }
-                    pend &= ~m;
-                }
+                // one LDS atomic per item on its key's tile slot (grouping the lanes of one
+                // key first cost more than it saved once long runs went to the run modes);
+                // lead: the pair's first item in the tile
                 if (act) {
                     hs[q] = lds_slot<LDS_SLOTS>(s_key, agg);
                     rank[q] = atomicAdd(&s_cnt[hs[q]], 1);
