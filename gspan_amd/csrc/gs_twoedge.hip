@@ -180,10 +180,19 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
         }
         __syncthreads();
     };
+    // A workgroup's first chunk is its own index (the grid holds no more workgroups than the
+    // device runs at once, so all start together); later ones come from the queue, whose
+    // claims then spread over the pass instead of all arriving at its start.
+    bool first_chunk = true;
     for (;;) {
         long long ch = 0;
-        if (lane == 0) ch = atomicAdd(queue, 1);
-        ch = __shfl(ch, 0);
+        if (first_chunk) {
+            ch = blockIdx.x;
+            first_chunk = false;
+        } else {
+            if (lane == 0) ch = gridDim.x + (long long)atomicAdd(queue, 1);
+            ch = __shfl(ch, 0);
+        }
         if (ch >= nchunks) break;
         const long long p0 = ch * TE_CHUNK;
         const int cnt = (int)min((long long)TE_CHUNK, npairs - p0);
@@ -323,28 +332,49 @@ __global__ __launch_bounds__(64) void k_te_accum(DevGraph g, int EL, int VL, int
 }
 
 // Per class: its pairs (npairs, from the class starts) and the CSR entries of its runs
-// (nlen, zero beforehand): each wave sums 64 consecutive pairs by class -- a segmented
-// scan over its lanes, the pairs being class-ordered -- and the last lane of each class
-// adds the class's sum.
+// (nlen, zero beforehand).  Each wave takes TE_CLS_SPAN consecutive (class-ordered) pairs,
+// all loaded at once, and sums them by class 64 at a time -- a segmented scan over its
+// lanes -- carrying the class open at the last lane into the next 64: one atomic per class
+// and span, not per 64 pairs (a large class's sums would queue on one address).
+constexpr int TE_CLS_SPAN = 512;
 __global__ __launch_bounds__(256) void k_te_classes(const int2* __restrict__ pcl, long long np,
                                                     const long long* __restrict__ cstart,
                                                     long long NC, int* __restrict__ npairs,
                                                     unsigned long long* __restrict__ nlen) {
     const int lane = threadIdx.x & 63;
-    const long long n = np > NC ? np : NC;
-    const long long stride = (long long)gridDim.x * blockDim.x;  // a multiple of 64
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i - lane < n;
-         i += stride) {
-        if (i < NC) npairs[i] = (int)(cstart[i + 1] - cstart[i]);
-        const int2 v = i < np ? pcl[i] : make_int2(-1, 0);
-        unsigned long long sum = (unsigned long long)v.y;
-        for (int off = 1; off < 64; off <<= 1) {
-            const unsigned long long up = __shfl_up(sum, off);
-            const int uc = __shfl_up(v.x, off);
-            if (lane >= off && uc == v.x) sum += up;
+    const long long gstride = (long long)gridDim.x * blockDim.x;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < NC; c += gstride)
+        npairs[c] = (int)(cstart[c + 1] - cstart[c]);
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long waves = gstride >> 6;
+    const long long nspans = (np + TE_CLS_SPAN - 1) / TE_CLS_SPAN;
+    for (long long sp = wave; sp < nspans; sp += waves) {
+        const long long p0 = sp * TE_CLS_SPAN, p1 = min(np, p0 + TE_CLS_SPAN);
+        int2 v[TE_CLS_SPAN / 64];
+#pragma unroll
+        for (int t = 0; t < TE_CLS_SPAN / 64; ++t) {
+            const long long i = p0 + 64 * t + lane;
+            v[t] = i < p1 ? pcl[i] : make_int2(-1, 0);
         }
-        const int nc = __shfl_down(v.x, 1);
-        if (v.x >= 0 && (lane == 63 || nc != v.x)) atomicAdd(&nlen[v.x], sum);
+        int ccls = -1;
+        unsigned long long carry = 0;
+#pragma unroll
+        for (int t = 0; t < TE_CLS_SPAN / 64; ++t) {
+            const int c0 = __shfl(v[t].x, 0);
+            if (ccls >= 0 && c0 != ccls && lane == 0) atomicAdd(&nlen[ccls], carry);
+            unsigned long long sum = (unsigned long long)v[t].y;
+            for (int off = 1; off < 64; off <<= 1) {
+                const unsigned long long up = __shfl_up(sum, off);
+                const int uc = __shfl_up(v[t].x, off);
+                if (lane >= off && uc == v[t].x) sum += up;
+            }
+            if (ccls >= 0 && v[t].x == ccls) sum += carry;
+            const int nc = __shfl_down(v[t].x, 1);
+            if (v[t].x >= 0 && lane != 63 && nc != v[t].x) atomicAdd(&nlen[v[t].x], sum);
+            ccls = __shfl(v[t].x, 63);
+            carry = __shfl(sum, 63);
+        }
+        if (ccls >= 0 && lane == 0) atomicAdd(&nlen[ccls], carry);
     }
 }
 
@@ -625,8 +655,27 @@ hipError_t te_accum(const DevGraph& g, KeySpace ks, const int* p_of, const uint3
     const long long np = g.nruns;
     const size_t lds = (size_t)NB * 12;
     const long long nchunks = (np + TE_CHUNK - 1) / TE_CHUNK;
-    const int grid = (int)(nchunks < 8192 ? (nchunks < 1 ? 1 : nchunks) : 8192);
     const int nt = (NB + 63) / 64;
+    // as many workgroups (one wave each) as the device holds at once, at most 8192
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) !=
+        hipSuccess)
+        return e;
+#define GS_TE_OCC(T) \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_te_accum<T>, 64, lds)
+    if (nt <= 1) GS_TE_OCC(1);
+    else if (nt <= 2) GS_TE_OCC(2);
+    else if (nt <= 4) GS_TE_OCC(4);
+    else if (nt <= 8) GS_TE_OCC(8);
+    else if (nt <= 16) GS_TE_OCC(16);
+    else return hipErrorInvalidValue;
+#undef GS_TE_OCC
+    if (e != hipSuccess) return e;
+    long long cap = (long long)cus * per;
+    cap = cap < 1 ? 1 : (cap > 8192 ? 8192 : cap);
+    const int grid = (int)(nchunks < cap ? (nchunks < 1 ? 1 : nchunks) : cap);
 #define GS_TE_ACCUM(T)                                                                        \
     hipLaunchKernelGGL(k_te_accum<T>, dim3(grid), dim3(64), lds, s, g, ks.EL, ks.VL, NB, W, WP, \
                        m1, m2, rec, rbl, np, p_of, pmask, queue, CNT, CEN, END, work)
