@@ -343,7 +343,9 @@ def main() -> int:
                      "train_samples": samples, "train_loss": loss, "train_s": train_s}
         if k <= 2:
             predicted["note"] = ("learned sizes apply to levels below the last; k = 2 has none, "
-                                 "so this run is the batched method")
+                                 "so this run is the batched method, timed right after the "
+                                 "predictor's training run (a batched miner made at that point "
+                                 "runs as slowly)")
         log(f"predicted: {predicted['ms_per_step']:.2f} ms/step")
 
     cpu = None
