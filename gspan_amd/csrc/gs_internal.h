@@ -187,6 +187,23 @@ struct KeySpace {
 // child's vertices, lw[l] words for label rank l, lcum[l] = lw[0] + ... + lw[l-1].  The
 // pattern's keys start at P.set_base: its backward keys (the parent's vertex sets), then
 // per (segment, edge label) one key per label rank (the parent's sets plus the new vertex's).
+// Batch index of the single-edge parent owning child key `key`: every single-edge pattern
+// has the same 2 * EL * VL child keys (no backward ones), laid out in batch order, so the
+// index is a division, checked against the pattern's key_base (otherwise searched).
+__host__ __device__ inline int owner_one_edge(const PatDev* pats, int npats, KeySpace ks,
+                                              int key) {
+    const int kp = 2 * ks.EL * ks.VL;
+    const int p = key / kp;
+    if (p < npats && pats[p].key_base <= key && key - pats[p].key_base < kp) return p;
+    int lo = 0, hi = npats - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pats[mid].key_base <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __host__ __device__ inline void all_key_set(const PatDev& P, int slot, KeySpace ks, const int* lw,
                                             const int* lcum, int kind, int tag, SetDesc* d) {
     const int nb = (P.r - 1) * ks.EL;

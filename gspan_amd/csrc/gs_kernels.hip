@@ -2053,13 +2053,7 @@ __global__ __launch_bounds__(GS_BLOCK) void k_canon_two(const PatDev* __restrict
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
         const int key = active_keys[a];
-        int lo = 0, hi = npats - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pats[mid].key_base <= key) lo = mid;
-            else hi = mid - 1;
-        }
-        const PatDev& P = pats[lo];
+        const PatDev& P = pats[owner_one_edge(pats, npats, ks, key)];
         const int slot = key - P.key_base;
         CanonCode C;
         C.m = 2;

@@ -452,12 +452,7 @@ __global__ void k_te_keys(const PatDev* __restrict__ pats, int npats, KeySpace k
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long key = (long long)blockIdx.x * blockDim.x + threadIdx.x; key < K;
          key += stride) {
-        int lo = 0, hi = npats - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pats[mid].key_base <= key) lo = mid;
-            else hi = mid - 1;
-        }
+        const int lo = owner_one_edge(pats, npats, ks, (int)key);
         const PatDev& P = pats[lo];
         const int slot = (int)(key - P.key_base);
         const int j = slot / (ks.EL * ks.VL), e = (slot / ks.VL) % ks.EL, l = slot % ks.VL;
@@ -527,14 +522,7 @@ __global__ void k_te_emit(const PatDev* __restrict__ pats, int npats, KeySpace k
             p = (int)t;
         } else {
             i = (int)(t - npats);
-            const int key = fkey[i];
-            int lo = 0, hi = npats - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (pats[mid].key_base <= key) lo = mid;
-                else hi = mid - 1;
-            }
-            p = lo;
+            p = owner_one_edge(pats, npats, ks, fkey[i]);
         }
         const PatDev& P = pats[p];
         const long long cp = te_lower_bound(fkey, F, P.key_base);
