@@ -318,8 +318,11 @@ struct gs_ctx {
     // order (flat): codes, edge counts and supports land in pinned host buffers.
     bool flat = false, flat_ok = false;
     int flat_seeds = 0;
-    int *flat_codes = nullptr, *flat_lens = nullptr, *flat_sups = nullptr;
-    long long flat_cap_e = 0, flat_cap_l = 0, flat_cap_s = 0, flat_reports = 0, flat_edges = 0;
+    // one pinned buffer: codes (5 per edge), then edge counts, then supports
+    int* flat_buf = nullptr;
+    long long flat_cap = 0;
+    int *flat_codes = nullptr, *flat_lens = nullptr, *flat_sups = nullptr;  // into flat_buf
+    long long flat_reports = 0, flat_edges = 0;
     // The two-edge pass over every frequent single-edge code, launched from the class sums
     // before the host has listed the codes (te_prelaunch): its tables, counters, the codes'
     // count on the device and the slot of its timing event (-1: untimed).  The step over
@@ -1135,31 +1138,28 @@ void emit_flat(gs_ctx* c, const PatDev* d_pats, const std::vector<HPat>& pats, K
     DevPool* P = &c->pool;
     const int B = (int)pats.size();
     const long long R = B + (long long)F, E = B + 2LL * F;
-    auto grow = [](int*& buf, long long& cap, long long need) {
-        if (cap >= need) return;
-        if (buf) HIPCHECK(hipHostFree(buf));
-        buf = nullptr;
-        const long long n = std::max<long long>(need + need / 2, 1024);
-        HIPCHECK(hipHostMalloc((void**)&buf, sizeof(int) * n, hipHostMallocDefault));
-        cap = n;
-    };
-    grow(c->flat_codes, c->flat_cap_e, 5 * E);
-    grow(c->flat_lens, c->flat_cap_l, R);
-    grow(c->flat_sups, c->flat_cap_s, R);
+    const long long words = 5 * E + 2 * R;  // codes, edge counts, supports: one copy
+    if (c->flat_cap < words) {
+        if (c->flat_buf) HIPCHECK(hipHostFree(c->flat_buf));
+        c->flat_buf = nullptr;
+        c->flat_cap = 0;
+        const long long n = std::max<long long>(words + words / 2, 1024);
+        HIPCHECK(hipHostMalloc((void**)&c->flat_buf, sizeof(int) * n, hipHostMallocDefault));
+        c->flat_cap = n;
+    }
+    c->flat_codes = c->flat_buf;
+    c->flat_lens = c->flat_buf + 5 * E;
+    c->flat_sups = c->flat_buf + 5 * E + R;
     std::vector<int> ssup(B);
     for (int i = 0; i < B; ++i) ssup[i] = pats[i].support;
-    DBuf d_ssup(P, sizeof(int) * std::max(B, 1)), d_codes(P, sizeof(int) * 5 * std::max(E, 1LL));
-    DBuf d_lens(P, sizeof(int) * std::max(R, 1LL)), d_sups(P, sizeof(int) * std::max(R, 1LL));
+    DBuf d_ssup(P, sizeof(int) * std::max(B, 1)), d_out(P, sizeof(int) * std::max(words, 1LL));
+    int* d_codes = d_out.as<int>();
     h2d(c, d_ssup.as<int>(), ssup.data(), B);
     HIPCHECK(gsk::te_emit(d_pats, B, ks, d_fkey, d_fsup, F, d_ssup.as<int>(),
-                          c->d_vl_vals.as<int>(), c->d_el_vals.as<int>(), d_codes.as<int>(),
-                          d_lens.as<int>(), d_sups.as<int>(), c->stream));
-    HIPCHECK(hipMemcpyAsync(c->flat_codes, d_codes.as<int>(), sizeof(int) * 5 * E,
-                            hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipMemcpyAsync(c->flat_lens, d_lens.as<int>(), sizeof(int) * R,
-                            hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipMemcpyAsync(c->flat_sups, d_sups.as<int>(), sizeof(int) * R,
-                            hipMemcpyDeviceToHost, c->stream));
+                          c->d_vl_vals.as<int>(), c->d_el_vals.as<int>(), d_codes,
+                          d_codes + 5 * E, d_codes + 5 * E + R, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->flat_buf, d_codes, sizeof(int) * words, hipMemcpyDeviceToHost,
+                            c->stream));
     sync(c);
     c->flat = true;
     c->flat_reports = R;
@@ -3145,9 +3145,7 @@ int gs_destroy(gs_ctx* c) {
     c->d_te_rbl.reset();
     c->d_vl_vals.reset();
     c->d_el_vals.reset();
-    if (c->flat_codes) (void)hipHostFree(c->flat_codes);
-    if (c->flat_lens) (void)hipHostFree(c->flat_lens);
-    if (c->flat_sups) (void)hipHostFree(c->flat_sups);
+    if (c->flat_buf) (void)hipHostFree(c->flat_buf);
     c->te_pre = gs_ctx::TePre{};
     if (c->h_te_np) (void)hipHostFree(c->h_te_np);
     if (c->h_te_nl) (void)hipHostFree(c->h_te_nl);
