@@ -420,10 +420,11 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
                       const long long* set_off, const long long* row_off,
                       const SetDesc* key_sets, SetDesc* sets, int* key_slot,
                       unsigned long long* cursor, hipStream_t s);
+// n_active (not null): the active keys are the first *n_active of na; the rest get sup 0
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
                     const int* acnt, const int* flags, int na, const int* counts,
                     const int* ksup, int mni_mode, int min_sup, int* sup,
-                    unsigned long long* totals, hipStream_t s);
+                    unsigned long long* totals, hipStream_t s, const int* n_active = nullptr);
 // Sets of every potential child key of a batch (single-pass last level): each key's vertex
 // sets are bitmaps or bytemaps over their label classes, lw[l] words for label rank l and
 // lcum[l] = lw[0] + ... + lw[l-1] (VL + 1 entries); a pattern's keys start at its set_base:

@@ -2315,11 +2315,16 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
                            const int* __restrict__ akeys, const int* __restrict__ acnt,
                            const int* __restrict__ flags, int na, const int* __restrict__ counts,
                            const int* __restrict__ ksup, int mni_mode, int min_sup, int* sup,
-                           unsigned long long* totals) {
+                           unsigned long long* totals, const int* __restrict__ n_active) {
     __shared__ unsigned long long s_t[2][GS_BLOCK / 64];
     unsigned long long rows = 0, canon = 0;
+    const long long nv = n_active ? *n_active : na;  // active keys among the na
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
+        if (a >= nv) {
+            sup[a] = 0;
+            continue;
+        }
         const long long cr = child_rows(acnt[a]);
         rows += (unsigned long long)cr;
         int out = 0;
@@ -2709,12 +2714,12 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
                     const int* acnt, const int* flags, int na, const int* counts,
                     const int* ksup, int mni_mode, int min_sup, int* sup,
-                    unsigned long long* totals, hipStream_t s) {
+                    unsigned long long* totals, hipStream_t s, const int* n_active) {
     if (na <= 0) return hipSuccess;
     // every block adds its totals with two atomics on the same two words: few blocks
     hipLaunchKernelGGL(k_supports, dim3(grid_for(na, GS_BLOCK, 512)), dim3(GS_BLOCK), 0, s,
                        pats, npats, ks, akeys, acnt, flags, na, counts, ksup, mni_mode, min_sup,
-                       sup, totals);
+                       sup, totals, n_active);
     return hipGetLastError();
 }
 

@@ -1686,18 +1686,6 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
                         m + 1, d_flags.as<int>(), c->err_flag(), c->stream));
     int na_err[2] = {0, 0};
     d2h(c, &W_full, c->work_full64(), 1);
-    ActiveHost ah;
-    fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(), na_err, 2, ah,
-                 all_live ? LLONG_MAX : kDeviceTailKeys);
-    resolve_timing(c);
-    for (long long w : Wc_host) W_total += w;
-    c->st.items += W_total;
-    if (na_err[1] & 1)
-        throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
-    if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
-    if (pre_codes >= 0 && pre_codes != B)
-        throw GsError(GS_ERR_INTERNAL, "single-edge codes of the device and the host differ");
-    const int A = na_err[0];
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
     auto account_bytes = [&](long long emitted) {
@@ -1721,6 +1709,59 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         c->st.alg_bytes_fill += bytes + 8.0 * (double)emitted;
     };
 
+    // A k = 2 run's step over every frequent single-edge code: supports, the frequent
+    // children and their reports follow on the device without a synchronisation for the
+    // active count (buffers sized by the key space; the count and the canonicality errors
+    // are read with the frequent count).
+    if (two_edge && c->flat_ok && B == c->flat_seeds) {
+        d2h(c, na_err, c->n_active(), 2);
+        DBuf d_sup(P, sizeof(int) * K), d_fidx(P, sizeof(int) * K), d_fsup(P, sizeof(int) * K);
+        DBuf d_fkey(P, sizeof(int) * K), d_fcnt(P, sizeof(int) * K);
+        DBuf d_frow(P, sizeof(long long) * K);
+        HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
+        HIPCHECK(gsk::supports(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), d_acnt.as<int>(),
+                               d_flags.as<int>(), (int)K, nullptr, d_te_sup.as<int>(),
+                               c->mni_mode, c->min_sup, d_sup.as<int>(), c->step_totals(),
+                               c->stream, c->n_active()));
+        compact_ordered(c, d_sup.as<int>(), K, d_fidx.as<int>(), d_fsup.as<int>());
+        HIPCHECK(gsk::gather_frequent(d_fidx.as<int>(), c->n_active(), (int)K,
+                                      d_active.as<int>(), d_acnt.as<int>(), nullptr,
+                                      d_fkey.as<int>(), d_fcnt.as<int>(), d_frow.as<long long>(),
+                                      c->stream));
+        int F = 0;
+        unsigned long long totals[2] = {0, 0};
+        d2h(c, &F, c->n_active(), 1);
+        d2h(c, totals, c->step_totals(), 2);
+        sync(c);
+        resolve_timing(c);
+        for (long long w : Wc_host) W_total += w;
+        c->st.items += W_total;
+        if (na_err[1] & 1) throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
+        if (na_err[1] & 2)
+            throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
+        if (pre_codes >= 0 && pre_codes != B)
+            throw GsError(GS_ERR_INTERNAL, "single-edge codes of the device and the host differ");
+        emit_flat(c, d_pats.as<PatDev>(), pats, ks, d_fkey.as<int>(), d_fsup.as<int>(), F);
+        out.candidates = na_err[0];
+        out.canonical = (long long)totals[1];
+        out.emitted = (long long)totals[0];
+        out.reported_inline = true;
+        account_bytes(out.emitted);
+        c->st.batches++;
+        return STEP_OK;
+    }
+    ActiveHost ah;
+    fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(), na_err, 2, ah,
+                 all_live ? LLONG_MAX : kDeviceTailKeys);
+    resolve_timing(c);
+    for (long long w : Wc_host) W_total += w;
+    c->st.items += W_total;
+    if (na_err[1] & 1)
+        throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
+    if (na_err[1] & 2) throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
+    if (pre_codes >= 0 && pre_codes != B)
+        throw GsError(GS_ERR_INTERNAL, "single-edge codes of the device and the host differ");
+    const int A = na_err[0];
     // ---- many candidate keys: plan, fill, supports and frequent children on the device
     if (!all_live && A > kDeviceTailKeys) {
         const auto t_plan0 = std::chrono::steady_clock::now();
@@ -1841,16 +1882,6 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         d2h(c, totals, c->step_totals(), 2);
         sync(c);
         resolve_timing(c);
-        if (two_edge && c->flat_ok && B == c->flat_seeds) {
-            emit_flat(c, d_pats.as<PatDev>(), pats, ks, d_fkey.as<int>(), d_fsup.as<int>(), F);
-            out.candidates = A;
-            out.canonical = (long long)totals[1];
-            out.emitted = (long long)totals[0];
-            out.reported_inline = true;
-            account_bytes(out.emitted);
-            c->st.batches++;
-            return STEP_OK;
-        }
         c->flat_ok = false;  // reports of this run go through the host
         std::vector<int> fkey(F), fcnt(F), fsup(F);
         std::vector<long long> frow(F);
