@@ -420,11 +420,16 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
                       const long long* set_off, const long long* row_off,
                       const SetDesc* key_sets, SetDesc* sets, int* key_slot,
                       unsigned long long* cursor, hipStream_t s);
-// n_active (not null): the active keys are the first *n_active of na; the rest get sup 0
+// A step over single-edge parents with the two-edge pass's key_count and ksup: sel[key] =
+// the support of a canonical child whose count and support reach min_sup, else 0;
+// totals[0] += child rows, totals[1] += canonical keys, totals[2] += keys with children.
+hipError_t two_edge_select(const PatDev* pats, int npats, KeySpace ks, long long K,
+                           const int* key_count, const int* ksup, int min_sup, int* sel,
+                           unsigned long long* totals, hipStream_t s);
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
                     const int* acnt, const int* flags, int na, const int* counts,
                     const int* ksup, int mni_mode, int min_sup, int* sup,
-                    unsigned long long* totals, hipStream_t s, const int* n_active = nullptr);
+                    unsigned long long* totals, hipStream_t s);
 // Sets of every potential child key of a batch (single-pass last level): each key's vertex
 // sets are bitmaps or bytemaps over their label classes, lw[l] words for label rank l and
 // lcum[l] = lw[0] + ... + lw[l-1] (VL + 1 entries); a pattern's keys start at its set_base:

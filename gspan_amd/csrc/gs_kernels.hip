@@ -2045,6 +2045,36 @@ __device__ __forceinline__ int canon_two(const CanonCode& C) {
 
 // Canonicality of the two-edge children of single-edge parents: one thread per active key
 // (the child of slot `slot` as decode_child writes it, for a parent of one edge).
+// The two-edge code of child key `key` of a batch of single-edge parents.
+__device__ __forceinline__ void two_edge_code(const PatDev* __restrict__ pats, int npats,
+                                              KeySpace ks, int key, CanonCode& C) {
+    const PatDev& P = pats[owner_one_edge(pats, npats, ks, key)];
+    const int slot = key - P.key_base;
+    C.m = 2;
+    C.cf[0] = P.cfrom[0];
+    C.ct[0] = P.cto[0];
+    C.cel[0] = P.cel[0];
+    C.ctl[0] = P.vlab[P.cto[0]];
+    C.lab[0] = P.vlab[0];
+    C.lab[1] = P.vlab[1];
+    C.lab[2] = 0;
+    const int nb = (P.r - 1) * ks.EL;
+    if (slot < nb) {  // backward: a second edge between the parent's two vertices
+        C.cf[1] = P.v_n;
+        C.ct[1] = P.cfrom[P.rm[slot / ks.EL + 1]];
+        C.cel[1] = slot % ks.EL;
+        C.ctl[1] = -1;
+    } else {
+        const int sl = slot - nb;
+        const int j = sl / (ks.EL * ks.VL), el = (sl / ks.VL) % ks.EL, tl = sl % ks.VL;
+        C.cf[1] = j == 0 ? P.v_n : P.cfrom[P.rm[j - 1]];
+        C.ct[1] = 2;
+        C.cel[1] = el;
+        C.ctl[1] = tl;
+        C.lab[2] = tl;
+    }
+}
+
 __global__ __launch_bounds__(GS_BLOCK) void k_canon_two(const PatDev* __restrict__ pats,
                                                         int npats, KeySpace ks,
                                                         const int* active_keys,
@@ -2052,34 +2082,62 @@ __global__ __launch_bounds__(GS_BLOCK) void k_canon_two(const PatDev* __restrict
     const long long na = *n_active;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
-        const int key = active_keys[a];
-        const PatDev& P = pats[owner_one_edge(pats, npats, ks, key)];
-        const int slot = key - P.key_base;
         CanonCode C;
-        C.m = 2;
-        C.cf[0] = P.cfrom[0];
-        C.ct[0] = P.cto[0];
-        C.cel[0] = P.cel[0];
-        C.ctl[0] = P.vlab[P.cto[0]];
-        C.lab[0] = P.vlab[0];
-        C.lab[1] = P.vlab[1];
-        C.lab[2] = 0;
-        const int nb = (P.r - 1) * ks.EL;
-        if (slot < nb) {  // backward: a second edge between the parent's two vertices
-            C.cf[1] = P.v_n;
-            C.ct[1] = P.cfrom[P.rm[slot / ks.EL + 1]];
-            C.cel[1] = slot % ks.EL;
-            C.ctl[1] = -1;
-        } else {
-            const int sl = slot - nb;
-            const int j = sl / (ks.EL * ks.VL), el = (sl / ks.VL) % ks.EL, tl = sl % ks.VL;
-            C.cf[1] = j == 0 ? P.v_n : P.cfrom[P.rm[j - 1]];
-            C.ct[1] = 2;
-            C.cel[1] = el;
-            C.ctl[1] = tl;
-            C.lab[2] = tl;
-        }
+        two_edge_code(pats, npats, ks, active_keys[a], C);
         flags[a] = canon_two(C);
+    }
+}
+
+// A step over single-edge parents whose children's counts and supports come from the
+// two-edge pass, in one launch over its K keys: sel[key] = the child's support when the
+// child has rows, its code is canonical and both its child count and its support reach
+// min_sup, else 0; totals[0] += child rows and totals[2] += keys, of the keys with
+// children, totals[1] += the canonical ones among them (what the compaction, canonicality
+// and support launches of other steps add up).
+__global__ __launch_bounds__(GS_BLOCK) void k_two_edge_select(
+    const PatDev* __restrict__ pats, int npats, KeySpace ks, long long K,
+    const int* __restrict__ key_count, const int* __restrict__ ksup, int min_sup, int* sel,
+    unsigned long long* totals) {
+    __shared__ unsigned long long s_t[3][GS_BLOCK / 64];
+    unsigned long long rows = 0, canon = 0, cand = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long key = (long long)blockIdx.x * blockDim.x + threadIdx.x; key < K;
+         key += stride) {
+        const int kc = key_count[key];
+        int out = 0;
+        if (kc != 0) {
+            const long long cr = child_rows(kc);
+            rows += (unsigned long long)cr;
+            ++cand;
+            CanonCode C;
+            two_edge_code(pats, npats, ks, (int)key, C);
+            if (canon_two(C)) {
+                ++canon;
+                if (cr >= (long long)min_sup) {
+                    const int sp = ksup[key];
+                    out = sp >= min_sup ? sp : 0;
+                }
+            }
+        }
+        sel[key] = out;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rows += __shfl_xor(rows, off);
+        canon += __shfl_xor(canon, off);
+        cand += __shfl_xor(cand, off);
+    }
+    if (lane_id() == 0) {
+        s_t[0][threadIdx.x / 64] = rows;
+        s_t[1][threadIdx.x / 64] = canon;
+        s_t[2][threadIdx.x / 64] = cand;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[3] = {0, 0, 0};
+        for (int w = 0; w < GS_BLOCK / 64; ++w)
+            for (int i = 0; i < 3; ++i) t[i] += s_t[i][w];
+        for (int i = 0; i < 3; ++i)
+            if (t[i]) atomicAdd(&totals[i], t[i]);
     }
 }
 
@@ -2315,16 +2373,11 @@ __global__ void k_supports(const PatDev* __restrict__ pats, int npats, KeySpace 
                            const int* __restrict__ akeys, const int* __restrict__ acnt,
                            const int* __restrict__ flags, int na, const int* __restrict__ counts,
                            const int* __restrict__ ksup, int mni_mode, int min_sup, int* sup,
-                           unsigned long long* totals, const int* __restrict__ n_active) {
+                           unsigned long long* totals) {
     __shared__ unsigned long long s_t[2][GS_BLOCK / 64];
     unsigned long long rows = 0, canon = 0;
-    const long long nv = n_active ? *n_active : na;  // active keys among the na
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long a = (long long)blockIdx.x * blockDim.x + threadIdx.x; a < na; a += stride) {
-        if (a >= nv) {
-            sup[a] = 0;
-            continue;
-        }
         const long long cr = child_rows(acnt[a]);
         rows += (unsigned long long)cr;
         int out = 0;
@@ -2714,12 +2767,22 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
                     const int* acnt, const int* flags, int na, const int* counts,
                     const int* ksup, int mni_mode, int min_sup, int* sup,
-                    unsigned long long* totals, hipStream_t s, const int* n_active) {
+                    unsigned long long* totals, hipStream_t s) {
     if (na <= 0) return hipSuccess;
     // every block adds its totals with two atomics on the same two words: few blocks
     hipLaunchKernelGGL(k_supports, dim3(grid_for(na, GS_BLOCK, 512)), dim3(GS_BLOCK), 0, s,
                        pats, npats, ks, akeys, acnt, flags, na, counts, ksup, mni_mode, min_sup,
-                       sup, totals, n_active);
+                       sup, totals);
+    return hipGetLastError();
+}
+
+hipError_t two_edge_select(const PatDev* pats, int npats, KeySpace ks, long long K,
+                           const int* key_count, const int* ksup, int min_sup, int* sel,
+                           unsigned long long* totals, hipStream_t s) {
+    if (K <= 0) return hipSuccess;
+    // every block adds its totals with three atomics on the same words: few blocks
+    hipLaunchKernelGGL(k_two_edge_select, dim3(grid_for(K, GS_BLOCK, 512)), dim3(GS_BLOCK), 0,
+                       s, pats, npats, ks, K, key_count, ksup, min_sup, sel, totals);
     return hipGetLastError();
 }
 

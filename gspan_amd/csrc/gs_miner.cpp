@@ -1679,13 +1679,6 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         c->n_logged += take;
     }
 
-    // ---- candidates and their canonicality
-    DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
-    compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
-    HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
-                        m + 1, d_flags.as<int>(), c->err_flag(), c->stream));
-    int na_err[2] = {0, 0};
-    d2h(c, &W_full, c->work_full64(), 1);
     // algorithmic bytes of the fill launches: per parent row 20*m (layer records + Edge
     // gathers) + sum over scanned vertices of (8 + 16*deg) + 8 per emitted child row
     auto account_bytes = [&](long long emitted) {
@@ -1709,40 +1702,29 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         c->st.alg_bytes_fill += bytes + 8.0 * (double)emitted;
     };
 
-    // A k = 2 run's step over every frequent single-edge code: supports, the frequent
-    // children and their reports follow on the device without a synchronisation for the
-    // active count (buffers sized by the key space; the count and the canonicality errors
-    // are read with the frequent count).
+    // A k = 2 run's step over every frequent single-edge code: one launch takes each key's
+    // canonicality and its frequent support from the two-edge pass's counts and supports,
+    // one compaction lists the frequent children, and their reports are built on the device
+    // (no host round trip before the frequent count).
     if (two_edge && c->flat_ok && B == c->flat_seeds) {
-        d2h(c, na_err, c->n_active(), 2);
-        DBuf d_sup(P, sizeof(int) * K), d_fidx(P, sizeof(int) * K), d_fsup(P, sizeof(int) * K);
-        DBuf d_fkey(P, sizeof(int) * K), d_fcnt(P, sizeof(int) * K);
-        DBuf d_frow(P, sizeof(long long) * K);
-        HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 4, c->stream));
-        HIPCHECK(gsk::supports(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), d_acnt.as<int>(),
-                               d_flags.as<int>(), (int)K, nullptr, d_te_sup.as<int>(),
-                               c->mni_mode, c->min_sup, d_sup.as<int>(), c->step_totals(),
-                               c->stream, c->n_active()));
-        compact_ordered(c, d_sup.as<int>(), K, d_fidx.as<int>(), d_fsup.as<int>());
-        HIPCHECK(gsk::gather_frequent(d_fidx.as<int>(), c->n_active(), (int)K,
-                                      d_active.as<int>(), d_acnt.as<int>(), nullptr,
-                                      d_fkey.as<int>(), d_fcnt.as<int>(), d_frow.as<long long>(),
-                                      c->stream));
+        DBuf d_sel(P, sizeof(int) * K), d_fkey(P, sizeof(int) * K), d_fsup(P, sizeof(int) * K);
+        HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 6, c->stream));
+        HIPCHECK(gsk::two_edge_select(d_pats.as<PatDev>(), B, ks, K, d_kc.as<int>(),
+                                      d_te_sup.as<int>(), c->min_sup, d_sel.as<int>(),
+                                      c->step_totals(), c->stream));
+        compact_ordered(c, d_sel.as<int>(), K, d_fkey.as<int>(), d_fsup.as<int>());
         int F = 0;
-        unsigned long long totals[2] = {0, 0};
+        unsigned long long totals[3] = {0, 0, 0};
         d2h(c, &F, c->n_active(), 1);
-        d2h(c, totals, c->step_totals(), 2);
+        d2h(c, totals, c->step_totals(), 3);
         sync(c);
         resolve_timing(c);
         for (long long w : Wc_host) W_total += w;
         c->st.items += W_total;
-        if (na_err[1] & 1) throw GsError(GS_ERR_LIMIT, kCanonCapMsg);
-        if (na_err[1] & 2)
-            throw GsError(GS_ERR_INTERNAL, "canonicality walk ran past the code");
         if (pre_codes >= 0 && pre_codes != B)
             throw GsError(GS_ERR_INTERNAL, "single-edge codes of the device and the host differ");
         emit_flat(c, d_pats.as<PatDev>(), pats, ks, d_fkey.as<int>(), d_fsup.as<int>(), F);
-        out.candidates = na_err[0];
+        out.candidates = (long long)totals[2];
         out.canonical = (long long)totals[1];
         out.emitted = (long long)totals[0];
         out.reported_inline = true;
@@ -1750,6 +1732,14 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
         c->st.batches++;
         return STEP_OK;
     }
+
+    // ---- candidates and their canonicality
+    DBuf d_active(P, sizeof(int) * K), d_acnt(P, sizeof(int) * K), d_flags(P, sizeof(int) * K);
+    compact_ordered(c, d_kc.as<int>(), K, d_active.as<int>(), d_acnt.as<int>());
+    HIPCHECK(gsk::canon(d_pats.as<PatDev>(), B, ks, d_active.as<int>(), c->n_active(), (int)K,
+                        m + 1, d_flags.as<int>(), c->err_flag(), c->stream));
+    int na_err[2] = {0, 0};
+    d2h(c, &W_full, c->work_full64(), 1);
     ActiveHost ah;
     fetch_active(c, K, d_active.as<int>(), d_acnt.as<int>(), d_flags.as<int>(), na_err, 2, ah,
                  all_live ? LLONG_MAX : kDeviceTailKeys);
