@@ -421,11 +421,13 @@ hipError_t build_sets(const PatDev* pats, int npats, KeySpace ks, const int* ake
                       const SetDesc* key_sets, SetDesc* sets, int* key_slot,
                       unsigned long long* cursor, hipStream_t s);
 // A step over single-edge parents with the two-edge pass's key_count and ksup: sel[key] =
-// the support of a canonical child whose count and support reach min_sup, else 0;
-// totals[0] += child rows, totals[1] += canonical keys, totals[2] += keys with children.
+// the support of a canonical child whose count and support reach min_sup, else 0; part:
+// per workgroup (two_edge_select_blocks(K) of them) its child rows, keys with children and
+// canonical keys among them, in that order.
+int two_edge_select_blocks(long long K);
 hipError_t two_edge_select(const PatDev* pats, int npats, KeySpace ks, long long K,
                            const int* key_count, const int* ksup, int min_sup, int* sel,
-                           unsigned long long* totals, hipStream_t s);
+                           unsigned long long* part, hipStream_t s);
 hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys,
                     const int* acnt, const int* flags, int na, const int* counts,
                     const int* ksup, int mni_mode, int min_sup, int* sup,

@@ -2091,13 +2091,14 @@ __global__ __launch_bounds__(GS_BLOCK) void k_canon_two(const PatDev* __restrict
 // A step over single-edge parents whose children's counts and supports come from the
 // two-edge pass, in one launch over its K keys: sel[key] = the child's support when the
 // child has rows, its code is canonical and both its child count and its support reach
-// min_sup, else 0; totals[0] += child rows and totals[2] += keys, of the keys with
-// children, totals[1] += the canonical ones among them (what the compaction, canonicality
-// and support launches of other steps add up).
+// min_sup, else 0.  Workgroup b writes part[3 b .. 3 b + 2] = the child rows and the keys
+// with children, and the canonical ones among them (what the compaction, canonicality and
+// support launches of other steps add up); the host adds the parts (atomics on three
+// words from every workgroup would queue on them).
 __global__ __launch_bounds__(GS_BLOCK) void k_two_edge_select(
     const PatDev* __restrict__ pats, int npats, KeySpace ks, long long K,
     const int* __restrict__ key_count, const int* __restrict__ ksup, int min_sup, int* sel,
-    unsigned long long* totals) {
+    unsigned long long* __restrict__ part) {
     __shared__ unsigned long long s_t[3][GS_BLOCK / 64];
     unsigned long long rows = 0, canon = 0, cand = 0;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -2136,8 +2137,9 @@ __global__ __launch_bounds__(GS_BLOCK) void k_two_edge_select(
         unsigned long long t[3] = {0, 0, 0};
         for (int w = 0; w < GS_BLOCK / 64; ++w)
             for (int i = 0; i < 3; ++i) t[i] += s_t[i][w];
-        for (int i = 0; i < 3; ++i)
-            if (t[i]) atomicAdd(&totals[i], t[i]);
+        part[3 * blockIdx.x] = t[0];
+        part[3 * blockIdx.x + 1] = t[2];
+        part[3 * blockIdx.x + 2] = t[1];
     }
 }
 
@@ -2776,13 +2778,14 @@ hipError_t supports(const PatDev* pats, int npats, KeySpace ks, const int* akeys
     return hipGetLastError();
 }
 
+int two_edge_select_blocks(long long K) { return K <= 0 ? 0 : grid_for(K, GS_BLOCK, 2048); }
+
 hipError_t two_edge_select(const PatDev* pats, int npats, KeySpace ks, long long K,
                            const int* key_count, const int* ksup, int min_sup, int* sel,
-                           unsigned long long* totals, hipStream_t s) {
+                           unsigned long long* part, hipStream_t s) {
     if (K <= 0) return hipSuccess;
-    // every block adds its totals with three atomics on the same words: few blocks
-    hipLaunchKernelGGL(k_two_edge_select, dim3(grid_for(K, GS_BLOCK, 512)), dim3(GS_BLOCK), 0,
-                       s, pats, npats, ks, K, key_count, ksup, min_sup, sel, totals);
+    hipLaunchKernelGGL(k_two_edge_select, dim3(two_edge_select_blocks(K)), dim3(GS_BLOCK), 0, s,
+                       pats, npats, ks, K, key_count, ksup, min_sup, sel, part);
     return hipGetLastError();
 }
 

@@ -1708,24 +1708,28 @@ int run_step(gs_ctx* c, const std::vector<HPat>& pats, const std::vector<Layer>&
     // (no host round trip before the frequent count).
     if (two_edge && c->flat_ok && B == c->flat_seeds) {
         DBuf d_sel(P, sizeof(int) * K), d_fkey(P, sizeof(int) * K), d_fsup(P, sizeof(int) * K);
-        HIPCHECK(gsk::fill_i32(reinterpret_cast<int*>(c->step_totals()), 0, 6, c->stream));
+        const int nb = gsk::two_edge_select_blocks(K);
+        DBuf d_part(P, sizeof(unsigned long long) * 3 * std::max(nb, 1));
         HIPCHECK(gsk::two_edge_select(d_pats.as<PatDev>(), B, ks, K, d_kc.as<int>(),
                                       d_te_sup.as<int>(), c->min_sup, d_sel.as<int>(),
-                                      c->step_totals(), c->stream));
+                                      d_part.as<unsigned long long>(), c->stream));
         compact_ordered(c, d_sel.as<int>(), K, d_fkey.as<int>(), d_fsup.as<int>());
         int F = 0;
-        unsigned long long totals[3] = {0, 0, 0};
+        std::vector<unsigned long long> part(3 * (size_t)nb);
         d2h(c, &F, c->n_active(), 1);
-        d2h(c, totals, c->step_totals(), 3);
+        d2h(c, part.data(), d_part.as<unsigned long long>(), part.size());
         sync(c);
+        unsigned long long totals[3] = {0, 0, 0};  // child rows, keys with children, canonical
+        for (int b = 0; b < nb; ++b)
+            for (int i = 0; i < 3; ++i) totals[i] += part[3 * (size_t)b + i];
         resolve_timing(c);
         for (long long w : Wc_host) W_total += w;
         c->st.items += W_total;
         if (pre_codes >= 0 && pre_codes != B)
             throw GsError(GS_ERR_INTERNAL, "single-edge codes of the device and the host differ");
         emit_flat(c, d_pats.as<PatDev>(), pats, ks, d_fkey.as<int>(), d_fsup.as<int>(), F);
-        out.candidates = (long long)totals[2];
-        out.canonical = (long long)totals[1];
+        out.candidates = (long long)totals[1];
+        out.canonical = (long long)totals[2];
         out.emitted = (long long)totals[0];
         out.reported_inline = true;
         account_bytes(out.emitted);
